@@ -1,0 +1,1272 @@
+// rc_oracle.hpp -- CPU oracle for the lowest-eigenpair Rayleigh-Chebyshev solve.
+//
+// TEST INFRASTRUCTURE ONLY.  This header restates, on the CPU in FP64 (real
+// and complex-Hermitian), the algorithm of the reference's RayleighChebyshevLM
+// driver so that the HIP product can be checked against it.  Only tests/,
+// the smoke check and the CPU-baseline leg of bench.py load it (through
+// liborc.so); the product library never links it and never falls back to it.
+//
+// Reference files followed (file:line in almaeder/RayleighChebyshevGPGPU):
+//   LanczosCpoly.h:203-305, 356-555           filter-polynomial scalars
+//   LanczosCpolyOperatorLM.h:192-304            block filter apply
+//   LanczosMaxMinFinder.h:248-544, 611-723      Lanczos spectral bounds
+//   Dstebz_C.h:112-995 (DSTEBZ, RANGE='I')      restated as Sturm bisection
+//   JacobiDiagonalizer.h:114-378, 486-540       small symmetric eigensolver
+//   RayleighChebyshevLM.h:271-278, 478-806, 817-2112   driver
+//   ArandomizeOpClassLMCuda.h:58-91             randomize() consumption order
+//
+// Parity pins (what checks each part; see tests/test_oracle_*.py):
+//   Cpoly            closed forms sin((m+1)t)/((m+1)sin t), next-to-last root,
+//                    P0/P1/P2, inverse round trips; degree/bound sequences vs
+//                    the compiled reference (golden traces).
+//   filter_apply     eigenvector identity P_m(x(lambda)) v on 1-D Dirichlet
+//                    Laplacian eigenvectors; dense brute force for small N.
+//   lanczos bounds   analytic Laplacian extremes; iteration counts vs the
+//                    compiled reference.
+//   tri_eig          numpy eigvalsh of random tridiagonals.
+//   jacobi_eig       numpy eigh of random symmetric matrices.
+//   hermitian_jacobi numpy eigh of random Hermitian matrices (the reference's
+//                    complex path needs ZHPEVX from an absent library, so the
+//                    complex small eigensolver here is pinned by numpy only).
+//   driver           analytic spectra (2-D/3-D Laplacians) and the compiled
+//                    reference's eigenvalues, iteration counts and countData.
+//
+// Deliberate deviations from the reference (each is a reference defect):
+//   * formVtAV copies every input vector before the apply (the reference
+//     copies vArray.size() of them, RayleighChebyshevLM.h:1932, which reads
+//     stale or out-of-range vectors in the non-monotone repair).
+//   * the non-monotone repair also stores the recomputed Ritz values into
+//     eigValues (RayleighChebyshevLM.h:1694 leaves them stale).
+//   * minEigValue is only updated from eigValues when something was found
+//     (RayleighChebyshevLM.h:1593 indexes eigValues[-1] otherwise).
+//   * Jacobi sweeps once with a zero threshold when no off-diagonal entry
+//     exceeds the RMS threshold (JacobiDiagonalizer.h:327,357 return the 2x2
+//     case, and any equal-magnitude case, unrotated).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+using cplx = std::complex<double>;
+
+// ------------------------------------------------------------------ RNG
+// Counter-based SplitMix64 stream: draw number idx of the stream keyed by
+// `seed` is mix64(key + (idx+1)*gamma) with key = mix64(seed + gamma), mapped
+// to [0,1) by the top 53 bits.  Any draw is computable independently, which
+// lets the GPU fill start vectors in place with bit-identical values.
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ULL;
+inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+inline double unit_draw(uint64_t seed, uint64_t idx) {
+  const uint64_t key = mix64(seed + kGamma);
+  const uint64_t z = mix64(key + (idx + 1) * kGamma);
+  return double(z >> 11) * 0x1.0p-53;
+}
+// U[-1,1): -1 + 2u is exact for u = m*2^-53.
+inline double sym_draw(uint64_t seed, uint64_t idx) { return -1.0 + 2.0 * unit_draw(seed, idx); }
+
+// randomize(V) semantics of ArandomizeOpClassLMCuda.h:58-72: every element
+// consumes one draw (complex: two consecutive draws, re then im), vectors are
+// filled in the order randomize() is called.
+struct RandomStream {
+  uint64_t seed = 0;
+  uint64_t counter = 0;
+  void randomize(std::vector<double>& v) {
+    for (size_t i = 0; i < v.size(); ++i) v[i] = sym_draw(seed, counter + i);
+    counter += v.size();
+  }
+  void randomize(std::vector<cplx>& v) {
+    for (size_t i = 0; i < v.size(); ++i)
+      v[i] = cplx(sym_draw(seed, counter + 2 * i), sym_draw(seed, counter + 2 * i + 1));
+    counter += 2 * v.size();
+  }
+};
+
+// ------------------------------------------------------- vector helpers
+inline double conjv(double x) { return x; }
+inline cplx conjv(cplx x) { return std::conj(x); }
+inline double realv(double x) { return x; }
+inline double realv(cplx x) { return x.real(); }
+inline double imagv(double) { return 0.0; }
+inline double imagv(cplx x) { return x.imag(); }
+
+// a.dot(b) = sum_i a_i * conj(b_i), summed sequentially (AvectorClassLMCuda.h:85-99).
+template <class T>
+T dot(const std::vector<T>& a, const std::vector<T>& b) {
+  T s = T(0.0);
+  const size_t n = a.size();
+  for (size_t i = 0; i < n; ++i) s += a[i] * conjv(b[i]);
+  return s;
+}
+template <class T>
+double norm2(const std::vector<T>& a) { return std::sqrt(std::abs(dot(a, a))); }
+template <class T>
+void scal(std::vector<T>& a, T s) { for (auto& x : a) x *= s; }
+// a += s*b with the product rounded before the sum (reference: tmp = b; tmp *= s; a += tmp).
+template <class T>
+void axpy(std::vector<T>& a, T s, const std::vector<T>& b) {
+  const size_t n = a.size();
+  for (size_t i = 0; i < n; ++i) a[i] += b[i] * s;
+}
+
+template <class T>
+using Block = std::vector<std::vector<T>>;
+
+// ------------------------------------------------------------- operators
+// CSR operator, int32 indices.  y_i = sum_j a_ij x_j accumulated from zero in
+// stored order.  Block apply is row-parallel so the CSR arrays stream once.
+template <class T>
+struct CsrOp {
+  long n = 0;
+  const int* rowptr = nullptr;
+  const int* col = nullptr;
+  const T* val = nullptr;
+
+  void apply_vec(const T* x, T* y) const {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      T s = T(0.0);
+      for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) s += val[p] * x[col[p]];
+      y[i] = s;
+    }
+  }
+  void apply(std::vector<T>& v) const {
+    std::vector<T> y(v.size());
+    apply_vec(v.data(), y.data());
+    v.swap(y);
+  }
+  void apply(Block<T>& V) const {
+    const long b = (long)V.size();
+    if (b == 0) return;
+    Block<T> Y(b, std::vector<T>(n));
+    std::vector<const T*> xs(b);
+    std::vector<T*> ys(b);
+    for (long k = 0; k < b; ++k) { xs[k] = V[k].data(); ys[k] = Y[k].data(); }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      for (long k = 0; k < b; ++k) {
+        const T* x = xs[k];
+        T s = T(0.0);
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) s += val[p] * x[col[p]];
+        ys[k][i] = s;
+      }
+    }
+    V.swap(Y);
+  }
+};
+
+// Dense column-major operator (n x n).
+template <class T>
+struct DenseOp {
+  long n = 0;
+  const T* a = nullptr;  // a[i + j*n]
+  void apply_vec(const T* x, T* y) const {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      T s = T(0.0);
+      for (long j = 0; j < n; ++j) s += a[i + j * n] * x[j];
+      y[i] = s;
+    }
+  }
+  void apply(std::vector<T>& v) const {
+    std::vector<T> y(v.size());
+    apply_vec(v.data(), y.data());
+    v.swap(y);
+  }
+  void apply(Block<T>& V) const {
+    for (auto& v : V) apply(v);
+  }
+};
+
+// ------------------------------------------------------------ Cpoly
+// Scaled Lanczos "C" polynomial scalars, LanczosCpoly.h.  Operation order is
+// kept so that degree/bound choices agree bit for bit with the reference.
+struct Cpoly {
+  long polyDegree = 0;
+  long repetitionFactor = 0;
+  double lambdaMax = 0.0;
+  double shift = 0.0;
+  double XStar = 0.0;
+  double UpperXStar = 1.0;
+
+  void set(long m, long rep, double lmax, double sh) {
+    polyDegree = m;
+    repetitionFactor = rep;
+    lambdaMax = lmax;
+    shift = sh;
+    XStar = getXStar();
+    UpperXStar = getUpperXStar();
+  }
+
+  // b_0 = 1, b_1 = C b_0, b_j = C b_{j-1} - b_{j-2}, C = 2 - 4x; P_m = b_m/(m+1)
+  // (LanczosCpoly.h:234-270).
+  double evaluatePm(double x) const {
+    if (polyDegree == 0) return 1.0;
+    if (polyDegree == 1) return 1.0 - 2.0 * x;
+    double bm2 = 1.0;
+    double bm1 = ((bm2 * x) * -2.0 + bm2) * 2.0;
+    for (long k = 2; k <= polyDegree; ++k) {
+      double t = ((bm1 * x) * -2.0 + bm1) * 2.0;
+      bm2 = -bm2;
+      t += bm2;
+      bm2 = bm1;
+      bm1 = t;
+    }
+    return bm1 * (1.0 / double(polyDegree + 1));
+  }
+  // Derivative of sin((m+1)t)/((m+1) sin t) with sin^2(t/2)=x (LanczosCpoly.h:400-415).
+  double evaluatePmPrime(double x) const {
+    const double m = double(polyDegree);
+    const double a1 = m + 1.0;
+    const double r = std::sqrt(x);
+    const double th = std::asin(r);
+    const double a1th = a1 * th;
+    const double ir = 1 / r;
+    const double q = std::sqrt(1.0 - x);
+    const double iq = 1 / q;
+    const double s2 = std::sin(2.0 * th);
+    const double s2sq = s2 * s2;
+    return std::cos(2.0 * a1th) * ir * iq / s2 -
+           std::sin(2.0 * a1th) / a1 / s2sq * std::cos(2.0 * th) * ir * iq;
+  }
+  // First root of P_m' by the secant method from 4.49^2/(4(m+1)^2) and
+  // 4.49^2/(4 m (m+1)) (LanczosCpoly.h:371-398).
+  double getPmPrimeFirstRoot() const {
+    const double tol = 1.0e-12;
+    const double m = double(polyDegree);
+    double xa = ((4.49 * 4.49) / (4.0 * (m + 1) * (m + 1)));
+    double fa = evaluatePmPrime(xa);
+    double xb = ((4.49 * 4.49) / (4.0 * m * (m + 1)));
+    double fb = evaluatePmPrime(xb);
+    for (int it = 0; std::abs(xa - xb) > tol && it < 10000; ++it) {
+      const double d = (fb - fa) / (xb - xa);
+      const double xc = xb - evaluatePmPrime(xb) / d;
+      xa = xb;
+      fa = fb;
+      xb = xc;
+      fb = evaluatePmPrime(xb);
+    }
+    return xb;
+  }
+  // Second-largest root of P_m: (cos(2 pi/(m+1)) + 1)/2 (LanczosCpoly.h:363-369).
+  double getPmNextToLastRoot() const {
+    const double pi = 3.1415926535897932384;
+    const double m = double(polyDegree);
+    return (std::cos((2.0 / (m + 1.0)) * pi) + 1) * 0.5;
+  }
+  // Inverse of P_m on its monotone main lobe; log-secant for y > 5, constant
+  // extension for y < 0 (LanczosCpoly.h:416-473).
+  double evaluateInversePm(double y) const {
+    const double tol = 1.0e-12;
+    const double m = double(polyDegree);
+    const double piApprox = 3.14159265359;
+    if (y < 0.0) {
+      const double s = std::sin(0.5 * (piApprox / (m + 1)));
+      return s * s;
+    }
+    double xa = 0.0, fa = 0.0, xb = 0.0, fb = 0.0;
+    if (y <= 5.0) {
+      xa = 0.0;
+      fa = 1.0;
+      xb = std::sin(0.5 * (piApprox / (m + 1)));
+      xb *= xb;
+      fb = 0.0;
+      for (int it = 0; std::abs(xa - xb) > tol && it < 10000; ++it) {
+        const double d = (fb - fa) / (xb - xa);
+        const double xc = xb - (evaluatePm(xb) - y) / d;
+        xa = xb;
+        fa = fb;
+        xb = xc;
+        fb = evaluatePm(xb);
+      }
+    }
+    if (y > 5.0) {
+      xa = 0.0;
+      fa = 0.0;
+      xb = -4.0 / ((m + 1.0) * (m + 1.0));
+      fb = std::log(evaluatePm(xb));
+      for (int it = 0; std::abs(xa - xb) > tol && it < 10000; ++it) {
+        const double d = (fb - fa) / (xb - xa);
+        const double xc = xb - (std::log(evaluatePm(xb)) - std::log(y)) / d;
+        xa = xb;
+        fa = fb;
+        xb = xc;
+        fb = std::log(evaluatePm(xb));
+      }
+    }
+    return xb;
+  }
+  double getXStar() const {
+    if (polyDegree == 0) return 1.0;
+    if (polyDegree == 1) return 0.3;
+    const double xp = getPmPrimeFirstRoot();
+    const double yp = evaluatePm(xp);
+    return evaluateInversePm(-yp);
+  }
+  double getUpperXStar() const {
+    if (polyDegree == 0) return 1.0;
+    if (polyDegree == 1) return 0.6;
+    if (polyDegree == 2) return ((1.0 - getXStar() + 0.6) / 2.0);
+    return getPmNextToLastRoot();
+  }
+  // P_m((lambda+shift)/rhoB)^rep, rhoB = lambdaMax/UpperX* + shift/UpperX*
+  // (LanczosCpoly.h:356-361).
+  double evaluatePolynomial(double lambda) const {
+    const double rhoB = lambdaMax / UpperXStar + shift / UpperXStar;
+    return std::pow(evaluatePm((lambda + shift) / rhoB), int(repetitionFactor));
+  }
+  double getLambdaStar() const {
+    const double rhoB = lambdaMax / UpperXStar + shift / UpperXStar;
+    return rhoB * XStar - shift;
+  }
+  // Linear search in the degree for the first m with
+  // (X*/UpperX*)(lambdaBound+shift) < lambdaStar+shift, then reset the bound
+  // so that lambdaStar maps to X* (LanczosCpoly.h:498-555).
+  void getStarDegreeAndSpectralRadius(double sh, double lambdaBound, double lambdaStar,
+                                      long degMax, long& starDegree, double& starBound) {
+    const double lss = lambdaStar + sh;
+    polyDegree = 1;
+    double ratio = getXStar() / getUpperXStar();
+    if (ratio * (lambdaBound + sh) < lss) {
+      starDegree = 1;
+      starBound = (lss / ratio) - sh;
+      return;
+    }
+    long MB = 2;
+    bool found = false;
+    while (!found && MB < degMax) {
+      polyDegree = MB;
+      ratio = getXStar() / getUpperXStar();
+      if (ratio * (lambdaBound + sh) < lss) found = true;
+      else MB += 1;
+    }
+    if (MB >= degMax) {
+      starDegree = degMax;
+      polyDegree = starDegree;
+      starBound = lambdaBound;
+      return;
+    }
+    starDegree = MB;
+    polyDegree = starDegree;
+    ratio = getXStar() / getUpperXStar();
+    starBound = (lss / ratio) - sh;
+  }
+};
+
+// -------------------------------------------------------- block filter
+// out = P_m(A')^rep X with A' = (A+shift)/rhoB (LanczosCpolyOperatorLM.h:192-304):
+//   X1 = g2 (g1 A X0 + X0),  Xj = g2 (g1 A X_{j-1} + X_{j-1}) - X_{j-2},  out = Xm/(m+1).
+template <class T, class Op>
+void filter_apply(const Op& op, Block<T>& V, long m, long rep, double lambdaMax, double shift,
+                  double upperXStar) {
+  const double rhoB = lambdaMax / upperXStar + shift / upperXStar;
+  const double g1 = -2.0 / (rhoB - 2.0 * shift);
+  const double g2 = 2.0 - (4.0 * shift) / rhoB;
+  const long b = (long)V.size();
+  for (long r = 1; r <= rep; ++r) {
+    Block<T> Xm2 = V;
+    Block<T> Xm1 = V;
+    op.apply(Xm1);
+#pragma omp parallel for schedule(static, 1)
+    for (long p = 0; p < b; ++p) {
+      auto& a = Xm1[p];
+      const auto& x0 = Xm2[p];
+      for (size_t i = 0; i < a.size(); ++i) {
+        T t = a[i] * g1;
+        t += x0[i];
+        t *= g2;
+        a[i] = t;
+      }
+    }
+    for (long k = 2; k <= m; ++k) {
+      Block<T> Xn = Xm1;
+      op.apply(Xn);
+#pragma omp parallel for schedule(static, 1)
+      for (long p = 0; p < b; ++p) {
+        auto& a = Xn[p];
+        const auto& x1 = Xm1[p];
+        const auto& x2 = Xm2[p];
+        for (size_t i = 0; i < a.size(); ++i) {
+          T t = a[i] * g1;
+          t += x1[i];
+          t *= g2;
+          t += -x2[i];
+          a[i] = t;
+        }
+      }
+      Xm2.swap(Xm1);
+      Xm1.swap(Xn);
+    }
+    const double sc = 1.0 / double(m + 1);
+#pragma omp parallel for schedule(static, 1)
+    for (long p = 0; p < b; ++p) {
+      auto& a = Xm1[p];
+      for (size_t i = 0; i < a.size(); ++i) a[i] *= sc;
+    }
+    V.swap(Xm1);
+  }
+}
+
+// ------------------------------------------------- tridiagonal bisection
+// idx-th smallest (0-based) eigenvalue of the symmetric tridiagonal (D, E) by
+// Sturm-count bisection, stopping at width max(abstol, 2 eps max|end|) and
+// returning the midpoint, as DSTEBZ does for RANGE='I' (Dstebz_C.h:112-995).
+inline long sturm_count(const std::vector<double>& D, const std::vector<double>& E, double x,
+                        double pivmin) {
+  long cnt = 0;
+  const long n = (long)D.size();
+  double q = D[0] - x;
+  if (std::abs(q) < pivmin) q = -pivmin;
+  if (q < 0) ++cnt;
+  for (long i = 1; i < n; ++i) {
+    q = D[i] - x - (E[i - 1] * E[i - 1]) / q;
+    if (std::abs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+inline double tri_eig(const std::vector<double>& D, const std::vector<double>& E, long idx,
+                      double abstol) {
+  const long n = (long)D.size();
+  double lo = D[0], hi = D[0], emax = 0.0;
+  for (long i = 0; i < n; ++i) {
+    double r = 0.0;
+    if (i > 0) r += std::abs(E[i - 1]);
+    if (i < n - 1) r += std::abs(E[i]);
+    lo = std::min(lo, D[i] - r);
+    hi = std::max(hi, D[i] + r);
+  }
+  for (long i = 0; i + 1 < n; ++i) emax = std::max(emax, E[i] * E[i]);
+  const double safmin = 2.2250738585072014e-308;
+  const double eps = 2.220446049250313e-16;
+  const double pivmin = std::max(safmin, safmin * emax) ;
+  const double pad = 2.0 * eps * std::max(std::abs(lo), std::abs(hi)) + 2.0 * pivmin;
+  lo -= pad;
+  hi += pad;
+  for (int it = 0; it < 200; ++it) {
+    const double w = hi - lo;
+    if (w <= std::max(abstol, 2.0 * eps * std::max(std::abs(lo), std::abs(hi)))) break;
+    const double mid = 0.5 * (lo + hi);
+    if (sturm_count(D, E, mid, pivmin) >= idx + 1) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// ------------------------------------------------------- Lanczos bounds
+// Single-vector Lanczos without reorthogonalization; every 5 steps the
+// extremal eigenvalues of the tridiagonal give the estimates; rate-
+// extrapolated stop (LanczosMaxMinFinder.h:248-544).
+struct LanczosResult {
+  double minEig = 0.0, maxEig = 0.0;
+  long iters = 0;
+};
+
+template <class T, class Op>
+LanczosResult lanczos_min_max(const Op& op, std::vector<T>& v, RandomStream& rnd,
+                              double errorTolerance, bool maxFlag, bool minFlag, long iterMax) {
+  const double smallTol = 1.0e-10;
+  auto relErr = [&](double val, double tol) {
+    return (std::abs(val) * tol > smallTol) ? std::abs(val) : smallTol / tol;
+  };
+  if (errorTolerance > 0.01) errorTolerance = 0.01;
+  double tol = errorTolerance;
+  if (tol < smallTol) tol = smallTol;
+  const double hermitianErrorTol = 1.0e-12;
+
+  std::vector<double> alpha(iterMax, 0.0), beta(iterMax, 0.0);
+  rnd.randomize(v);
+  const double vNorm = std::sqrt(std::abs(dot(v, v)));
+  scal(v, T(1.0 / vNorm));
+  std::vector<T> w = v, wTmp = v;
+  scal(v, T(0.0));
+  scal(wTmp, T(0.0));
+
+  double minDiffError = 1.0e10, maxDiffError = 1.0e10;
+  double lanczosMin = 0.0, lanczosMax = 0.0;
+  double rateFactorMax = 9.0, rateFactorMin = 9.0;
+  double rateEigMax = 1.0, rateEigMin = 1.0;
+  double eigMaxOld = 0.0, eigMinOld = 0.0;
+  double diffMaxA = 0.0, diffMinA = 0.0, diffMaxB = 0.0, diffMinB = 0.0;
+  int diffCompareFlag = 0, rateCompareFlag = 0;
+  long k = 0;
+  bool exitFlag = false;
+  while (k < iterMax && !exitFlag) {
+    if (k != 0 && std::abs(beta[k - 1]) > 1.0e-14) {
+      wTmp = w;
+      w = v;
+      scal(w, T(1.0 / beta[k - 1]));
+      v = wTmp;
+      scal(v, T(-beta[k - 1]));
+    }
+    wTmp = w;
+    op.apply(wTmp);
+    for (size_t i = 0; i < v.size(); ++i) v[i] += wTmp[i];
+    const T ak = dot(w, v);
+    alpha[k] = realv(ak);
+    if (std::abs(imagv(ak)) > (1.0 + std::abs(realv(ak))) * hermitianErrorTol)
+      throw std::invalid_argument("Lanczos max/min finder: operator is not Hermitian");
+    wTmp = w;
+    scal(wTmp, T(-alpha[k]));
+    for (size_t i = 0; i < v.size(); ++i) v[i] += wTmp[i];
+    beta[k] = std::sqrt(std::abs(dot(v, v)));
+    k += 1;
+    if (k % 5 == 0) {
+      std::vector<double> D(alpha.begin(), alpha.begin() + k);
+      std::vector<double> E(beta.begin(), beta.begin() + (k - 1));
+      lanczosMax = tri_eig(D, E, k - 1, 1.0e-14);
+      lanczosMin = tri_eig(D, E, 0, 1.0e-14);
+      if (diffCompareFlag == 0) {
+        eigMaxOld = lanczosMax;
+        eigMinOld = lanczosMin;
+        diffCompareFlag = 1;
+      } else {
+        maxDiffError = std::abs(lanczosMax - eigMaxOld) / relErr(lanczosMax, tol);
+        minDiffError = std::abs(lanczosMin - eigMinOld) / relErr(lanczosMin, tol);
+        if (rateCompareFlag == 0) {
+          diffMaxA = std::abs(lanczosMax - eigMaxOld);
+          diffMinA = std::abs(lanczosMin - eigMinOld);
+          rateCompareFlag = 1;
+        } else {
+          if (maxDiffError > 1.0e-09) {
+            diffMaxB = std::abs(lanczosMax - eigMaxOld);
+            rateEigMax = diffMaxB / diffMaxA;
+          } else {
+            rateEigMax = 0.95;
+          }
+          if (minDiffError > 1.0e-09) {
+            diffMinB = std::abs(lanczosMin - eigMinOld);
+            rateEigMin = diffMinB / diffMinA;
+          } else {
+            rateEigMin = 0.95;
+          }
+          rateFactorMax = 9.0;
+          rateFactorMin = 9.0;
+          if (rateEigMax < 0.90) rateFactorMax = rateEigMax / (1.0 - rateEigMax);
+          if (rateEigMin < 0.90) rateFactorMin = rateEigMin / (1.0 - rateEigMin);
+          diffMaxA = diffMaxB;
+          diffMinA = diffMinB;
+        }
+        eigMaxOld = lanczosMax;
+        eigMinOld = lanczosMin;
+        minDiffError *= rateFactorMin;
+        maxDiffError *= rateFactorMax;
+      }
+      if (maxFlag && minFlag) {
+        if (minDiffError < tol && maxDiffError < tol) exitFlag = true;
+      } else if (maxFlag) {
+        if (maxDiffError < tol) exitFlag = true;
+      } else if (minFlag) {
+        if (minDiffError < tol) exitFlag = true;
+      }
+    }
+  }
+  LanczosResult r;
+  r.iters = k;
+  if (lanczosMin > lanczosMax) std::swap(lanczosMin, lanczosMax);
+  r.minEig = lanczosMin;
+  r.maxEig = lanczosMax;
+  return r;
+}
+
+// --------------------------------------------------------------- Jacobi
+// Cyclic-threshold Jacobi for a real symmetric n x n matrix in column storage
+// (upper triangle read), eigenvalues sorted increasingly with their vectors
+// (JacobiDiagonalizer.h:114-378, 486-540; threshold 1e-15 at line 4).
+struct Jacobi {
+  long n = 0;
+  std::vector<double> M, Ev, diag;
+  std::vector<long> sidx;
+  double& m(long i, long j) { return M[i + j * n]; }
+  double& ev(long i, long j) { return Ev[i + j * n]; }
+
+  double offNorm2() {
+    double f = 0.0;
+    for (long i = 0; i < n; ++i)
+      for (long j = i + 1; j < n; ++j) f += m(i, j) * m(i, j);
+    return f;
+  }
+  void rotate(long i, long j, double c, double s, double t) {
+    const double tau = s / (1.0 + c);
+    double t1, t2;
+    for (long k = 0; k < i; ++k) {
+      t1 = m(k, i); t2 = m(k, j);
+      m(k, i) = t1 + s * (t2 - tau * t1);
+      m(k, j) = t2 - s * (t1 + tau * t2);
+    }
+    for (long k = j + 1; k < n; ++k) {
+      t1 = m(i, k); t2 = m(j, k);
+      m(i, k) = t1 + s * (t2 - tau * t1);
+      m(j, k) = t2 - s * (t1 + tau * t2);
+    }
+    for (long k = i + 1; k < j; ++k) {
+      t1 = m(i, k); t2 = m(k, j);
+      m(i, k) = t1 + s * (t2 - tau * t1);
+      m(k, j) = t2 - s * (t1 + tau * t2);
+    }
+    diag[i] += t * m(i, j);
+    diag[j] -= t * m(i, j);
+    m(i, j) = 0.0;
+    for (long k = 0; k < n; ++k) {
+      t1 = ev(k, i); t2 = ev(k, j);
+      ev(k, i) = t1 + s * (t2 - tau * t1);
+      ev(k, j) = t2 - s * (t1 + tau * t2);
+    }
+  }
+  void decompose() {
+    const double kTol = 1.0e-15;
+    double omega = offNorm2();
+    const double npairs = (n * (n - 1)) / 2.0;
+    double thr = std::sqrt(omega / npairs);
+    bool forced = false;
+    while (thr > kTol || forced) {
+      int rotFlag = 0;
+      double omegaInc = 0.0;
+      double omegaBase = omega;
+      for (long k = 0; k < n; ++k) diag[k] = 0.0;
+      for (long i = 0; i < n - 1; ++i) {
+        for (long j = i + 1; j < n; ++j) {
+          if (std::abs(m(i, j)) > thr) {
+            omegaInc += m(i, j) * m(i, j);
+            rotFlag = 1;
+            omega = omegaBase - omegaInc;
+            if (omega < 0.0) {
+              omega = offNorm2();
+              omegaInc = 0.0;
+              omegaBase = omega;
+            }
+            thr = std::sqrt(omega / npairs);
+            const double M1 = m(i, i) - m(j, j);
+            const double M2 = diag[i] - diag[j];
+            const double tau = (M1 + M2) / (2.0 * m(i, j));
+            const double sg = (tau >= 0.0) ? 1.0 : -1.0;
+            const double t = sg / (std::abs(tau) + std::sqrt(1.0 + tau * tau));
+            const double c = 1.0 / (std::sqrt(1.0 + t * t));
+            const double s = c * t;
+            rotate(i, j, c, s, t);
+          }
+        }
+      }
+      if (rotFlag == 0) {
+        // Every off-diagonal magnitude equals the RMS threshold (always so for
+        // n == 2): the reference stops here unrotated.  Sweep once more with a
+        // zero threshold instead.
+        if (omega > 0.0 && !forced) {
+          forced = true;
+          thr = 0.0;
+          continue;
+        }
+        break;
+      }
+      forced = false;
+      for (long k = 0; k < n; ++k) m(k, k) += diag[k];
+      omega = offNorm2();
+      thr = std::sqrt(omega / npairs);
+    }
+  }
+  // Shell sort (increments 1,4,13,...) carrying the index, increasing order.
+  static void sortWithIndex(double* vals, long* index, long nn) {
+    for (long i = 0; i < nn; ++i) index[i] = i;
+    long inc = 1;
+    do { inc *= 3; inc++; } while (inc <= nn);
+    do {
+      inc /= 3;
+      for (long i = inc + 1; i <= nn; ++i) {
+        const double vt = vals[i - 1];
+        const long it = index[i - 1];
+        long j = i;
+        while (vals[(j - inc) - 1] > vt) {
+          vals[j - 1] = vals[(j - inc) - 1];
+          index[j - 1] = index[(j - inc) - 1];
+          j -= inc;
+          if (j <= inc) break;
+        }
+        vals[j - 1] = vt;
+        index[j - 1] = it;
+      }
+    } while (inc > 1);
+  }
+  // H: column storage n x n (upper triangle used). evals: n. evecs: column storage.
+  void eigensystem(const double* H, long nn, double* evals, double* evecs) {
+    n = nn;
+    M.assign(n * n, 0.0);
+    Ev.assign(n * n, 0.0);
+    diag.assign(n, 0.0);
+    sidx.assign(n, 0);
+    for (long i = 0; i < n; ++i)
+      for (long j = i; j < n; ++j) m(i, j) = H[i + j * n];
+    for (long i = 0; i < n; ++i) ev(i, i) = 1.0;
+    if (n > 1) decompose();
+    for (long k = 0; k < n; ++k) evals[k] = m(k, k);
+    sortWithIndex(evals, sidx.data(), n);
+    for (long i = 0; i < n; ++i)
+      for (long j = 0; j < n; ++j) evecs[i + j * n] = ev(i, sidx[j]);
+  }
+};
+
+// Complex Hermitian Jacobi (no reference counterpart runs here: the reference
+// calls ZHPEVX from SCC::LapackInterface, absent from the image).  Each
+// rotation zeroes h_pq with the unitary [[c, -s e^{i phi}],[s e^{-i phi}, c]].
+struct HermitianJacobi {
+  static void eigensystem(const cplx* H, long n, double* evals, cplx* evecs) {
+    std::vector<cplx> A(H, H + n * n);
+    std::vector<cplx> V(n * n, cplx(0.0));
+    for (long i = 0; i < n; ++i) V[i + i * n] = 1.0;
+    auto a = [&](long i, long j) -> cplx& { return A[i + j * n]; };
+    // make exactly Hermitian from the upper triangle
+    for (long j = 0; j < n; ++j) {
+      a(j, j) = cplx(a(j, j).real(), 0.0);
+      for (long i = j + 1; i < n; ++i) a(i, j) = std::conj(a(j, i));
+    }
+    for (int sweep = 0; sweep < 100; ++sweep) {
+      double off = 0.0, tot = 0.0;
+      for (long j = 0; j < n; ++j)
+        for (long i = 0; i < n; ++i) {
+          const double v = std::norm(a(i, j));
+          tot += v;
+          if (i != j) off += v;
+        }
+      if (off <= 1e-30 * tot || off == 0.0) break;
+      for (long p = 0; p < n - 1; ++p) {
+        for (long q = p + 1; q < n; ++q) {
+          const cplx apq = a(p, q);
+          const double mag = std::abs(apq);
+          if (mag == 0.0) continue;
+          const cplx ph = apq / mag;  // e^{i phi}
+          const double app = a(p, p).real(), aqq = a(q, q).real();
+          const double tau = (aqq - app) / (2.0 * mag);
+          const double t = (tau >= 0 ? 1.0 : -1.0) / (std::abs(tau) + std::sqrt(1.0 + tau * tau));
+          const double c = 1.0 / std::sqrt(1.0 + t * t);
+          const double s = t * c;
+          // Givens: columns p,q of A := A G with G(p,p)=c, G(q,q)=c,
+          // G(p,q)=s*ph, G(q,p)=-s*conj(ph); then rows: A := G^H A.
+          for (long k = 0; k < n; ++k) {
+            const cplx akp = a(k, p), akq = a(k, q);
+            a(k, p) = c * akp - s * std::conj(ph) * akq;
+            a(k, q) = s * ph * akp + c * akq;
+          }
+          for (long k = 0; k < n; ++k) {
+            const cplx apk = a(p, k), aqk = a(q, k);
+            a(p, k) = c * apk - s * ph * aqk;
+            a(q, k) = s * std::conj(ph) * apk + c * aqk;
+          }
+          a(p, q) = 0.0;
+          a(q, p) = 0.0;
+          a(p, p) = cplx(a(p, p).real(), 0.0);
+          a(q, q) = cplx(a(q, q).real(), 0.0);
+          for (long k = 0; k < n; ++k) {
+            const cplx vkp = V[k + p * n], vkq = V[k + q * n];
+            V[k + p * n] = c * vkp - s * std::conj(ph) * vkq;
+            V[k + q * n] = s * ph * vkp + c * vkq;
+          }
+        }
+      }
+    }
+    std::vector<double> d(n);
+    std::vector<long> idx(n);
+    for (long i = 0; i < n; ++i) d[i] = a(i, i).real();
+    Jacobi::sortWithIndex(d.data(), idx.data(), n);
+    for (long j = 0; j < n; ++j) {
+      evals[j] = d[j];
+      for (long i = 0; i < n; ++i) evecs[i + j * n] = V[i + idx[j] * n];
+    }
+  }
+};
+
+// ------------------------------------------------------------- driver
+enum StopCondition { DEFAULT = 0, COMBINATION = 1, EIGENVALUE_ONLY = 2, RESIDUAL_ONLY = 3 };
+
+struct TraceRow {
+  long pass, iter, degree;
+  double starBound, shift, lambdaStar, maxResidual, maxEigDiff, eigDiffRatio, maxGap;
+};
+
+// RayleighChebyshevLM restated (RayleighChebyshevLM.h:213-2246).  T is the
+// vector scalar (double or complex<double>); Op provides apply(vector&) and
+// apply(Block&).
+template <class T, class Op>
+class RCLM {
+ public:
+  // settings, defaults RayleighChebyshevLM.h:190-202, 223-259
+  double maxMinTol = 1.0e-6;
+  long maxInnerLoopCount = 10000;
+  long minIntervalPolyDegreeMax = 200;
+  long filterRepetitionCount = 1;
+  bool intervalStopConditionFlag = false;
+  bool hardIntervalStopFlag = false;
+  bool nonRandomStartFlag = false;
+  bool fixedIterationCount = false;
+  StopCondition stopCondition = COMBINATION;
+  long lanczosIterMax = 10000;
+
+  // outputs
+  double minEigValueEst = 0.0, maxEigValueEst = 0.0, guardValue = 0.0;
+  long lanczosIters = 0;
+  std::vector<double> eigVecResiduals;
+  std::map<std::string, double> finalData;
+  std::map<std::string, long> countData;
+  std::vector<TraceRow> trace;
+
+  static double relErrFactor(double val, double tol) {
+    const double small = 1.0e-11;  // RayleighChebyshevLM.h:202
+    return (std::abs(val) * tol > small) ? std::abs(val) : small / tol;
+  }
+
+  // getMinEigenSystem(eigCount, tol, inc, buf, ...) (RayleighChebyshevLM.h:712-738)
+  long getMinEigenSystem(long eigCount, double tol, long inc, long buf, std::vector<T>& vStart,
+                         const Op& op, RandomStream& rnd, std::vector<double>& eigValues,
+                         Block<T>& eigVectors) {
+    LanczosResult lr =
+        lanczos_min_max<T>(op, vStart, rnd, maxMinTol, true, false, lanczosIterMax);
+    lanczosIters = lr.iters;
+    double minEig = lr.minEig, maxEig = lr.maxEig;
+    minEigValueEst = minEig;
+    maxEigValueEst = maxEig;
+    if (std::abs(maxEig - minEig) > 1.0e-12) maxEig += 0.001 * std::abs(maxEig - minEig);
+    intervalStopConditionFlag = false;
+    return getMinEigenSystem(eigCount, minEig, maxEig, tol, inc, buf, vStart, op, rnd, eigValues,
+                             eigVectors);
+  }
+  // getMinEigenSystem with given bounds (RayleighChebyshevLM.h:693-708)
+  long getMinEigenSystem(long eigCount, double minEig, double maxEigBound, double tol, long inc,
+                         long buf, std::vector<T>& vStart, const Op& op, RandomStream& rnd,
+                         std::vector<double>& eigValues, Block<T>& eigVectors) {
+    intervalStopConditionFlag = false;
+    hardIntervalStopFlag = true;
+    return base(minEig, maxEigBound, maxEigBound, tol, inc, buf, eigCount, vStart, op, rnd,
+                eigValues, eigVectors);
+  }
+  // getMinIntervalEigenSystem(lambdaMax, ...) (RayleighChebyshevLM.h:745-782)
+  long getMinIntervalEigenSystem(double lambdaMax, double tol, long inc, long buf, long maxDim,
+                                 std::vector<T>& vStart, const Op& op, RandomStream& rnd,
+                                 std::vector<double>& eigValues, Block<T>& eigVectors) {
+    LanczosResult lr =
+        lanczos_min_max<T>(op, vStart, rnd, maxMinTol, true, true, lanczosIterMax);
+    lanczosIters = lr.iters;
+    double minEig = lr.minEig, maxEig = lr.maxEig;
+    minEigValueEst = minEig;
+    maxEigValueEst = maxEig;
+    if (lambdaMax < minEig) {
+      eigValues.clear();
+      eigVectors.clear();
+      return 0;
+    }
+    if (std::abs(maxEig - minEig) > 1.0e-12) maxEig += 0.001 * std::abs(maxEig - minEig);
+    intervalStopConditionFlag = true;
+    return base(minEig, lambdaMax, maxEig, tol, inc, buf, maxDim, vStart, op, rnd, eigValues,
+                eigVectors);
+  }
+
+  // ---- block helpers (RayleighChebyshevLM.h:1782-2087)
+  // Modified Gram-Schmidt, column order.
+  static void orthogonalize(Block<T>& V) {
+    const long b = (long)V.size();
+    for (long k = 1; k <= b; ++k) {
+      const double rkk = std::sqrt(std::abs(dot(V[k - 1], V[k - 1])));
+      scal(V[k - 1], T(1.0 / rkk));
+#pragma omp parallel for schedule(static, 1)
+      for (long j = k + 1; j <= b; ++j) {
+        const T rkj = dot(V[j - 1], V[k - 1]);
+        axpy(V[j - 1], T(-rkj), V[k - 1]);
+      }
+    }
+  }
+  // Project A[ia0..ia1] against B[ib0..ib1] one B vector at a time, then normalize.
+  static void orthogonalizeAtoB(Block<T>& A, long ia0, long ia1, const Block<T>& B, long ib0,
+                                long ib1) {
+    for (long j = ib0; j <= ib1; ++j) {
+#pragma omp parallel for schedule(static, 1)
+      for (long k = ia0; k <= ia1; ++k) {
+        const T rkj = dot(A[k], B[j]);
+        axpy(A[k], T(-rkj), B[j]);
+      }
+    }
+#pragma omp parallel for schedule(static, 1)
+    for (long k = ia0; k <= ia1; ++k) {
+      const double rkk = std::sqrt(std::abs(dot(A[k], A[k])));
+      scal(A[k], T(1.0 / rkk));
+    }
+  }
+  // H(j,i) = V_j . (A V_i) for j >= i, mirrored (RayleighChebyshevLM.h:1928-1956).
+  void formVtAV(const Op& op, const Block<T>& V, std::vector<T>& H) {
+    const long b = (long)V.size();
+    Block<T> W = V;
+    op.apply(W);
+    H.assign(b * b, T(0.0));
+#pragma omp parallel for schedule(static, 1)
+    for (long i = 0; i < b; ++i)
+      for (long j = i; j < b; ++j) H[j + i * b] = dot(V[j], W[i]);
+    for (long i = 0; i < b; ++i)
+      for (long j = i + 1; j < b; ++j) H[i + j * b] = H[j + i * b];
+  }
+  void smallEig(const std::vector<T>& H, long b, std::vector<double>& evals, std::vector<T>& Y) {
+    evals.assign(b, 0.0);
+    Y.assign(b * b, T(0.0));
+    if constexpr (std::is_same<T, double>::value) {
+      Jacobi J;
+      J.eigensystem(H.data(), b, evals.data(), Y.data());
+    } else {
+      // formVtAV stores H(j,i) = V_j.dot(A V_i) = conj(V_j^H A V_i) for j >= i,
+      // which equals the Hermitian entry V_i^H A V_j; place it at (i,j).
+      std::vector<T> Hh(b * b);
+      for (long j = 0; j < b; ++j)
+        for (long i = 0; i <= j; ++i) Hh[i + j * b] = H[j + i * b];
+      HermitianJacobi::eigensystem(Hh.data(), b, evals.data(), Y.data());
+    }
+  }
+  // V <- normalize(V Y); residuals r_k = ||theta_k V_k - A V_k|| for k < count.
+  void eigenVectorsAndResiduals(const Op& op, const std::vector<T>& Y,
+                                const std::vector<double>& theta, Block<T>& V, long count,
+                                std::vector<double>& res) {
+    const long b = (long)V.size();
+    Block<T> Tm(b, std::vector<T>(V.empty() ? 0 : V[0].size()));
+#pragma omp parallel for schedule(static, 1)
+    for (long k = 0; k < b; ++k) {
+      auto& t = Tm[k];
+      const T y0 = Y[0 + k * b];
+      for (size_t i = 0; i < t.size(); ++i) t[i] = V[0][i] * y0;
+      for (long i = 1; i < b; ++i) axpy(t, Y[i + k * b], V[i]);
+      const double rkk = std::sqrt(std::abs(dot(t, t)));
+      scal(t, T(1.0 / rkk));
+    }
+    V = Tm;
+    res.assign(count, 0.0);
+    op.apply(Tm);
+#pragma omp parallel for schedule(static, 1)
+    for (long k = 0; k < count; ++k) {
+      std::vector<T> m = V[k];
+      scal(m, T(theta[k]));
+      for (size_t i = 0; i < m.size(); ++i) m[i] -= Tm[k][i];
+      res[k] = std::sqrt(std::abs(dot(m, m)));
+    }
+  }
+
+  // getMinIntervalEigenSystem_Base (RayleighChebyshevLM.h:817-1776)
+  long base(double minEigValue, double lambdaMax, double maxEigValue, double subspaceTol,
+            long inc, long buf, long maxEigensystemDim, std::vector<T>& vStart, const Op& op,
+            RandomStream& rnd, std::vector<double>& eigValues, Block<T>& eigVectors) {
+    countData = {{"ortho", 0}, {"OpApply", 0}, {"eigenvalue", 0}};
+    finalData = {{"maxResidual", 0.0}, {"maxEigValueDifference", 0.0}, {"maxRelEigValueGap", 0.0}};
+    trace.clear();
+    std::vector<double> residualHistory;
+    if (subspaceTol < 1.0e-11) subspaceTol = 1.0e-11;
+    eigValues.clear();
+    eigVecResiduals.clear();
+    if (!nonRandomStartFlag) eigVectors.clear();
+
+    double lambdaStar = maxEigValue;
+    long subspaceSize = inc + buf;
+    long foundSize = 0, foundCount = 0;
+    bool completedBasisFlag = false;
+    const long N = (long)vStart.size();
+    if (subspaceSize > N) {
+      if (inc < N) {
+        buf = N - inc;
+        subspaceSize = N;
+      } else {
+        subspaceSize = N;
+        inc = N;
+        buf = 0;
+      }
+      maxEigensystemDim = N;
+    }
+    std::vector<double> oldEigs(subspaceSize, 0.0), eigDiffs(subspaceSize, 1.0),
+        oldEigDiffs(subspaceSize, 1.0), subspaceResiduals;
+    Block<T> vArray(subspaceSize);
+    std::vector<double> theta(subspaceSize, 0.0);
+    std::vector<T> H, Y;
+
+    long starDegree = 0, starDegreeSave = 0;
+    double starBoundSave = 0.0, shift = 0.0, starBound = 0.0, maxEigDiff = 0.0,
+           maxResidual = 0.0, maxGap = 0.0, stopCheckValue = 0.0, eigDiffRatio = 0.0;
+    long residualCheckCount = 0, innerLoopCount = 0;
+
+    if (!nonRandomStartFlag) {
+      for (long k = 0; k < subspaceSize; ++k) {
+        vArray[k] = vStart;
+        rnd.randomize(vArray[k]);
+      }
+    } else {
+      const long have = (long)eigVectors.size();
+      for (long k = 0; k < subspaceSize; ++k) {
+        if (k < have) vArray[k] = eigVectors[k];
+        else {
+          vArray[k] = vStart;
+          rnd.randomize(vArray[k]);
+        }
+      }
+      eigVectors.clear();
+    }
+
+    if (N == subspaceSize) {
+      long checks = 1;
+      orthogonalize(vArray);
+      while (orthoError(vArray) > 1.0e-12 && checks <= 10) {
+        ++checks;
+        orthogonalize(vArray);
+      }
+      if (checks > 10)
+        throw std::runtime_error("RayleighChebyshevLM: unable to create basis for complete vector space");
+      formVtAV(op, vArray, H);
+      smallEig(H, subspaceSize, theta, Y);
+      eigenVectorsAndResiduals(op, Y, theta, vArray, subspaceSize, eigVecResiduals);
+      eigVectors = vArray;
+      eigValues = theta;
+      return subspaceSize;
+    }
+
+    if (fixedIterationCount) maxInnerLoopCount = 1;  // bool->long assignment, LM:1047-1050
+
+    bool exitFlag = false;
+    long applyCountCumulative = 0;
+    long pass = 0;
+    while (!exitFlag) {
+      for (long k = buf; k < subspaceSize; ++k) oldEigs[k] = oldEigs[buf];
+      if (applyCountCumulative > 0)
+        for (long k = buf; k < subspaceSize; ++k) rnd.randomize(vArray[k]);
+      orthogonalize(vArray);
+      orthogonalize(vArray);
+      countData["ortho"] += 2;
+
+      lambdaStar = maxEigValue;
+      eigDiffRatio = 1.0;
+      innerLoopCount = 0;
+      starDegreeSave = 0;
+      starBoundSave = 0.0;
+      stopCheckValue = subspaceTol + 1.0e10;
+      long applyCount = 0;
+      residualHistory.clear();
+      maxGap = 0.0;
+      maxResidual = 0.0;
+      maxEigDiff = 0.0;
+      long oscillationCount = 0;
+
+      while (stopCheckValue > subspaceTol && innerLoopCount < maxInnerLoopCount) {
+        shift = -minEigValue;
+        starDegreeSave = starDegree;
+        starBoundSave = starBound;
+        starDegree = 0;
+        const double eMin = std::abs(minEigValue - lambdaStar) / relErrFactor(minEigValue, subspaceTol);
+        const double eMax = std::abs(maxEigValue - lambdaStar) / relErrFactor(maxEigValue, subspaceTol);
+        if (eMin < subspaceTol && eMax < subspaceTol) {
+          starDegree = 1;
+          starBound = maxEigValue;
+        } else if (!(maxEigDiff > subspaceTol) && innerLoopCount > 3 && eigDiffRatio < .2) {
+          starDegree = starDegreeSave;
+          starBound = starBoundSave;
+        }
+        if (starDegree == 0) {
+          Cpoly cp;
+          cp.getStarDegreeAndSpectralRadius(shift, maxEigValue, lambdaStar,
+                                            minIntervalPolyDegreeMax, starDegree, starBound);
+        }
+        if (starDegree < starDegreeSave) {
+          starDegree = starDegreeSave;
+          starBound = starBoundSave;
+        }
+        Cpoly fp;
+        fp.set(starDegree, filterRepetitionCount, starBound, shift);
+        if (!completedBasisFlag)
+          filter_apply<T>(op, vArray, starDegree, filterRepetitionCount, starBound, shift,
+                          fp.UpperXStar);
+        applyCount += 1;
+        countData["OpApply"] += starDegree * subspaceSize;
+
+        if (foundSize > 0) orthogonalizeAtoB(vArray, 0, subspaceSize - 1, eigVectors, 0, foundSize - 1);
+        else orthogonalizeAtoB(vArray, 0, subspaceSize - 1, eigVectors, 0, -1);
+        orthogonalize(vArray);
+        countData["ortho"] += 1;
+
+        formVtAV(op, vArray, H);
+        countData["OpApply"] += subspaceSize;
+        smallEig(H, subspaceSize, theta, Y);
+
+        residualCheckCount = (foundSize + subspaceSize < N) ? inc : subspaceSize;
+        subspaceResiduals.clear();
+        eigenVectorsAndResiduals(op, Y, theta, vArray, residualCheckCount, subspaceResiduals);
+        countData["OpApply"] += residualCheckCount;
+
+        maxResidual = 0.0;
+        for (size_t k = 0; k < subspaceResiduals.size(); ++k)
+          maxResidual = std::max(maxResidual, subspaceResiduals[k] / relErrFactor(theta[k], subspaceTol));
+        residualHistory.push_back(maxResidual);
+        countData["eigenvalue"] += 1;
+
+        const long checkSize = intervalStopConditionFlag ? inc + 1 : inc;
+        for (long i = 0; i < checkSize; ++i) oldEigDiffs[i] = eigDiffs[i];
+        maxEigDiff = 0.0;
+        for (long i = 0; i < checkSize; ++i) {
+          double d = std::abs(theta[i] - oldEigs[i]);
+          d = d / relErrFactor(oldEigs[i], subspaceTol);
+          eigDiffs[i] = d;
+          maxEigDiff = (d > maxEigDiff) ? d : maxEigDiff;
+        }
+        for (long i = 0; i < subspaceSize; ++i) oldEigs[i] = theta[i];
+
+        long diffCount = 0;
+        eigDiffRatio = 0.0;
+        if (maxEigDiff > subspaceTol) {
+          for (long i = 0; i < checkSize; ++i) {
+            if (std::abs(oldEigDiffs[i]) > subspaceTol / 10.0) {
+              eigDiffRatio += std::abs(eigDiffs[i] / oldEigDiffs[i]);
+              diffCount++;
+            }
+          }
+          if (diffCount > 0) eigDiffRatio /= (double)diffCount;
+          else eigDiffRatio = 1.0;
+        }
+        const double spectralRange = std::abs(lambdaMax - minEigValue);
+        maxGap = 0.0;
+        for (long i = 1; i < checkSize; ++i)
+          maxGap = std::max(maxGap, std::abs(theta[i] - theta[i - 1]) / spectralRange);
+
+        trace.push_back({pass, innerLoopCount, starDegree, starBound, shift, lambdaStar,
+                         maxResidual, maxEigDiff, eigDiffRatio, maxGap});
+
+        if (stopCondition == RESIDUAL_ONLY) stopCheckValue = maxResidual;
+        else if (stopCondition == EIGENVALUE_ONLY) stopCheckValue = maxEigDiff;
+        else {
+          stopCheckValue = maxEigDiff;
+          if (maxResidual > std::sqrt(subspaceTol)) stopCheckValue = maxResidual;
+        }
+        if (inc == 0) stopCheckValue = 0.0;
+        if (stopCondition == RESIDUAL_ONLY && maxResidual < std::sqrt(subspaceTol)) {
+          const long r = (long)residualHistory.size() - 1;
+          if (r > 3) {
+            const double dA = (residualHistory[r - 3] - 2.0 * residualHistory[r - 2] + residualHistory[r - 1]) / std::abs(maxResidual);
+            const double dB = (residualHistory[r - 2] - 2.0 * residualHistory[r - 1] + residualHistory[r]) / std::abs(maxResidual);
+            if (dA * dB < 0.0) {
+              oscillationCount += 1;
+              if (oscillationCount > 5) stopCheckValue = 0.0;
+            }
+          }
+        }
+        lambdaStar = theta[subspaceSize - 1];
+        minEigValue = (minEigValue < theta[0]) ? minEigValue : theta[0];
+        innerLoopCount++;
+      }
+
+      finalData["maxResidual"] = std::max(maxResidual, finalData["maxResidual"]);
+      finalData["maxEigValueDifference"] = std::max(maxEigDiff, finalData["maxEigValueDifference"]);
+      finalData["maxRelEigValueGap"] = std::max(maxGap, finalData["maxRelEigValueGap"]);
+      applyCountCumulative += applyCount;
+
+      foundCount = 0;
+      const long checkIndexCount = (foundSize + subspaceSize < N) ? inc : subspaceSize;
+      for (long i = 0; i < checkIndexCount; ++i) {
+        const double chk = (theta[i] - lambdaMax) / relErrFactor(lambdaMax, subspaceTol);
+        if (chk < subspaceTol) foundCount++;
+      }
+      if (foundCount + foundSize >= maxEigensystemDim) {
+        foundCount = maxEigensystemDim - foundSize;
+        exitFlag = true;
+      }
+      if (foundCount + foundSize >= N) {
+        foundCount = N - foundSize;
+        exitFlag = true;
+      }
+      if (foundCount > 0) {
+        eigVectors.resize(foundSize + foundCount);
+        eigValues.resize(foundSize + foundCount, 0.0);
+        for (long i = 0; i < foundCount; ++i) {
+          eigVectors[foundSize + i] = vArray[i];
+          eigValues[foundSize + i] = theta[i];
+          eigVecResiduals.push_back(subspaceResiduals[i]);
+        }
+        foundSize += foundCount;
+      }
+      if (!exitFlag)
+        for (long k = 0; k + foundCount < subspaceSize; ++k) vArray[k] = vArray[k + foundCount];
+
+      const double gv = (buf > 0) ? theta[subspaceSize - buf] : lambdaMax;
+      guardValue = gv;
+      const double gchk = (gv - lambdaMax) / relErrFactor(lambdaMax, subspaceTol);
+      if (hardIntervalStopFlag) {
+        if (guardValue > lambdaMax) exitFlag = true;
+      } else {
+        if (gchk >= 10.0 * subspaceTol) exitFlag = true;
+      }
+      if (foundSize > 0) minEigValue = eigValues[foundSize - 1];
+      starDegree = 1;
+      starBound = maxEigValue;
+
+      if (!exitFlag && (foundSize + subspaceSize) >= N) {
+        if (foundSize + inc >= N) {
+          buf = 0;
+          inc = N - foundSize;
+          subspaceSize = inc;
+          completedBasisFlag = true;
+          vArray.resize(subspaceSize);
+          for (long k = 0; k < subspaceSize; ++k) {
+            if (vArray[k].size() != (size_t)N) vArray[k] = vStart;
+            rnd.randomize(vArray[k]);
+          }
+        } else {
+          buf = N - (foundSize + inc);
+          subspaceSize = inc + buf;
+          vArray.resize(subspaceSize);
+        }
+        theta.resize(subspaceSize, 0.0);
+      }
+      ++pass;
+    }
+
+    bool nonMonotone = false;
+    for (long i = 0; i + 1 < foundSize; ++i)
+      if (eigValues[i] > eigValues[i + 1]) nonMonotone = true;
+    if (nonMonotone) {
+      foundSize = (long)eigVectors.size();
+      formVtAV(op, eigVectors, H);
+      countData["OpApply"] += foundSize;
+      std::vector<double> th;
+      smallEig(H, foundSize, th, Y);
+      eigVecResiduals.clear();
+      eigenVectorsAndResiduals(op, Y, th, eigVectors, foundSize, eigVecResiduals);
+      eigValues = th;
+      countData["OpApply"] += foundSize;
+      countData["eigenvalue"] += 1;
+    }
+    if (fixedIterationCount) maxInnerLoopCount = 10000;
+    if (intervalStopConditionFlag) {
+      long fc = 0;
+      const double rf = relErrFactor(lambdaMax, subspaceTol);
+      for (size_t i = 0; i < eigValues.size(); ++i)
+        if ((eigValues[i] - lambdaMax) / rf < subspaceTol) fc++;
+      if (fc < (long)eigValues.size()) {
+        eigValues.resize(fc);
+        eigVectors.resize(fc);
+        eigVecResiduals.resize(fc);
+        foundSize = fc;
+      }
+    }
+    return foundSize;
+  }
+
+  static double orthoError(const Block<T>& A) {
+    double e = 0.0;
+    for (size_t i = 0; i < A.size(); ++i)
+      for (size_t j = 0; j < A.size(); ++j) {
+        const double d = std::abs(dot(A[i], A[j]) - ((i == j) ? T(1.0) : T(0.0)));
+        e = std::max(e, d);
+      }
+    return e;
+  }
+};
+
+}  // namespace orc
