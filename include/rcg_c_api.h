@@ -1,0 +1,212 @@
+/*
+ * rcg_c_api.h -- C ABI of librcg.so, the MI355X (gfx950) Rayleigh-Chebyshev
+ * lowest-eigenpair solver.
+ *
+ * Plain pointers, sizes and opaque handles; no HIP or C++ types cross the
+ * boundary (a stream is passed as void*).  Every function returns an int
+ * status (0 = RCG_OK); rcg_get_last_error() returns the message of the last
+ * failure on the calling thread.  No function exits the process (the
+ * reference aborts through cudaErrchk.h:8-50).
+ *
+ * Each entry point names the reference interface it replaces
+ * (file:line in almaeder/RayleighChebyshevGPGPU).
+ *
+ * Device layout: an N x cols block is row-interleaved, element (i, c) at
+ * ((T*)data)[i*ld + c] with ld a multiple of 4; host arrays crossing the
+ * boundary are column-major like the reference's AmatrixClassLMCuda
+ * (AmatrixClassLMCuda.h:333-341).
+ */
+#ifndef RCG_C_API_H_
+#define RCG_C_API_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rcg_context_s* rcg_context;
+typedef struct rcg_operator_s* rcg_operator;
+typedef struct rcg_block_s* rcg_block;
+
+enum rcg_status {
+  RCG_OK = 0,
+  RCG_ERR_INVALID = 1,       /* bad argument (std::runtime_error in the reference) */
+  RCG_ERR_HIP = 2,           /* HIP runtime failure */
+  RCG_ERR_ALLOC = 3,         /* device or workspace allocation failed */
+  RCG_ERR_NOT_HERMITIAN = 4, /* std::invalid_argument, LanczosMaxMinFinder.h:350-356 */
+  RCG_ERR_NUMERIC = 5,       /* breakdown the solve could not recover from */
+  RCG_ERR_UNSUPPORTED = 6
+};
+
+enum rcg_dtype { RCG_F64 = 0, RCG_F32 = 1, RCG_C128 = 2, RCG_C64 = 3 };
+
+/* RC_Types.h:37 StopCondition */
+enum rcg_stop {
+  RCG_STOP_DEFAULT = 0,
+  RCG_STOP_COMBINATION = 1,
+  RCG_STOP_EIGENVALUE_ONLY = 2,
+  RCG_STOP_RESIDUAL_ONLY = 3
+};
+
+/* Operator memory flags for rcg_operator_csr_create. */
+enum rcg_mem { RCG_MEM_HOST = 0, RCG_MEM_DEVICE = 1 };
+
+/* ------------------------------------------------------------ errors */
+/* Replaces cudaErrchk/cublasErrchk/cusolverErrchk/cusparseErrchk
+ * (cudaErrchk.h:8-50): copy the last error message (NUL-terminated). */
+int rcg_get_last_error(char* buf, size_t len);
+/* Library version string, e.g. "rcg 0.1 gfx950". */
+const char* rcg_version(void);
+/* Number of HIP devices visible to the process. */
+int rcg_device_count(int* count);
+
+/* ----------------------------------------------------------- context */
+/* Device + stream binding for every handle made from it.  hip_stream may be
+ * NULL (the library creates a default-priority stream).  The reference runs
+ * on the legacy default stream (LMC). */
+int rcg_context_create(int device, void* hip_stream, rcg_context* out);
+int rcg_context_destroy(rcg_context ctx);
+int rcg_context_synchronize(rcg_context ctx);
+
+/* ------------------------------------------------------------ blocks */
+/* AmatrixClassLMCuda(m, n) / resize / create_cuda_memory
+ * (AmatrixClassLMCuda.h:21-80, 251-266): an N x cols device block,
+ * zero-filled. */
+int rcg_block_create(rcg_context ctx, int64_t n, int64_t cols, int dtype, rcg_block* out);
+int rcg_block_destroy(rcg_block b);
+/* host_to_device (AmatrixClassLMCuda.h:109-117): host column-major, leading
+ * dimension ld_host >= n, first `cols` columns of the block. */
+int rcg_block_upload(rcg_block b, const void* host, int64_t ld_host, int64_t cols);
+/* device_to_host (AmatrixClassLMCuda.h:119-127). */
+int rcg_block_download(rcg_block b, void* host, int64_t ld_host, int64_t cols);
+/* Device-to-device block copy (the cudaMemcpy D2D calls at
+ * LanczosCpolyOperatorLMCuda.h:216-263 and RayleighChebyshevLMCuda.h:2232). */
+int rcg_block_copy(rcg_block dst, rcg_block src);
+/* ArandomizeOpClassLMCuda::randomize(AmatrixClassLMCuda&)
+ * (ArandomizeOpClassLMCuda.h:74-91): columns [col0, col0+ncols) get U[-1,1)
+ * draws counter + j*n + i of stream `seed` (two draws per complex element);
+ * *counter advances by the draws consumed. */
+int rcg_block_randomize(rcg_block b, int64_t col0, int64_t ncols, uint64_t seed,
+                        uint64_t* counter);
+/* Shape and device pointer of a block. */
+int rcg_block_info(rcg_block b, int64_t* n, int64_t* cols, int64_t* ld, int* dtype,
+                   void** device_ptr);
+
+/* --------------------------------------------------------- operators */
+/* The user operator of the reference (not shipped there; its contract is
+ * Otype::apply(Atype& in, Atype& out, alpha, beta), LanczosCpolyOperatorLMCuda.h
+ * :236,245, RayleighChebyshevLMCuda.h:2135,2235): an N x N CSR matrix with
+ * int32 indices and sorted columns.  Arrays are copied to the device
+ * (mem = RCG_MEM_HOST) or referenced (RCG_MEM_DEVICE: the caller keeps them
+ * alive).  val_dtype: RCG_F64 (the FP32 copy for mixed-precision filtering is
+ * made on demand). */
+int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32_t* rowptr,
+                            const int32_t* col, const void* val, int val_dtype, int mem,
+                            rcg_operator* out);
+int rcg_operator_destroy(rcg_operator op);
+/* Otype::apply(in, out, alpha, beta): out = alpha*A*in + beta*out over the
+ * first `cols` columns (blocks share n and ld). */
+int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alpha, double beta,
+                       int64_t cols);
+
+/* ---------------------------------------------------- solve pieces */
+/* LanczosCpolyOperatorLMCuda::apply (LanczosCpolyOperatorLMCuda.h:194-265)
+ * and LanczosCpolyOperatorLM::apply (LanczosCpolyOperatorLM.h:192-304):
+ * X <- P_m(A')^rep X with A' = (A + shift)/rhoB, rhoB = (lambda_max + shift)
+ * / UpperX*(m), over the first `cols` columns; tmp is a block of X's shape
+ * used as the second recurrence buffer. */
+int rcg_cheb_filter(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols, int64_t degree,
+                    int64_t repetition, double lambda_max, double shift);
+/* RayleighChebyshevLM::orthogonalize (RayleighChebyshevLM.h:1782-1837) /
+ * cuSOLVER geqrf+orgqr (RayleighChebyshevLMCuda.h:1979-2107): X <- Q with
+ * X = QR, R upper triangular with positive diagonal (CholQR2, shifted when
+ * the Gram matrix is numerically singular); tmp as above. */
+int rcg_orthonormalize(rcg_block X, rcg_block tmp, int64_t cols);
+/* RayleighChebyshevLM::OrthogonalizeAtoB (RayleighChebyshevLM.h:1844-1917):
+ * X[:, :cols] <- normalize((I - Y Y^H) X) with Y = Ysrc[:, :found]
+ * (classical Gram-Schmidt applied twice). */
+int rcg_deflate(rcg_block X, int64_t cols, rcg_block Y, int64_t found);
+/* formVtAV (RayleighChebyshevLM.h:1928-1969, RayleighChebyshevLMCuda.h
+ * :2115-2163): W = A V, H = V^H W (cols x cols, column-major, host). */
+int rcg_project(rcg_operator op, rcg_block V, rcg_block W, int64_t cols, double* H_host);
+/* V*Y rotation of createEigenVectorsAndResiduals
+ * (RayleighChebyshevLMCuda.h:2200-2232): V <- normalize(V Y), Y host
+ * column-major cols x cols; tmp as above. */
+int rcg_rotate(rcg_block V, rcg_block tmp, int64_t cols, const double* Y_host);
+/* computeVtVeigensystem (RayleighChebyshevLM.h:478-569,
+ * RayleighChebyshevLMCuda.h:570-642): ascending eigenvalues and column
+ * eigenvectors of the symmetric n x n column-major H (FP64). */
+int rcg_small_eigensystem(int64_t n, const double* H, double* evals, double* evecs);
+/* cuda_kernels::residuals (cuda_kernels.cu:307-382) with the apply at
+ * RayleighChebyshevLMCuda.h:2235: W = A V, r_c = ||theta_c V_c - W_c|| for
+ * c < count. */
+int rcg_residuals(rcg_operator op, rcg_block V, rcg_block W, int64_t count, const double* theta,
+                  double* res_host);
+/* LanczosMaxMinFinder::getMinMaxEigenvalues (LanczosMaxMinFinder.h:248-544)
+ * on the device: stop_flags bit0 = converge the maximum, bit1 = the minimum.
+ * The start vector takes n draws of `seed` from *counter. */
+int rcg_lanczos_bounds(rcg_operator op, double tol, int stop_flags, int64_t itmax, uint64_t seed,
+                       uint64_t* counter, double* min_eig, double* max_eig, int64_t* iters);
+
+/* ------------------------------------------------------ whole solve */
+typedef struct rcg_solve_params {
+  int64_t k;              /* eigCount (or maxEigensystemDim for the interval solve) */
+  int64_t inc;            /* subspaceIncrementSize */
+  int64_t buf;            /* bufferSize */
+  double tol;             /* subspaceTol */
+  uint64_t seed;          /* start-vector stream */
+  int32_t stop;           /* rcg_stop */
+  int32_t interval;       /* 1: getMinIntervalEigenSystem(lambda_max, ...) */
+  int64_t max_inner;      /* setMaxInnerLoopCount, 0 = 10000 */
+  int64_t deg_max;        /* setMinIntervalPolyDegreeMax, 0 = 200 */
+  int64_t rep;            /* setFilterRepetitionCount, 0 = 1 */
+  double lambda_max;      /* interval bound */
+  int64_t lanczos_itmax;  /* setMaxSpectralBoundsIter, 0 = 10000 */
+  double maxmin_tol;      /* setMaxMinTolerance, 0 = 1e-6 */
+  int32_t fixed_iteration;/* setFixedIterationCount */
+  int32_t filter_dtype;   /* RCG_F64 (plain) or RCG_F32 (mixed precision filter) */
+  int32_t refine;         /* 1: final FP64 Rayleigh-Ritz over all found vectors */
+  int32_t verbose;        /* setVerboseFlag: one line per inner iteration on stdout */
+  int32_t use_bounds;     /* 1: getMinEigenSystem(k, min_eig, max_eig, ...) (no Lanczos) */
+  int32_t nonrandom_start;/* setNonRandomStartFlag: > 0 starts from the first
+                             nonrandom_start columns of `evecs` (LM:957-985) */
+  double min_eig;
+  double max_eig;
+} rcg_solve_params;
+
+typedef struct rcg_solve_result {
+  int64_t found;
+  int64_t lanczos_iters;     /* getSpectralBoundsIterCount */
+  double min_est, max_est;   /* getMinEigValueEst / getMaxEigValueEst */
+  double guard;              /* getGuardEigenvalue */
+  int64_t count_ortho, count_apply, count_eig; /* getCountData */
+  double final_maxres, final_maxdiff, final_maxgap; /* getFinalData */
+  int64_t trace_len;
+  uint64_t rng_counter;
+  int64_t filter_degree_sum; /* sum of starDegree over filter applications */
+  double t_total, t_lanczos, t_filter, t_ortho, t_rr; /* seconds */
+} rcg_solve_result;
+
+/* RayleighChebyshevLM::getMinEigenSystem (RayleighChebyshevLM.h:693-738;
+ * RayleighChebyshevLMCuda.h:775-820) and getMinIntervalEigenSystem
+ * (RayleighChebyshevLM.h:745-806).  evals and res (capacity `cap`) receive
+ * the eigenvalues and residual norms; evecs (optional, cols >= cap) receives
+ * the eigenvectors on the device; trace (optional, 10 doubles per inner
+ * iteration: pass, iteration, degree, starBound, shift, lambdaStar,
+ * maxResidual, maxEigDiff, eigDiffRatio, maxGap) up to trace_cap rows.
+ * Returns the found count through result->found. */
+int rcg_solve_lowest(rcg_operator op, const rcg_solve_params* params, rcg_solve_result* result,
+                     double* evals, double* res, int64_t cap, rcg_block evecs, double* trace,
+                     int64_t trace_cap);
+
+/* Sizes of the two structs (ABI check for bindings). */
+int64_t rcg_sizeof_solve_params(void);
+int64_t rcg_sizeof_solve_result(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RCG_C_API_H_ */

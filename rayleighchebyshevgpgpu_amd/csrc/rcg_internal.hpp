@@ -1,0 +1,114 @@
+// rcg_internal.hpp -- shared declarations of the HIP kernels' host launchers
+// and the library's internal handle types.
+//
+// Device layout of a block (the reference's AmatrixClassLMCuda, AM:12-359, is
+// column-major with ld = N): here an N x cols block is stored row-interleaved,
+// element (i, c) at data[i*ld + c], ld a multiple of 4 elements.  A CSR row of
+// the operator then gathers whole, contiguous rows of the block.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace rcg {
+
+enum DType : int { F64 = 0, F32 = 1, C128 = 2, C64 = 3 };
+
+inline size_t dtype_size(int dt) {
+  switch (dt) {
+    case F64: return 8;
+    case F32: return 4;
+    case C128: return 16;
+    case C64: return 8;
+  }
+  return 0;
+}
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define RCG_HIP(call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      throw ::rcg::Error(2, std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                                __FILE__ + ":" + std::to_string(__LINE__) + " (" #call ")"); \
+  } while (0)
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// -------------------------------------------------------------- launchers
+// All pointers are device pointers; every launch goes on `st`.
+
+// out = alpha*A*X + beta*out over `ncols` columns (ld shared by X and out).
+template <typename T>
+void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val, const T* X,
+                      T* out, int ld, int ncols, T alpha, T beta, hipStream_t st);
+
+// One three-term Chebyshev step of the scaled C polynomial:
+//   first: out = s * g2*(g1*A*X1 + X1)
+//   else : out = s * (g2*(g1*A*X1 + X1) - X0)
+// `out` may alias X0 (each row reads its own X0 entries before writing them).
+template <typename T>
+void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val, const T* X1,
+                      const T* X0, T* out, int ld, int ncols, T g1, T g2, T s, bool first,
+                      hipStream_t st);
+
+// G (bx x by, column-major, ldg = bx) = X^H Y over n rows; ws holds partials.
+template <typename T>
+void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, int by, T* G,
+                 T* ws, size_t ws_elems, hipStream_t st);
+template <typename T>
+size_t gram_ws_elems(int64_t n, int bx, int by);
+// Bound of gram_ws_elems over all widths <= bmax.
+size_t gram_ws_bound(int64_t n, int bmax);
+
+// out = alpha * X*C + beta * Z, X: n x bx (ldx), C: bx x by column-major (ldc),
+// Z/out: n x by (ldo).  out may alias Z, not X.
+template <typename T>
+void launch_tall_gemm(int64_t n, const T* X, int ldx, int bx, const T* C, int ldc, int by,
+                      T alpha, T beta, const T* Z, T* out, int ldo, hipStream_t st);
+
+// sums[c] = sum_i |X[i,c]|^2, c < nc.
+template <typename T>
+void launch_col_sumsq(int64_t n, const T* X, int ld, int nc, double* sums, double* ws,
+                      size_t ws_elems, hipStream_t st);
+// sums[c] = sum_i |theta[c] V[i,c] - W[i,c]|^2, c < nc.
+template <typename T>
+void launch_residual_sumsq(int64_t n, const T* V, const T* W, int ld, int nc, const double* theta,
+                           double* sums, double* ws, size_t ws_elems, hipStream_t st);
+size_t colred_ws_elems(int64_t n, int nc);
+
+// X[i,c] *= 1/sqrt(sums[c]), c < nc.
+template <typename T>
+void launch_normalize_cols(int64_t n, T* X, int ld, int nc, const double* sums, hipStream_t st);
+// dst[i, d0+j] = src[i, s0+j], j < cnt.
+template <typename T>
+void launch_copy_cols(int64_t n, const T* src, int lds, int s0, T* dst, int ldd, int d0, int cnt,
+                      hipStream_t st);
+// X[i, c0+j] = U[-1,1) draw (counter + j*n + i) of stream `seed` (complex: two
+// draws per element, counter + 2(j*n + i) + {0,1}).
+template <typename T>
+void launch_randomize_cols(int64_t n, T* X, int ld, int c0, int cnt, uint64_t seed,
+                           uint64_t counter, hipStream_t st);
+template <typename T>
+void launch_fill_cols(int64_t n, T* X, int ld, int c0, int cnt, T value, hipStream_t st);
+
+// ---- single-vector Lanczos (LanczosMaxMinFinder.h:248-544), FP64 only.
+// Scalars alpha[k], beta[k] live in device memory.
+void launch_lz_start(int64_t n, double* v, double* w, double* ws, size_t ws_elems,
+                     double* scratch, hipStream_t st);
+void launch_lz_swap(int64_t n, double* v, double* w, const double* beta_prev, hipStream_t st);
+void launch_lz_apply_dot(int64_t n, const int* rowptr, const int* col, const double* val,
+                         const double* w, double* v, double* alpha_k, double* ws,
+                         size_t ws_elems, hipStream_t st);
+void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* alpha_k,
+                         double* beta_k, double* ws, size_t ws_elems, hipStream_t st);
+size_t lz_ws_elems(int64_t n);
+
+}  // namespace rcg

@@ -1,0 +1,734 @@
+// rcg_kernels.hip -- CDNA4 (gfx950, wave64) kernels of the Rayleigh-Chebyshev solve.
+//
+// Replaces LanczosCpolyOperatorLMCuda::apply (CPO-LMC:194-265: copy + SpMM +
+// subtract per degree), the cuda_kernels.cu ops (scale, substract, normalize,
+// residuals; CK:9-382), the cuBLAS projections (LMC:2139-2161, 2200-2222) and
+// cuSOLVER QR (LMC:1979-2107) with kernels written for this chip:
+//  * csr_block_kernel: one fused three-term Chebyshev step per degree.  A row
+//    of the block (ld elements, 16-byte vectors) is owned by G lanes, so each
+//    CSR gather is a single contiguous read of that row; launches are remapped
+//    so that each XCD's L2 streams a contiguous range of rows.
+//  * gram_partial_kernel / tall_gemm_kernel: LDS-tiled tall-skinny products
+//    with 4x4 register tiles per thread; partial sums are reduced in a fixed
+//    order so results do not depend on scheduling.
+//  * column reductions (norms, residuals) and the Lanczos vector kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "rcg_internal.hpp"
+
+namespace rcg {
+namespace {
+
+constexpr int kXcds = 8;
+constexpr int kThreads = 256;
+
+// Logical block for physical block `bid` of `nb` (nb a multiple of 8).  The
+// hardware deals workgroups round-robin to the 8 XCDs; this mapping gives
+// XCD x the logical blocks [x*nb/8, (x+1)*nb/8), i.e. contiguous rows.
+__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
+  const int64_t per = nb / kXcds;
+  return (bid % kXcds) * per + bid / kXcds;
+}
+
+// 16-byte vector access to a row segment.
+template <typename T>
+struct V16;
+template <>
+struct V16<double> {
+  static constexpr int W = 2;
+  __device__ static void ld(const double* p, double (&o)[2]) {
+    const double2 t = *reinterpret_cast<const double2*>(p);
+    o[0] = t.x;
+    o[1] = t.y;
+  }
+  __device__ static void st(double* p, const double (&o)[2]) {
+    *reinterpret_cast<double2*>(p) = make_double2(o[0], o[1]);
+  }
+};
+template <>
+struct V16<float> {
+  static constexpr int W = 4;
+  __device__ static void ld(const float* p, float (&o)[4]) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    o[0] = t.x;
+    o[1] = t.y;
+    o[2] = t.z;
+    o[3] = t.w;
+  }
+  __device__ static void st(float* p, const float (&o)[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+  }
+};
+
+// ------------------------------------------------------------ CSR x block
+enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3 };
+
+// G lanes own one row; each lane handles NV 16-byte column vectors at
+// cv = g + k*G.  256/G rows are in flight per workgroup.
+template <typename T, int G, int NV, int MODE>
+__global__ __launch_bounds__(kThreads) void csr_block_kernel(
+    int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
+    T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk) {
+  using VT = V16<T>;
+  constexpr int W = VT::W;
+  constexpr int RS = kThreads / G;
+  const int64_t lb = xcd_remap(blockIdx.x, gridDim.x);
+  if (lb >= nblk) return;
+  const int g = threadIdx.x % G;
+  const int r = threadIdx.x / G;
+  const int64_t i0 = lb * rows_per_blk;
+  const int64_t i1 = min(n, i0 + rows_per_blk);
+  for (int64_t i = i0 + r; i < i1; i += RS) {
+    T acc[NV][W];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int w = 0; w < W; ++w) acc[k][w] = T(0);
+    const int pb = rowptr[i];
+    const int pe = rowptr[i + 1];
+    int p = pb;
+    for (; p + 1 < pe; p += 2) {
+      const int j0 = col[p];
+      const int j1 = col[p + 1];
+      const T a0 = val[p];
+      const T a1 = val[p + 1];
+      const T* x0 = X + (int64_t)j0 * ld;
+      const T* x1 = X + (int64_t)j1 * ld;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const int cv = g + k * G;
+        if (cv < ncv) {
+          T u0[W], u1[W];
+          VT::ld(x0 + cv * W, u0);
+          VT::ld(x1 + cv * W, u1);
+#pragma unroll
+          for (int w = 0; w < W; ++w) acc[k][w] = fma(a1, u1[w], fma(a0, u0[w], acc[k][w]));
+        }
+      }
+    }
+    if (p < pe) {
+      const int j0 = col[p];
+      const T a0 = val[p];
+      const T* x0 = X + (int64_t)j0 * ld;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const int cv = g + k * G;
+        if (cv < ncv) {
+          T u0[W];
+          VT::ld(x0 + cv * W, u0);
+#pragma unroll
+          for (int w = 0; w < W; ++w) acc[k][w] = fma(a0, u0[w], acc[k][w]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int cv = g + k * G;
+      if (cv >= ncv) continue;
+      const int64_t off = i * (int64_t)ld + cv * W;
+      T o[W];
+      if constexpr (MODE == kApply) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) o[w] = p0 * acc[k][w];
+      } else if constexpr (MODE == kApplyBeta) {
+        T y[W];
+        VT::ld(out + off, y);
+#pragma unroll
+        for (int w = 0; w < W; ++w) o[w] = fma(p0, acc[k][w], p1 * y[w]);
+      } else if constexpr (MODE == kChebFirst) {
+        T xs[W];
+        VT::ld(X + off, xs);
+#pragma unroll
+        for (int w = 0; w < W; ++w) o[w] = p2 * (p1 * fma(p0, acc[k][w], xs[w]));
+      } else {
+        T xs[W], zs[W];
+        VT::ld(X + off, xs);
+        VT::ld(Z + off, zs);
+#pragma unroll
+        for (int w = 0; w < W; ++w) o[w] = p2 * fma(p1, fma(p0, acc[k][w], xs[w]), -zs[w]);
+      }
+      VT::st(out + off, o);
+    }
+  }
+}
+
+template <typename T, int MODE, int G, int NV>
+void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
+                  T* out, int ld, int ncv, T p0, T p1, T p2, hipStream_t st) {
+  constexpr int RS = kThreads / G;
+  // ~16 row-passes per workgroup keeps the grid >> 256 CUs at large n.
+  int64_t rows_per_blk = (int64_t)RS * 16;
+  int64_t nblk = (n + rows_per_blk - 1) / rows_per_blk;
+  if (nblk < 2048) {
+    rows_per_blk = RS;
+    nblk = (n + rows_per_blk - 1) / rows_per_blk;
+  }
+  const int64_t grid = round_up(nblk, kXcds);
+  hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE>), dim3((unsigned)grid), dim3(kThreads), 0,
+                     st, n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, rows_per_blk, nblk);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T, int MODE>
+void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
+                T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st) {
+  constexpr int W = V16<T>::W;
+  const int ncv = (ncols + W - 1) / W;
+  if (ld % 4 != 0) throw Error(1, "block leading dimension must be a multiple of 4");
+  if (ncv <= 1) return csr_launch_g<T, MODE, 1, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  if (ncv <= 2) return csr_launch_g<T, MODE, 2, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  if (ncv <= 4) return csr_launch_g<T, MODE, 4, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  if (ncv <= 8) return csr_launch_g<T, MODE, 8, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  if (ncv <= 16) return csr_launch_g<T, MODE, 16, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  if (ncv <= 32) return csr_launch_g<T, MODE, 32, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  if (ncv <= 64) return csr_launch_g<T, MODE, 64, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  if (ncv <= 128) return csr_launch_g<T, MODE, 64, 2>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  if (ncv <= 256) return csr_launch_g<T, MODE, 64, 4>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+  throw Error(6, "CSR block kernels support at most 256 16-byte column vectors per row");
+}
+
+// ------------------------------------------------------ tall-skinny Gram
+// Partial X^H Y over a row range for one 64x64 output tile.
+constexpr int kTile = 64;
+constexpr int kRb = 32;  // rows staged per LDS round
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gram_partial_kernel(
+    int64_t n, const T* __restrict__ X, int ldx, int bx, const T* __restrict__ Y, int ldy, int by,
+    T* __restrict__ P, int64_t rows_per_blk, int ntb) {
+  __shared__ T xs[kRb][kTile + 1];
+  __shared__ T ys[kRb][kTile + 1];
+  const int tile = blockIdx.y;
+  const int ta = tile % ((bx + kTile - 1) / kTile);
+  const int tb = tile / ((bx + kTile - 1) / kTile);
+  const int a0 = ta * kTile;
+  const int b0 = tb * kTile;
+  const int tx = threadIdx.x % 16;
+  const int ty = threadIdx.x / 16;
+  const int64_t blk = blockIdx.x;
+  const int64_t i0 = blk * rows_per_blk;
+  const int64_t i1 = min(n, i0 + rows_per_blk);
+  T acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = T(0);
+  for (int64_t r0 = i0; r0 < i1; r0 += kRb) {
+#pragma unroll
+    for (int e = threadIdx.x; e < kRb * kTile; e += kThreads) {
+      const int rr = e / kTile;
+      const int cc = e % kTile;
+      const int64_t i = r0 + rr;
+      const bool rowok = i < i1;
+      xs[rr][cc] = (rowok && a0 + cc < bx) ? X[i * ldx + a0 + cc] : T(0);
+      ys[rr][cc] = (rowok && b0 + cc < by) ? Y[i * ldy + b0 + cc] : T(0);
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < kRb; ++rr) {
+      T xa[4], yb[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) xa[p] = xs[rr][tx * 4 + p];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) yb[q] = ys[rr][ty * 4 + q];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = fma(xa[p], yb[q], acc[p][q]);
+    }
+    __syncthreads();
+  }
+  T* Pb = P + blk * (int64_t)bx * by;
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int a = a0 + tx * 4 + p;
+      const int b = b0 + ty * 4 + q;
+      if (a < bx && b < by) Pb[a + (int64_t)b * bx] = acc[p][q];
+    }
+}
+
+// out[o] = sum_b P[b*m + o] in block order.
+template <typename T, typename TO>
+__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const T* __restrict__ P,
+                                                                   int64_t nblk, int64_t m,
+                                                                   TO* __restrict__ out) {
+  const int64_t o = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  if (o >= m) return;
+  T s = T(0);
+  for (int64_t b = 0; b < nblk; ++b) s += P[b * m + o];
+  out[o] = TO(s);
+}
+
+template <typename T>
+int64_t gram_blocks(int64_t n, int bx, int by, int64_t* rows_per_blk) {
+  const int ntiles = ((bx + kTile - 1) / kTile) * ((by + kTile - 1) / kTile);
+  int64_t target = std::max<int64_t>(1, 1024 / ntiles);
+  int64_t rpb = round_up((n + target - 1) / target, kRb);
+  rpb = std::max<int64_t>(rpb, kRb);
+  *rows_per_blk = rpb;
+  return (n + rpb - 1) / rpb;
+}
+
+// -------------------------------------------------------- tall x small GEMM
+template <typename T>
+__global__ __launch_bounds__(kThreads) void tall_gemm_kernel(
+    int64_t n, const T* __restrict__ X, int ldx, int bx, const T* __restrict__ C, int ldc, int by,
+    T alpha, T beta, const T* Z, T* out, int ldo, int64_t nrb) {
+  constexpr int KC = 16;
+  __shared__ T xs[kTile][KC + 1];
+  __shared__ T cs[KC][kTile + 1];
+  const int64_t rb = xcd_remap(blockIdx.x, gridDim.x);
+  if (rb >= nrb) return;
+  const int c0 = blockIdx.y * kTile;
+  const int tc = threadIdx.x % 16;
+  const int tr = threadIdx.x / 16;
+  const int64_t i0 = rb * kTile;
+  T acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = T(0);
+  for (int k0 = 0; k0 < bx; k0 += KC) {
+    for (int e = threadIdx.x; e < kTile * KC; e += kThreads) {
+      const int rr = e / KC;
+      const int kk = e % KC;
+      const int64_t i = i0 + rr;
+      xs[rr][kk] = (i < n && k0 + kk < bx) ? X[i * ldx + k0 + kk] : T(0);
+    }
+    for (int e = threadIdx.x; e < KC * kTile; e += kThreads) {
+      const int kk = e / kTile;
+      const int cc = e % kTile;
+      cs[kk][cc] = (k0 + kk < bx && c0 + cc < by) ? C[(k0 + kk) + (int64_t)(c0 + cc) * ldc] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) {
+      T xr[4], cr[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) xr[p] = xs[tr * 4 + p][kk];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cr[q] = cs[kk][tc * 4 + q];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = fma(xr[p], cr[q], acc[p][q]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int64_t i = i0 + tr * 4 + p;
+    if (i >= n) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = c0 + tc * 4 + q;
+      if (c >= by) continue;
+      const int64_t off = i * ldo + c;
+      T v = alpha * acc[p][q];
+      if (beta != T(0)) v = fma(beta, Z[off], v);
+      out[off] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------ column reductions
+// Threads are laid out as (row slot, column): CW columns per pass.
+constexpr int kColRows = 512;  // rows per partial block
+
+template <typename T, bool RESID>
+__global__ __launch_bounds__(kThreads) void col_sumsq_kernel(int64_t n, const T* __restrict__ V,
+                                                             const T* __restrict__ Wm, int ld,
+                                                             int nc, const double* theta,
+                                                             double* __restrict__ P) {
+  __shared__ double red[kThreads];
+  const int64_t blk = blockIdx.x;
+  const int cw = min(nc, kThreads);  // columns per pass
+  const int slots = kThreads / cw;
+  const int c_in = threadIdx.x % cw;
+  const int slot = threadIdx.x / cw;
+  const int64_t i0 = blk * kColRows;
+  const int64_t i1 = min(n, i0 + kColRows);
+  for (int cbase = 0; cbase < nc; cbase += cw) {
+    const int c = cbase + c_in;
+    double s = 0.0;
+    if (slot < slots && c < nc) {
+      const double th = RESID ? theta[c] : 0.0;
+      for (int64_t i = i0 + slot; i < i1; i += slots) {
+        double v = double(V[i * ld + c]);
+        if (RESID) v = th * v - double(Wm[i * ld + c]);
+        s += v * v;
+      }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (slot == 0 && c < nc) {
+      double t = red[threadIdx.x];
+      for (int q = 1; q < slots; ++q) t += red[threadIdx.x + q * cw];
+      P[blk * nc + c] = t;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void normalize_cols_kernel(int64_t n, T* X, int ld, int nc,
+                                                                  const double* __restrict__ sums) {
+  const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  const int64_t i = e / nc;
+  const int c = (int)(e % nc);
+  if (i >= n) return;
+  const double s = 1.0 / sqrt(fabs(sums[c]));
+  X[i * ld + c] = T(double(X[i * ld + c]) * s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void copy_cols_kernel(int64_t n, const T* __restrict__ src,
+                                                             int lds, int s0, T* __restrict__ dst,
+                                                             int ldd, int d0, int cnt) {
+  const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  const int64_t i = e / cnt;
+  const int j = (int)(e % cnt);
+  if (i >= n) return;
+  dst[i * ldd + d0 + j] = src[i * lds + s0 + j];
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double sym_draw(uint64_t key, uint64_t idx) {
+  const uint64_t z = mix64(key + (idx + 1) * 0x9E3779B97F4A7C15ULL);
+  return -1.0 + 2.0 * (double(z >> 11) * 0x1.0p-53);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void randomize_cols_kernel(int64_t n, T* X, int ld, int c0,
+                                                                  int cnt, uint64_t key,
+                                                                  uint64_t counter) {
+  // thread per (row, column): column-fastest so stores are coalesced.
+  const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  const int64_t i = e / cnt;
+  const int j = (int)(e % cnt);
+  if (i >= n) return;
+  X[i * ld + c0 + j] = T(sym_draw(key, counter + (uint64_t)j * (uint64_t)n + (uint64_t)i));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void fill_cols_kernel(int64_t n, T* X, int ld, int c0,
+                                                             int cnt, T value) {
+  const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  const int64_t i = e / cnt;
+  const int j = (int)(e % cnt);
+  if (i >= n) return;
+  X[i * ld + c0 + j] = value;
+}
+
+// ----------------------------------------------------------------- Lanczos
+constexpr int kLzRows = 2048;  // rows per partial block
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// v = v/||v||, w = v, v = 0 (LanczosMaxMinFinder.h:272-279) -- two phases.
+__global__ __launch_bounds__(kThreads) void lz_sumsq_kernel(int64_t n, const double* v, double* P) {
+  __shared__ double red[kThreads];
+  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
+  const int64_t i1 = min(n, i0 + kLzRows);
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s += v[i] * v[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) P[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(kThreads) void lz_finalize_kernel(const double* P, int64_t nblk,
+                                                               double* out, int take_sqrt) {
+  __shared__ double red[kThreads];
+  // fixed-order: each thread sums a strided subset, then a tree.
+  double s = 0.0;
+  for (int64_t b = threadIdx.x; b < nblk; b += kThreads) s += P[b];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) *out = take_sqrt ? sqrt(fabs(s)) : s;
+}
+__global__ __launch_bounds__(kThreads) void lz_init_kernel(int64_t n, double* v, double* w,
+                                                           const double* nrm) {
+  const int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  if (i >= n) return;
+  const double s = 1.0 / *nrm;
+  w[i] = v[i] * s;
+  v[i] = 0.0;
+}
+// if |beta| > 1e-14: w <- v/beta, v <- -beta*w_old (LanczosMaxMinFinder.h:329-339)
+__global__ __launch_bounds__(kThreads) void lz_swap_kernel(int64_t n, double* v, double* w,
+                                                           const double* beta_prev) {
+  const double b = *beta_prev;
+  if (!(fabs(b) > 1.0e-14)) return;
+  const int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  if (i >= n) return;
+  const double t = w[i];
+  w[i] = v[i] * (1.0 / b);
+  v[i] = t * (-b);
+}
+// v += A w; partial dot(w, v)
+__global__ __launch_bounds__(kThreads) void lz_apply_dot_kernel(int64_t n,
+                                                                const int* __restrict__ rowptr,
+                                                                const int* __restrict__ col,
+                                                                const double* __restrict__ val,
+                                                                const double* __restrict__ w,
+                                                                double* v, double* P) {
+  __shared__ double red[kThreads];
+  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
+  const int64_t i1 = min(n, i0 + kLzRows);
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    double a = 0.0;
+    for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) a = fma(val[p], w[col[p]], a);
+    const double vn = v[i] + a;
+    v[i] = vn;
+    s = fma(w[i], vn, s);
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) P[blockIdx.x] = s;
+}
+// v += (-alpha) w; partial |v|^2
+__global__ __launch_bounds__(kThreads) void lz_axpy_norm_kernel(int64_t n, const double* w,
+                                                                double* v, const double* alpha,
+                                                                double* P) {
+  __shared__ double red[kThreads];
+  const double a = -*alpha;
+  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
+  const int64_t i1 = min(n, i0 + kLzRows);
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const double vn = fma(w[i], a, v[i]);
+    v[i] = vn;
+    s = fma(vn, vn, s);
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) P[blockIdx.x] = s;
+}
+
+inline unsigned grid_for(int64_t work) {
+  return (unsigned)((work + kThreads - 1) / kThreads);
+}
+
+}  // namespace
+
+// ============================================================== launchers
+template <typename T>
+void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val, const T* X,
+                      T* out, int ld, int ncols, T alpha, T beta, hipStream_t st) {
+  if (n <= 0 || ncols <= 0) return;
+  if (beta == T(0))
+    csr_launch<T, kApply>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, T(0), T(0), st);
+  else
+    csr_launch<T, kApplyBeta>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, beta, T(0), st);
+}
+
+template <typename T>
+void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val, const T* X1,
+                      const T* X0, T* out, int ld, int ncols, T g1, T g2, T s, bool first,
+                      hipStream_t st) {
+  if (n <= 0 || ncols <= 0) return;
+  if (first)
+    csr_launch<T, kChebFirst>(n, rowptr, col, val, X1, nullptr, out, ld, ncols, g1, g2, s, st);
+  else
+    csr_launch<T, kCheb>(n, rowptr, col, val, X1, X0, out, ld, ncols, g1, g2, s, st);
+}
+
+template <typename T>
+size_t gram_ws_elems(int64_t n, int bx, int by) {
+  int64_t rpb;
+  const int64_t nblk = gram_blocks<T>(n, bx, by, &rpb);
+  return (size_t)nblk * bx * by;
+}
+
+// Upper bound of gram_ws_elems over every bx, by <= bmax: a launch has at
+// most 1024/ntiles + 1 row blocks (and at most ceil(n/kRb)), each holding at
+// most ntiles 64x64 tiles.
+size_t gram_ws_bound(int64_t n, int bmax) {
+  const int64_t t = (bmax + kTile - 1) / kTile;
+  const int64_t by_tiles = (1024 + t * t) * (int64_t)kTile * kTile;
+  const int64_t by_rows = ((n + kRb - 1) / kRb) * (int64_t)bmax * bmax;
+  return (size_t)std::max<int64_t>(1, std::min(by_tiles, by_rows));
+}
+
+template <typename T>
+void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, int by, T* G,
+                 T* ws, size_t ws_elems, hipStream_t st) {
+  if (bx <= 0 || by <= 0) return;
+  if (n <= 0) {
+    RCG_HIP(hipMemsetAsync(G, 0, sizeof(T) * bx * by, st));
+    return;
+  }
+  int64_t rpb;
+  const int64_t nblk = gram_blocks<T>(n, bx, by, &rpb);
+  if ((size_t)nblk * bx * by > ws_elems) throw Error(3, "gram workspace too small");
+  const int ntiles = ((bx + kTile - 1) / kTile) * ((by + kTile - 1) / kTile);
+  hipLaunchKernelGGL(gram_partial_kernel<T>, dim3((unsigned)nblk, ntiles), dim3(kThreads), 0, st, n,
+                     X, ldx, bx, Y, ldy, by, ws, rpb, ntiles);
+  RCG_HIP(hipGetLastError());
+  const int64_t m = (int64_t)bx * by;
+  hipLaunchKernelGGL((reduce_partials_kernel<T, T>), dim3(grid_for(m)), dim3(kThreads), 0, st, ws,
+                     nblk, m, G);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_tall_gemm(int64_t n, const T* X, int ldx, int bx, const T* C, int ldc, int by,
+                      T alpha, T beta, const T* Z, T* out, int ldo, hipStream_t st) {
+  if (n <= 0 || by <= 0) return;
+  const int64_t nrb = (n + kTile - 1) / kTile;
+  const int64_t grid = round_up(nrb, kXcds);
+  const unsigned ct = (unsigned)((by + kTile - 1) / kTile);
+  hipLaunchKernelGGL(tall_gemm_kernel<T>, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
+                     ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);
+  RCG_HIP(hipGetLastError());
+}
+
+size_t colred_ws_elems(int64_t n, int nc) {
+  return (size_t)((n + kColRows - 1) / kColRows) * (size_t)std::max(nc, 1);
+}
+
+template <typename T>
+void launch_col_sumsq(int64_t n, const T* X, int ld, int nc, double* sums, double* ws,
+                      size_t ws_elems, hipStream_t st) {
+  if (nc <= 0) return;
+  const int64_t nblk = (n + kColRows - 1) / kColRows;
+  if ((size_t)(nblk * nc) > ws_elems) throw Error(3, "column-reduction workspace too small");
+  hipLaunchKernelGGL((col_sumsq_kernel<T, false>), dim3((unsigned)nblk), dim3(kThreads), 0, st, n,
+                     X, (const T*)nullptr, ld, nc, (const double*)nullptr, ws);
+  RCG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(grid_for(nc)), dim3(kThreads),
+                     0, st, ws, nblk, (int64_t)nc, sums);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_residual_sumsq(int64_t n, const T* V, const T* W, int ld, int nc, const double* theta,
+                           double* sums, double* ws, size_t ws_elems, hipStream_t st) {
+  if (nc <= 0) return;
+  const int64_t nblk = (n + kColRows - 1) / kColRows;
+  if ((size_t)(nblk * nc) > ws_elems) throw Error(3, "column-reduction workspace too small");
+  hipLaunchKernelGGL((col_sumsq_kernel<T, true>), dim3((unsigned)nblk), dim3(kThreads), 0, st, n, V,
+                     W, ld, nc, theta, ws);
+  RCG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(grid_for(nc)), dim3(kThreads),
+                     0, st, ws, nblk, (int64_t)nc, sums);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_normalize_cols(int64_t n, T* X, int ld, int nc, const double* sums, hipStream_t st) {
+  if (n <= 0 || nc <= 0) return;
+  hipLaunchKernelGGL(normalize_cols_kernel<T>, dim3(grid_for(n * nc)), dim3(kThreads), 0, st, n, X,
+                     ld, nc, sums);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_copy_cols(int64_t n, const T* src, int lds, int s0, T* dst, int ldd, int d0, int cnt,
+                      hipStream_t st) {
+  if (n <= 0 || cnt <= 0) return;
+  hipLaunchKernelGGL(copy_cols_kernel<T>, dim3(grid_for(n * cnt)), dim3(kThreads), 0, st, n, src,
+                     lds, s0, dst, ldd, d0, cnt);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_randomize_cols(int64_t n, T* X, int ld, int c0, int cnt, uint64_t seed,
+                           uint64_t counter, hipStream_t st) {
+  if (n <= 0 || cnt <= 0) return;
+  const uint64_t key = [&] {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }();
+  hipLaunchKernelGGL(randomize_cols_kernel<T>, dim3(grid_for(n * cnt)), dim3(kThreads), 0, st, n,
+                     X, ld, c0, cnt, key, counter);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_fill_cols(int64_t n, T* X, int ld, int c0, int cnt, T value, hipStream_t st) {
+  if (n <= 0 || cnt <= 0) return;
+  hipLaunchKernelGGL(fill_cols_kernel<T>, dim3(grid_for(n * cnt)), dim3(kThreads), 0, st, n, X, ld,
+                     c0, cnt, value);
+  RCG_HIP(hipGetLastError());
+}
+
+size_t lz_ws_elems(int64_t n) { return (size_t)((n + kLzRows - 1) / kLzRows) + 4; }
+
+void launch_lz_start(int64_t n, double* v, double* w, double* ws, size_t ws_elems,
+                     double* scratch, hipStream_t st) {
+  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
+  hipLaunchKernelGGL(lz_sumsq_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, v, ws);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, scratch, 1);
+  hipLaunchKernelGGL(lz_init_kernel, dim3(grid_for(n)), dim3(kThreads), 0, st, n, v, w, scratch);
+  RCG_HIP(hipGetLastError());
+}
+void launch_lz_swap(int64_t n, double* v, double* w, const double* beta_prev, hipStream_t st) {
+  hipLaunchKernelGGL(lz_swap_kernel, dim3(grid_for(n)), dim3(kThreads), 0, st, n, v, w, beta_prev);
+  RCG_HIP(hipGetLastError());
+}
+void launch_lz_apply_dot(int64_t n, const int* rowptr, const int* col, const double* val,
+                         const double* w, double* v, double* alpha_k, double* ws, size_t ws_elems,
+                         hipStream_t st) {
+  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
+  hipLaunchKernelGGL(lz_apply_dot_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, rowptr,
+                     col, val, w, v, ws);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, alpha_k, 0);
+  RCG_HIP(hipGetLastError());
+}
+void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* alpha_k,
+                         double* beta_k, double* ws, size_t ws_elems, hipStream_t st) {
+  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
+  hipLaunchKernelGGL(lz_axpy_norm_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, w, v,
+                     alpha_k, ws);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, beta_k, 1);
+  RCG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------- instantiations
+#define RCG_INST(T)                                                                              \
+  template void launch_csr_apply<T>(int64_t, const int*, const int*, const T*, const T*, T*, int, \
+                                    int, T, T, hipStream_t);                                      \
+  template void launch_cheb_step<T>(int64_t, const int*, const int*, const T*, const T*,          \
+                                    const T*, T*, int, int, T, T, T, bool, hipStream_t);          \
+  template size_t gram_ws_elems<T>(int64_t, int, int);                                           \
+  template void launch_gram<T>(int64_t, const T*, int, int, const T*, int, int, T*, T*, size_t,   \
+                               hipStream_t);                                                      \
+  template void launch_tall_gemm<T>(int64_t, const T*, int, int, const T*, int, int, T, T,        \
+                                    const T*, T*, int, hipStream_t);                              \
+  template void launch_col_sumsq<T>(int64_t, const T*, int, int, double*, double*, size_t,        \
+                                    hipStream_t);                                                 \
+  template void launch_residual_sumsq<T>(int64_t, const T*, const T*, int, int, const double*,    \
+                                         double*, double*, size_t, hipStream_t);                  \
+  template void launch_normalize_cols<T>(int64_t, T*, int, int, const double*, hipStream_t);      \
+  template void launch_copy_cols<T>(int64_t, const T*, int, int, T*, int, int, int, hipStream_t); \
+  template void launch_randomize_cols<T>(int64_t, T*, int, int, int, uint64_t, uint64_t,          \
+                                         hipStream_t);                                            \
+  template void launch_fill_cols<T>(int64_t, T*, int, int, int, T, hipStream_t);
+
+RCG_INST(double)
+RCG_INST(float)
+
+}  // namespace rcg

@@ -1,0 +1,1243 @@
+// rcg_solver.cpp -- handles, the device engine, the Rayleigh-Chebyshev state
+// machine and the extern "C" entry points of librcg.so.
+//
+// The state machine is RayleighChebyshevLM::getMinIntervalEigenSystem_Base
+// (RayleighChebyshevLM.h:817-1776): the same shift/degree rules, stop
+// conditions, locking, guard and shrink logic, driven over device blocks.
+// What runs where:
+//   device : Lanczos vectors, Chebyshev filter steps, Gram matrices, block
+//            rotations and deflation, norms and residuals, start vectors;
+//   host   : scalars of the state machine, the b x b Cholesky and Jacobi.
+// One stream per context; the host waits once per Gram/projection result.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/rcg_c_api.h"
+#include "rcg_cpoly.hpp"
+#include "rcg_dense.hpp"
+#include "rcg_internal.hpp"
+
+using rcg::Error;
+
+// ------------------------------------------------------------------ memory
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw Error(RCG_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(b) +
+                                     " bytes failed: " + hipGetErrorString(e));
+    }
+    bytes = b;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    RCG_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+    bytes = b;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// ----------------------------------------------------------------- handles
+struct rcg_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  DevBuf ws;      // reduction partials
+  DevBuf small;   // b x b matrices and b-vectors on the device
+  HostBuf pinned; // host staging of the same
+};
+
+struct rcg_block_s {
+  rcg_context_s* ctx = nullptr;
+  int64_t n = 0, cols = 0, ld = 0;
+  int dtype = rcg::F64;
+  DevBuf mem;
+  void* d() const { return mem.p; }
+};
+
+struct rcg_operator_s {
+  rcg_context_s* ctx = nullptr;
+  int64_t n = 0, nnz = 0;
+  int dtype = rcg::F64;
+  const int* rowptr = nullptr;
+  const int* col = nullptr;
+  const double* val = nullptr;
+  DevBuf own_rowptr, own_col, own_val;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return RCG_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return RCG_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return RCG_ERR_INVALID;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) throw Error(RCG_ERR_INVALID, msg);
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    RCG_HIP(hipGetDevice(&prev));
+    if (prev != dev) RCG_HIP(hipSetDevice(dev));
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+int64_t block_ld(int64_t cols) { return std::max<int64_t>(4, rcg::round_up(cols, 4)); }
+
+double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------------ engine
+// Device operations of the solve over attached row-interleaved buffers:
+// V (the subspace, ld), T (second recurrence / rotation buffer, ld),
+// W (A V, ld) and Y (found vectors, ldy).
+class Engine {
+ public:
+  Engine(rcg_context_s* ctx, const rcg_operator_s* op, int64_t n) : ctx_(ctx), op_(op), n_(n) {
+    st_ = ctx->stream;
+  }
+
+  // Size the context's workspaces for widths up to bmax and kmax.
+  void reserve(int64_t bmax, int64_t kmax) {
+    bb_ = std::max<int64_t>({bmax, kmax, 4});
+    size_t e = rcg::gram_ws_bound(n_, (int)bb_);
+    e = std::max(e, rcg::colred_ws_elems(n_, (int)bb_));
+    e = std::max(e, rcg::lz_ws_elems(n_));
+    ctx_->ws.ensure(e * sizeof(double));
+    ws_elems_ = ctx_->ws.bytes / sizeof(double);
+    const size_t sm = sizeof(double) * (size_t)(2 * bb_ * bb_ + 4 * bb_ + 64);
+    ctx_->small.ensure(sm);
+    ctx_->pinned.ensure(sm);
+  }
+  void attach(double* V, double* T, double* W, int64_t ld, double* Y, int64_t ldy) {
+    buf_[0] = V;
+    buf_[1] = T;
+    iv_ = 0;
+    W_ = W;
+    ld_ = ld;
+    Y_ = Y;
+    ldy_ = ldy;
+  }
+
+  double* V() const { return buf_[iv_]; }
+  double* T() const { return buf_[1 - iv_]; }
+  double* W() const { return W_; }
+  double* Y() const { return Y_; }
+  bool v_is_first() const { return iv_ == 0; }
+  void swapVT() { iv_ = 1 - iv_; }
+  int64_t ld() const { return ld_; }
+  int64_t ldy() const { return ldy_; }
+  hipStream_t stream() const { return st_; }
+  void sync() { RCG_HIP(hipStreamSynchronize(st_)); }
+
+  // Scratch layout (device `small` and host `pinned` alike):
+  //   [0, bb^2)          matrix A (Gram results, uploaded factors)
+  //   [bb^2, 2bb^2)      matrix B
+  //   [2bb^2, +bb)       vector: column sums
+  //   [2bb^2+bb, +bb)    vector: uploaded eigenvalues
+  //   [2bb^2+2bb, +bb)   vector: downloaded sums
+  double* dmatA() { return ctx_->small.as<double>(); }
+  double* dsums() { return ctx_->small.as<double>() + 2 * bb_ * bb_; }
+  double* dtheta() { return dsums() + bb_; }
+  double* hmatA() { return ctx_->pinned.as<double>(); }
+  double* hmatB() { return ctx_->pinned.as<double>() + bb_ * bb_; }
+  double* hvecUp() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + bb_; }
+  double* hvecDown() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + 2 * bb_; }
+  double* hlz() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + 3 * bb_; }
+  double* ws() { return ctx_->ws.as<double>(); }
+
+  // V <- P_m(A')^rep V over b columns (LanczosCpolyOperatorLM.h:192-304):
+  // X1 = g2(g1 A X0 + X0), Xj = g2(g1 A X_{j-1} + X_{j-1}) - X_{j-2}, out = Xm/(m+1).
+  void filter(int64_t b, long m, long rep, double lmax, double shift) {
+    const auto c = rcg::cpoly::coefficients(m, lmax, shift);
+    for (long r = 0; r < rep; ++r) {
+      double* X0 = V();
+      double* X1 = T();
+      const double s1 = (m == 1) ? 0.5 : 1.0;
+      rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, X0, nullptr, X1,
+                                    (int)ld_, (int)b, c.g1, c.g2, s1, true, st_);
+      for (long j = 2; j <= m; ++j) {
+        const double s = (j == m) ? 1.0 / double(m + 1) : 1.0;
+        rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, X1, X0, X0,
+                                      (int)ld_, (int)b, c.g1, c.g2, s, false, st_);
+        std::swap(X0, X1);
+      }
+      if (X1 != V()) swapVT();
+    }
+  }
+
+  // host[bx x by] = X^H Y (column-major).
+  void gram_to_host(const double* X, int64_t ldx, int64_t bx, const double* Yp, int64_t ldy,
+                    int64_t by, double* host) {
+    rcg::launch_gram<double>(n_, X, (int)ldx, (int)bx, Yp, (int)ldy, (int)by, dmatA(), ws(),
+                             ws_elems_, st_);
+    RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * bx * by, hipMemcpyDeviceToHost, st_));
+    sync();
+    std::memcpy(host, hmatA(), sizeof(double) * bx * by);
+  }
+
+  // V <- V C, C host b x b column-major; the result lands in T, then swaps.
+  void right_multiply(int64_t b, const double* C) {
+    std::memcpy(hmatB(), C, sizeof(double) * b * b);
+    RCG_HIP(hipMemcpyAsync(dmatA(), hmatB(), sizeof(double) * b * b, hipMemcpyHostToDevice, st_));
+    rcg::launch_tall_gemm<double>(n_, V(), (int)ld_, (int)b, dmatA(), (int)b, (int)b, 1.0, 0.0,
+                                  nullptr, T(), (int)ld_, st_);
+    swapVT();
+  }
+
+  // One CholQR pass on V[:, :b]: G = V^T V scaled to unit diagonal,
+  // G = R^T R, V <- V D R^{-1}.  A failed factorisation retries with the
+  // shift 11(nb + b(b+1)) u ||G|| (shifted CholeskyQR); returns false then.
+  bool cholqr_pass(int64_t b, bool allow_shift) {
+    std::vector<double> G(b * b);
+    gram_to_host(V(), ld_, b, V(), ld_, b, G.data());
+    std::vector<double> d(b);
+    for (int64_t j = 0; j < b; ++j) {
+      const double g = G[j + j * b];
+      d[j] = (g > 0.0 && std::isfinite(g)) ? 1.0 / std::sqrt(g) : 1.0;
+    }
+    for (int64_t j = 0; j < b; ++j)
+      for (int64_t i = 0; i < b; ++i) G[i + j * b] *= d[i] * d[j];
+    std::vector<double> R, Ri;
+    bool clean = rcg::dense::cholesky_upper(G.data(), b, R);
+    if (!clean) {
+      if (!allow_shift) throw Error(RCG_ERR_NUMERIC, "CholQR breakdown after a shifted pass");
+      double gn = 0.0;
+      for (double x : G) gn += x * x;
+      const double s = 11.0 * (double(n_) * b + double(b) * (b + 1)) * 1.1102230246251565e-16 * std::sqrt(gn);
+      for (int64_t j = 0; j < b; ++j) G[j + j * b] += s;
+      if (!rcg::dense::cholesky_upper(G.data(), b, R))
+        throw Error(RCG_ERR_NUMERIC, "shifted CholQR breakdown (block is numerically rank deficient)");
+    }
+    rcg::dense::upper_inverse(R, b, Ri);
+    for (int64_t j = 0; j < b; ++j)
+      for (int64_t i = 0; i < b; ++i) Ri[i + j * b] *= d[i];
+    right_multiply(b, Ri.data());
+    return clean;
+  }
+
+  // orthogonalize (RayleighChebyshevLM.h:1782-1837): CholQR2, plus a third
+  // pass when the first needed a shift.
+  void orthonormalize(int64_t b) {
+    if (b <= 0) return;
+    const bool clean = cholqr_pass(b, true);
+    cholqr_pass(b, true);
+    if (!clean) cholqr_pass(b, false);
+  }
+
+  // OrthogonalizeAtoB (RayleighChebyshevLM.h:1844-1917) against Y[:, :found]:
+  // V -= Y (Y^T V), twice.  The normalisation of the reference is implied by
+  // the CholQR that follows in the solve.
+  void deflate(int64_t b, int64_t found) {
+    if (found <= 0 || b <= 0) return;
+    for (int pass = 0; pass < 2; ++pass) {
+      rcg::launch_gram<double>(n_, Y_, (int)ldy_, (int)found, V(), (int)ld_, (int)b, dmatA(),
+                               ws(), ws_elems_, st_);
+      rcg::launch_tall_gemm<double>(n_, Y_, (int)ldy_, (int)found, dmatA(), (int)found, (int)b,
+                                    -1.0, 1.0, V(), V(), (int)ld_, st_);
+    }
+  }
+
+  void normalize(int64_t b) {
+    rcg::launch_col_sumsq<double>(n_, V(), (int)ld_, (int)b, dsums(), ws(), ws_elems_, st_);
+    rcg::launch_normalize_cols<double>(n_, V(), (int)ld_, (int)b, dsums(), st_);
+  }
+
+  // formVtAV (RayleighChebyshevLM.h:1928-1969): W = A V, H(j,i) = V_j . W_i
+  // for j >= i, mirrored.
+  void project(int64_t b, std::vector<double>& H) {
+    rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, V(), W_, (int)ld_, (int)b,
+                                  1.0, 0.0, st_);
+    H.assign(b * b, 0.0);
+    gram_to_host(V(), ld_, b, W_, ld_, b, H.data());
+    for (int64_t i = 0; i < b; ++i)
+      for (int64_t j = i + 1; j < b; ++j) H[i + j * b] = H[j + i * b];
+  }
+
+  // createEigenVectorsAndResiduals (RayleighChebyshevLM.h:1984-2087):
+  // V <- normalize(V Y); W = A V; r_c = ||theta_c V_c - W_c||, c < count.
+  void rotate_residuals(int64_t b, const std::vector<double>& Yh, const std::vector<double>& th,
+                        int64_t count, std::vector<double>& res) {
+    right_multiply(b, Yh.data());
+    normalize(b);
+    residuals(count, th.data(), res);
+  }
+
+  void residuals(int64_t count, const double* th, std::vector<double>& res) {
+    res.assign(count, 0.0);
+    if (count <= 0) return;
+    rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, V(), W_, (int)ld_,
+                                  (int)count, 1.0, 0.0, st_);
+    std::memcpy(hvecUp(), th, sizeof(double) * count);
+    RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * count, hipMemcpyHostToDevice, st_));
+    rcg::launch_residual_sumsq<double>(n_, V(), W_, (int)ld_, (int)count, dtheta(), dsums(), ws(),
+                                       ws_elems_, st_);
+    RCG_HIP(hipMemcpyAsync(hvecDown(), dsums(), sizeof(double) * count, hipMemcpyDeviceToHost, st_));
+    sync();
+    for (int64_t c = 0; c < count; ++c) res[c] = std::sqrt(std::abs(hvecDown()[c]));
+  }
+
+  // ArandomizeOpClassLMCuda::randomize semantics: column j of the range
+  // takes draws counter + j*n + i.
+  void randomize(int64_t c0, int64_t cnt, uint64_t seed, uint64_t* counter) {
+    rcg::launch_randomize_cols<double>(n_, V(), (int)ld_, (int)c0, (int)cnt, seed, *counter, st_);
+    *counter += (uint64_t)cnt * (uint64_t)n_;
+  }
+
+  // LanczosMaxMinFinder::getMinMaxEigenvalues (LanczosMaxMinFinder.h:248-544)
+  // with the vectors, alpha and beta on the device; every 5 steps the host
+  // bisects the tridiagonal for its extreme eigenvalues.
+  void lanczos(double errorTolerance, bool maxFlag, bool minFlag, long itmax, uint64_t seed,
+               uint64_t* counter, double* mnOut, double* mxOut, long* itOut) {
+    const double smallTol = 1.0e-10;
+    auto relErr = [&](double v, double t) {
+      return (std::abs(v) * t > smallTol) ? std::abs(v) : smallTol / t;
+    };
+    if (errorTolerance > 0.01) errorTolerance = 0.01;
+    const double tol = errorTolerance < smallTol ? smallTol : errorTolerance;
+    DevBuf vecs, ab;
+    vecs.ensure(sizeof(double) * (size_t)n_ * 2);
+    ab.ensure(sizeof(double) * (size_t)(2 * itmax + 8));
+    double* v = vecs.as<double>();
+    double* w = v + n_;
+    double* alpha = ab.as<double>();
+    double* beta = alpha + itmax;
+    double* scratch = beta + itmax;
+    rcg::launch_randomize_cols<double>(n_, v, 1, 0, 1, seed, *counter, st_);
+    *counter += (uint64_t)n_;
+    rcg::launch_lz_start(n_, v, w, ws(), ws_elems_, scratch, st_);
+    std::vector<double> ha(itmax), hb(itmax);
+    double minDiff = 1.0e10, maxDiff = 1.0e10, lmin = 0.0, lmax = 0.0;
+    double rfMax = 9.0, rfMin = 9.0, rateMax = 1.0, rateMin = 1.0;
+    double oldMax = 0.0, oldMin = 0.0, dMaxA = 0.0, dMinA = 0.0, dMaxB = 0.0, dMinB = 0.0;
+    int diffFlag = 0, rateFlag = 0;
+    long k = 0;
+    bool done = false;
+    while (k < itmax && !done) {
+      if (k != 0) rcg::launch_lz_swap(n_, v, w, beta + (k - 1), st_);
+      rcg::launch_lz_apply_dot(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k, ws(),
+                               ws_elems_, st_);
+      rcg::launch_lz_axpy_norm(n_, w, v, alpha + k, beta + k, ws(), ws_elems_, st_);
+      k += 1;
+      if (k % 5 == 0) {
+        RCG_HIP(hipMemcpyAsync(hlz(), alpha + (k - 5), sizeof(double) * 5, hipMemcpyDeviceToHost, st_));
+        RCG_HIP(hipMemcpyAsync(hlz() + 8, beta + (k - 5), sizeof(double) * 5, hipMemcpyDeviceToHost, st_));
+        sync();
+        for (int q = 0; q < 5; ++q) {
+          ha[k - 5 + q] = hlz()[q];
+          hb[k - 5 + q] = hlz()[8 + q];
+        }
+        std::vector<double> D(ha.begin(), ha.begin() + k), E(hb.begin(), hb.begin() + (k - 1));
+        lmax = rcg::dense::tridiag_eigenvalue(D, E, k - 1, 1.0e-14);
+        lmin = rcg::dense::tridiag_eigenvalue(D, E, 0, 1.0e-14);
+        if (diffFlag == 0) {
+          oldMax = lmax;
+          oldMin = lmin;
+          diffFlag = 1;
+        } else {
+          maxDiff = std::abs(lmax - oldMax) / relErr(lmax, tol);
+          minDiff = std::abs(lmin - oldMin) / relErr(lmin, tol);
+          if (rateFlag == 0) {
+            dMaxA = std::abs(lmax - oldMax);
+            dMinA = std::abs(lmin - oldMin);
+            rateFlag = 1;
+          } else {
+            if (maxDiff > 1.0e-09) {
+              dMaxB = std::abs(lmax - oldMax);
+              rateMax = dMaxB / dMaxA;
+            } else {
+              rateMax = 0.95;
+            }
+            if (minDiff > 1.0e-09) {
+              dMinB = std::abs(lmin - oldMin);
+              rateMin = dMinB / dMinA;
+            } else {
+              rateMin = 0.95;
+            }
+            rfMax = 9.0;
+            rfMin = 9.0;
+            if (rateMax < 0.90) rfMax = rateMax / (1.0 - rateMax);
+            if (rateMin < 0.90) rfMin = rateMin / (1.0 - rateMin);
+            dMaxA = dMaxB;
+            dMinA = dMinB;
+          }
+          oldMax = lmax;
+          oldMin = lmin;
+          minDiff *= rfMin;
+          maxDiff *= rfMax;
+        }
+        if (maxFlag && minFlag) done = (minDiff < tol && maxDiff < tol);
+        else if (maxFlag) done = maxDiff < tol;
+        else if (minFlag) done = minDiff < tol;
+      }
+    }
+    sync();
+    if (lmin > lmax) std::swap(lmin, lmax);
+    *mnOut = lmin;
+    *mxOut = lmax;
+    *itOut = k;
+  }
+
+ private:
+  rcg_context_s* ctx_;
+  const rcg_operator_s* op_;
+  int64_t n_;
+  hipStream_t st_;
+  double* buf_[2] = {nullptr, nullptr};
+  int iv_ = 0;
+  double* W_ = nullptr;
+  double* Y_ = nullptr;
+  int64_t ld_ = 0, ldy_ = 0, bb_ = 4;
+  size_t ws_elems_ = 0;
+};
+
+// ------------------------------------------------------------- the solver
+struct TraceRow {
+  double v[10];
+};
+
+class Solver {
+ public:
+  Solver(rcg_operator_s* op, const rcg_solve_params& p) : op_(op), p_(p), eng_(op->ctx, op, op->n) {}
+
+  static double rel(double val, double tol) {
+    return (std::abs(val) * tol > 1.0e-11) ? std::abs(val) : 1.0e-11 / tol;
+  }
+
+  void record(long pass, long it, long deg, double sb, double sh, double ls, double mr, double md,
+              double ratio, double gap) {
+    trace_.push_back({{double(pass), double(it), double(deg), sb, sh, ls, mr, md, ratio, gap}});
+  }
+
+  long run(rcg_solve_result* R, std::vector<double>& eigValues, std::vector<double>& eigRes) {
+    const double t0 = wall();
+    const int64_t n = op_->n;
+    const long N = (long)n;
+    long inc = (long)p_.inc, buf = (long)p_.buf;
+    need(inc >= 0 && buf >= 0 && inc + buf >= 1, "subspace increment + buffer must be >= 1");
+    long subspaceSize = inc + buf;
+    long maxDim = (long)p_.k;
+    if (subspaceSize > N) {
+      if (inc < N) {
+        buf = N - inc;
+        subspaceSize = N;
+      } else {
+        subspaceSize = N;
+        inc = N;
+        buf = 0;
+      }
+      maxDim = N;
+    }
+    const int64_t ld = block_ld(subspaceSize);
+    const int64_t ycols = std::max<long>(std::min<long>(maxDim, N), subspaceSize);
+    const int64_t ldy = block_ld(ycols);
+    const size_t blk = sizeof(double) * (size_t)n * (size_t)ld;
+    Vb_.ensure(blk);
+    Tb_.ensure(blk);
+    Wb_.ensure(blk);
+    Yb_.ensure(sizeof(double) * (size_t)n * (size_t)ldy);
+    hipStream_t st = eng_.stream();
+    RCG_HIP(hipMemsetAsync(Vb_.p, 0, blk, st));
+    RCG_HIP(hipMemsetAsync(Tb_.p, 0, blk, st));
+    RCG_HIP(hipMemsetAsync(Wb_.p, 0, blk, st));
+    eng_.reserve(ld, ldy);
+    eng_.attach(Vb_.as<double>(), Tb_.as<double>(), Wb_.as<double>(), ld, Yb_.as<double>(), ldy);
+
+    const long maxInner = p_.fixed_iteration ? 1 : (p_.max_inner > 0 ? (long)p_.max_inner : 10000);
+    const long degMax = p_.deg_max > 0 ? (long)p_.deg_max : 200;
+    const long rep = p_.rep > 0 ? (long)p_.rep : 1;
+    const int stop = p_.stop == 0 ? RCG_STOP_COMBINATION : p_.stop;
+    const double subspaceTol = p_.tol < 1.0e-11 ? 1.0e-11 : p_.tol;
+
+    // spectral bounds (getMinEigenSystem, LM:712-738; interval, LM:745-782)
+    double minEigValue, maxEigValue, lambdaMax;
+    const double tl0 = wall();
+    if (p_.use_bounds && !p_.interval) {
+      minEigValue = p_.min_eig;
+      maxEigValue = p_.max_eig;
+      lambdaMax = maxEigValue;
+    } else {
+      double mn, mx;
+      long it;
+      eng_.lanczos(p_.maxmin_tol > 0 ? p_.maxmin_tol : 1.0e-6, true, p_.interval != 0,
+                   p_.lanczos_itmax > 0 ? (long)p_.lanczos_itmax : 10000, p_.seed, &counter_, &mn,
+                   &mx, &it);
+      R->lanczos_iters = it;
+      R->min_est = mn;
+      R->max_est = mx;
+      if (p_.interval && p_.lambda_max < mn) {
+        R->found = 0;
+        R->t_lanczos = wall() - tl0;
+        R->t_total = wall() - t0;
+        return 0;
+      }
+      if (std::abs(mx - mn) > 1.0e-12) mx += 0.001 * std::abs(mx - mn);
+      minEigValue = mn;
+      maxEigValue = mx;
+      lambdaMax = p_.interval ? p_.lambda_max : mx;
+    }
+    R->t_lanczos = wall() - tl0;
+    const bool intervalFlag = p_.interval != 0;
+
+    long foundSize = 0;
+    std::vector<double> theta(subspaceSize, 0.0), oldEigs(subspaceSize, 0.0),
+        eigDiffs(subspaceSize, 1.0), oldEigDiffs(subspaceSize, 1.0), subspaceResiduals,
+        residualHistory, H, Yh;
+
+    // start vectors (LM:948-985)
+    if (start_ && start_cols_ > 0) {
+      const long have = std::min<long>((long)start_cols_, subspaceSize);
+      rcg::launch_copy_cols<double>(n, start_, (int)start_ld_, 0, eng_.V(), (int)ld, 0, (int)have, st);
+      if (have < subspaceSize) eng_.randomize(have, subspaceSize - have, p_.seed, &counter_);
+    } else {
+      eng_.randomize(0, subspaceSize, p_.seed, &counter_);
+    }
+
+    if (N == subspaceSize) {  // quick return (LM:1005-1036)
+      eng_.orthonormalize(subspaceSize);
+      eng_.project(subspaceSize, H);
+      rcg::dense::jacobi_eigensystem(H.data(), subspaceSize, theta, Yh);
+      eng_.rotate_residuals(subspaceSize, Yh, theta, subspaceSize, eigRes);
+      rcg::launch_copy_cols<double>(n, eng_.V(), (int)ld, 0, eng_.Y(), (int)ldy, 0, (int)subspaceSize, st);
+      eigValues = theta;
+      found_ = subspaceSize;
+      R->found = subspaceSize;
+      R->t_total = wall() - t0;
+      return subspaceSize;
+    }
+
+    bool completedBasis = false, exitFlag = false;
+    long starDegree = 0, starDegreeSave = 0, pass = 0;
+    double starBound = 0.0, starBoundSave = 0.0, shift = 0.0, maxEigDiff = 0.0,
+           maxResidual = 0.0, maxGap = 0.0, eigDiffRatio = 0.0;
+    long applyCountCumulative = 0;
+    double fmaxres = 0.0, fmaxdiff = 0.0, fmaxgap = 0.0;
+
+    while (!exitFlag) {
+      for (long k = buf; k < subspaceSize; ++k) oldEigs[k] = oldEigs[buf];
+      if (applyCountCumulative > 0) eng_.randomize(buf, subspaceSize - buf, p_.seed, &counter_);
+      double ta = wall();
+      eng_.orthonormalize(subspaceSize);
+      count_ortho_ += 2;
+      t_ortho_ += wall() - ta;
+
+      double lambdaStar = maxEigValue;
+      eigDiffRatio = 1.0;
+      long innerLoopCount = 0;
+      starDegreeSave = 0;
+      starBoundSave = 0.0;
+      double stopCheckValue = subspaceTol + 1.0e10;
+      long applyCount = 0;
+      residualHistory.clear();
+      maxGap = maxResidual = maxEigDiff = 0.0;
+      long oscillationCount = 0;
+
+      while (stopCheckValue > subspaceTol && innerLoopCount < maxInner) {
+        // filter parameters (LM:1135-1206)
+        shift = -minEigValue;
+        starDegreeSave = starDegree;
+        starBoundSave = starBound;
+        starDegree = 0;
+        const double eMin = std::abs(minEigValue - lambdaStar) / rel(minEigValue, subspaceTol);
+        const double eMax = std::abs(maxEigValue - lambdaStar) / rel(maxEigValue, subspaceTol);
+        if (eMin < subspaceTol && eMax < subspaceTol) {
+          starDegree = 1;
+          starBound = maxEigValue;
+        } else if (!(maxEigDiff > subspaceTol) && innerLoopCount > 3 && eigDiffRatio < .2) {
+          starDegree = starDegreeSave;
+          starBound = starBoundSave;
+        }
+        if (starDegree == 0)
+          rcg::cpoly::star_degree_and_bound(shift, maxEigValue, lambdaStar, degMax, &starDegree,
+                                            &starBound);
+        if (starDegree < starDegreeSave) {
+          starDegree = starDegreeSave;
+          starBound = starBoundSave;
+        }
+        ta = wall();
+        if (!completedBasis) {
+          eng_.filter(subspaceSize, starDegree, rep, starBound, shift);
+          degree_sum_ += starDegree * rep;
+        }
+        applyCount += 1;
+        count_apply_ += starDegree * subspaceSize;
+        t_filter_ += wall() - ta;  // enqueue time; the device time lands in the next sync
+
+        ta = wall();
+        eng_.deflate(subspaceSize, foundSize);
+        eng_.orthonormalize(subspaceSize);
+        count_ortho_ += 1;
+        t_ortho_ += wall() - ta;
+
+        ta = wall();
+        eng_.project(subspaceSize, H);
+        count_apply_ += subspaceSize;
+        rcg::dense::jacobi_eigensystem(H.data(), subspaceSize, theta, Yh);
+        const long residualCheckCount = (foundSize + subspaceSize < N) ? inc : subspaceSize;
+        eng_.rotate_residuals(subspaceSize, Yh, theta, residualCheckCount, subspaceResiduals);
+        count_apply_ += residualCheckCount;
+        count_eig_ += 1;
+        t_rr_ += wall() - ta;
+
+        // convergence bookkeeping (LM:1285-1457)
+        maxResidual = 0.0;
+        for (size_t k = 0; k < subspaceResiduals.size(); ++k)
+          maxResidual = std::max(maxResidual, subspaceResiduals[k] / rel(theta[k], subspaceTol));
+        residualHistory.push_back(maxResidual);
+        const long checkSize = intervalFlag ? inc + 1 : inc;
+        for (long i = 0; i < checkSize; ++i) oldEigDiffs[i] = eigDiffs[i];
+        maxEigDiff = 0.0;
+        for (long i = 0; i < checkSize; ++i) {
+          const double d = std::abs(theta[i] - oldEigs[i]) / rel(oldEigs[i], subspaceTol);
+          eigDiffs[i] = d;
+          maxEigDiff = d > maxEigDiff ? d : maxEigDiff;
+        }
+        for (long i = 0; i < subspaceSize; ++i) oldEigs[i] = theta[i];
+        long diffCount = 0;
+        eigDiffRatio = 0.0;
+        if (maxEigDiff > subspaceTol) {
+          for (long i = 0; i < checkSize; ++i)
+            if (std::abs(oldEigDiffs[i]) > subspaceTol / 10.0) {
+              eigDiffRatio += std::abs(eigDiffs[i] / oldEigDiffs[i]);
+              diffCount++;
+            }
+          eigDiffRatio = diffCount > 0 ? eigDiffRatio / double(diffCount) : 1.0;
+        }
+        const double spectralRange = std::abs(lambdaMax - minEigValue);
+        maxGap = 0.0;
+        for (long i = 1; i < checkSize; ++i)
+          maxGap = std::max(maxGap, std::abs(theta[i] - theta[i - 1]) / spectralRange);
+        record(pass, innerLoopCount, starDegree, starBound, shift, lambdaStar, maxResidual,
+               maxEigDiff, eigDiffRatio, maxGap);
+        if (p_.verbose)
+          std::printf("%-5ld : Degree %-3ld  Residual Max: %-10.5g  Eig Diff Max: %-10.5g  Eig Conv Factor: %-10.5g Max Gap %-10.5g\n",
+                      innerLoopCount, starDegree, maxResidual, maxEigDiff, eigDiffRatio, maxGap);
+
+        if (stop == RCG_STOP_RESIDUAL_ONLY) stopCheckValue = maxResidual;
+        else if (stop == RCG_STOP_EIGENVALUE_ONLY) stopCheckValue = maxEigDiff;
+        else {
+          stopCheckValue = maxEigDiff;
+          if (maxResidual > std::sqrt(subspaceTol)) stopCheckValue = maxResidual;
+        }
+        if (inc == 0) stopCheckValue = 0.0;
+        if (stop == RCG_STOP_RESIDUAL_ONLY && maxResidual < std::sqrt(subspaceTol)) {
+          const long r = (long)residualHistory.size() - 1;
+          if (r > 3) {
+            const double dA = (residualHistory[r - 3] - 2.0 * residualHistory[r - 2] + residualHistory[r - 1]) / std::abs(maxResidual);
+            const double dB = (residualHistory[r - 2] - 2.0 * residualHistory[r - 1] + residualHistory[r]) / std::abs(maxResidual);
+            if (dA * dB < 0.0 && ++oscillationCount > 5) stopCheckValue = 0.0;
+          }
+        }
+        lambdaStar = theta[subspaceSize - 1];
+        minEigValue = minEigValue < theta[0] ? minEigValue : theta[0];
+        innerLoopCount++;
+      }
+      fmaxres = std::max(fmaxres, maxResidual);
+      fmaxdiff = std::max(fmaxdiff, maxEigDiff);
+      fmaxgap = std::max(fmaxgap, maxGap);
+      applyCountCumulative += applyCount;
+
+      // lock found pairs (LM:1486-1543)
+      long foundCount = 0;
+      const long checkIndexCount = (foundSize + subspaceSize < N) ? inc : subspaceSize;
+      for (long i = 0; i < checkIndexCount; ++i)
+        if ((theta[i] - lambdaMax) / rel(lambdaMax, subspaceTol) < subspaceTol) foundCount++;
+      if (foundCount + foundSize >= maxDim) {
+        foundCount = maxDim - foundSize;
+        exitFlag = true;
+      }
+      if (foundCount + foundSize >= N) {
+        foundCount = N - foundSize;
+        exitFlag = true;
+      }
+      if (foundCount > 0) {
+        rcg::launch_copy_cols<double>(n, eng_.V(), (int)ld, 0, eng_.Y(), (int)ldy, (int)foundSize,
+                                      (int)foundCount, st);
+        for (long i = 0; i < foundCount; ++i) {
+          eigValues.push_back(theta[i]);
+          eigRes.push_back(subspaceResiduals[i]);
+        }
+        foundSize += foundCount;
+      }
+      if (!exitFlag && foundCount > 0) {  // shuffle (LM:1549-1555)
+        rcg::launch_copy_cols<double>(n, eng_.V(), (int)ld, (int)foundCount, eng_.T(), (int)ld, 0,
+                                      (int)(subspaceSize - foundCount), st);
+        eng_.swapVT();
+      }
+      const double gv = buf > 0 ? theta[subspaceSize - buf] : lambdaMax;  // guard (LM:1559-1587)
+      guard_ = gv;
+      if (!intervalFlag) {
+        if (gv > lambdaMax) exitFlag = true;
+      } else if ((gv - lambdaMax) / rel(lambdaMax, subspaceTol) >= 10.0 * subspaceTol) {
+        exitFlag = true;
+      }
+      if (foundSize > 0) minEigValue = eigValues[foundSize - 1];  // LM:1593
+      starDegree = 1;
+      starBound = maxEigValue;
+      if (!exitFlag && foundSize + subspaceSize >= N) {  // shrink (LM:1605-1638)
+        if (foundSize + inc >= N) {
+          buf = 0;
+          inc = N - foundSize;
+          subspaceSize = inc;
+          completedBasis = true;
+          eng_.randomize(0, subspaceSize, p_.seed, &counter_);
+        } else {
+          buf = N - (foundSize + inc);
+          subspaceSize = inc + buf;
+        }
+        theta.resize(subspaceSize, 0.0);
+      }
+      ++pass;
+    }
+
+    // non-monotone repair (LM:1664-1707), or the requested FP64 refinement
+    bool nonMonotone = false;
+    for (long i = 0; i + 1 < foundSize; ++i)
+      if (eigValues[i] > eigValues[i + 1]) nonMonotone = true;
+    if (nonMonotone || (p_.refine && foundSize > 0)) {
+      const double tr = wall();
+      repair(foundSize, eigValues, eigRes);
+      count_apply_ += 2 * foundSize;
+      count_eig_ += 1;
+      t_rr_ += wall() - tr;
+    }
+    if (intervalFlag) {  // trim (LM:1715-1732)
+      long fc = 0;
+      for (double e : eigValues)
+        if ((e - lambdaMax) / rel(lambdaMax, subspaceTol) < subspaceTol) fc++;
+      if (fc < (long)eigValues.size()) {
+        eigValues.resize(fc);
+        eigRes.resize(fc);
+        foundSize = fc;
+      }
+    }
+    eng_.sync();
+    found_ = foundSize;
+    R->found = foundSize;
+    R->guard = guard_;
+    R->count_ortho = count_ortho_;
+    R->count_apply = count_apply_;
+    R->count_eig = count_eig_;
+    R->final_maxres = fmaxres;
+    R->final_maxdiff = fmaxdiff;
+    R->final_maxgap = fmaxgap;
+    R->t_total = wall() - t0;
+    R->t_filter = t_filter_;
+    R->t_ortho = t_ortho_;
+    R->t_rr = t_rr_;
+    R->filter_degree_sum = degree_sum_;
+    return foundSize;
+  }
+
+  // Rayleigh-Ritz over the found vectors Y[:, :f] (LM:1672-1707): the
+  // projection, its eigensystem, the rotated and normalized vectors and
+  // their residuals, with the Ritz values stored as the eigenvalues.
+  void repair(long f, std::vector<double>& eigValues, std::vector<double>& eigRes) {
+    const int64_t n = op_->n;
+    const int64_t ldy = eng_.ldy();
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)ldy;
+    DevBuf AY, T2;
+    AY.ensure(bytes);
+    T2.ensure(bytes);
+    Engine e2(op_->ctx, op_, n);
+    e2.reserve(ldy, ldy);
+    e2.attach(eng_.Y(), T2.as<double>(), AY.as<double>(), ldy, eng_.Y(), ldy);
+    std::vector<double> H, th, Z, r;
+    e2.project(f, H);
+    rcg::dense::jacobi_eigensystem(H.data(), f, th, Z);
+    e2.rotate_residuals(f, Z, th, f, r);
+    if (!e2.v_is_first())
+      rcg::launch_copy_cols<double>(n, e2.V(), (int)ldy, 0, eng_.Y(), (int)ldy, 0, (int)f, e2.stream());
+    e2.sync();
+    eigValues.assign(th.begin(), th.end());
+    eigRes = r;
+  }
+
+  void export_vectors(rcg_block_s* out) {
+    if (!out || found_ <= 0) return;
+    const long cnt = std::min<long>(found_, (long)out->cols);
+    rcg::launch_copy_cols<double>(op_->n, eng_.Y(), (int)eng_.ldy(), 0,
+                                  static_cast<double*>(out->d()), (int)out->ld, 0, (int)cnt,
+                                  eng_.stream());
+    eng_.sync();
+  }
+
+  std::vector<TraceRow> trace_;
+  uint64_t counter_ = 0;
+  const double* start_ = nullptr;
+  int64_t start_ld_ = 0, start_cols_ = 0;
+
+ private:
+  rcg_operator_s* op_;
+  rcg_solve_params p_;
+  Engine eng_;
+  DevBuf Vb_, Tb_, Wb_, Yb_;
+  long found_ = 0;
+  long count_ortho_ = 0, count_apply_ = 0, count_eig_ = 0, degree_sum_ = 0;
+  double guard_ = 0.0, t_filter_ = 0.0, t_ortho_ = 0.0, t_rr_ = 0.0;
+};
+
+void check_pair(rcg_operator op, rcg_block a, rcg_block b, int64_t cols) {
+  need(op && a && b, "NULL argument");
+  need(a->n == op->n && b->n == op->n, "block row count differs from the operator");
+  need(a->ld == b->ld, "blocks must share the leading dimension");
+  need(cols >= 0 && cols <= a->cols && cols <= b->cols, "column count exceeds the block");
+  need(a->dtype == RCG_F64 && b->dtype == RCG_F64, "FP64 blocks expected");
+  need(a->ctx == op->ctx && b->ctx == op->ctx, "handles belong to different contexts");
+}
+
+// Engine over caller blocks for the piecewise entry points.
+struct Piece {
+  Engine eng;
+  Piece(rcg_block V, rcg_block T, const rcg_operator_s* op, int64_t kmax = 4)
+      : eng(V->ctx, op, V->n) {
+    eng.reserve(V->ld, std::max<int64_t>(kmax, 4));
+    eng.attach(static_cast<double*>(V->d()), T ? static_cast<double*>(T->d()) : nullptr, nullptr,
+               V->ld, nullptr, 0);
+  }
+  // make sure the result sits in the caller's V block
+  void settle(rcg_block V, int64_t cols) {
+    if (!eng.v_is_first())
+      rcg::launch_copy_cols<double>(V->n, eng.V(), (int)V->ld, 0, static_cast<double*>(V->d()),
+                                    (int)V->ld, 0, (int)cols, eng.stream());
+    eng.sync();
+  }
+};
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int rcg_get_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return RCG_ERR_INVALID;
+  std::snprintf(buf, len, "%s", g_last_error.c_str());
+  return RCG_OK;
+}
+
+const char* rcg_version(void) { return "rcg 0.1 gfx950"; }
+
+int rcg_device_count(int* count) {
+  return guarded([&] {
+    need(count != nullptr, "count is NULL");
+    RCG_HIP(hipGetDeviceCount(count));
+  });
+}
+
+int rcg_context_create(int device, void* hip_stream, rcg_context* out) {
+  return guarded([&] {
+    need(out != nullptr, "out is NULL");
+    int ndev = 0;
+    RCG_HIP(hipGetDeviceCount(&ndev));
+    need(device >= 0 && device < ndev, "device index out of range");
+    auto c = std::make_unique<rcg_context_s>();
+    c->device = device;
+    DeviceGuard g(device);
+    if (hip_stream) {
+      c->stream = static_cast<hipStream_t>(hip_stream);
+    } else {
+      RCG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      c->own_stream = true;
+    }
+    *out = c.release();
+  });
+}
+
+int rcg_context_destroy(rcg_context ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    DeviceGuard g(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int rcg_context_synchronize(rcg_context ctx) {
+  return guarded([&] {
+    need(ctx != nullptr, "context is NULL");
+    DeviceGuard g(ctx->device);
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int rcg_block_create(rcg_context ctx, int64_t n, int64_t cols, int dtype, rcg_block* out) {
+  return guarded([&] {
+    need(ctx && out, "NULL argument");
+    need(n > 0 && cols > 0, "block dimensions must be positive");
+    need(dtype == RCG_F64 || dtype == RCG_F32, "blocks are FP64 or FP32");
+    DeviceGuard g(ctx->device);
+    auto b = std::make_unique<rcg_block_s>();
+    b->ctx = ctx;
+    b->n = n;
+    b->cols = cols;
+    b->ld = block_ld(cols);
+    b->dtype = dtype;
+    const size_t bytes = rcg::dtype_size(dtype) * (size_t)n * (size_t)b->ld;
+    b->mem.ensure(bytes);
+    RCG_HIP(hipMemsetAsync(b->mem.p, 0, bytes, ctx->stream));
+    RCG_HIP(hipStreamSynchronize(ctx->stream));
+    *out = b.release();
+  });
+}
+
+int rcg_block_destroy(rcg_block b) {
+  return guarded([&] {
+    if (!b) return;
+    DeviceGuard g(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+  });
+}
+
+namespace {
+// Host column-major <-> device row-interleaved through a host staging copy.
+void transfer(rcg_block b, void* host, int64_t ld_host, int64_t cols, bool up) {
+  need(b && host, "NULL argument");
+  need(cols >= 0 && cols <= b->cols, "column count exceeds the block");
+  need(ld_host >= b->n, "host leading dimension smaller than n");
+  DeviceGuard g(b->ctx->device);
+  const size_t es = rcg::dtype_size(b->dtype);
+  const size_t bytes = es * (size_t)b->n * (size_t)b->ld;
+  std::vector<char> rows(bytes);
+  char* h = static_cast<char*>(host);
+  hipStream_t st = b->ctx->stream;
+  if (up) {
+    RCG_HIP(hipMemcpyAsync(rows.data(), b->mem.p, bytes, hipMemcpyDeviceToHost, st));
+    RCG_HIP(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < b->n; ++i)
+      for (int64_t c = 0; c < cols; ++c)
+        std::memcpy(&rows[es * (i * b->ld + c)], h + es * (c * ld_host + i), es);
+    RCG_HIP(hipMemcpyAsync(b->mem.p, rows.data(), bytes, hipMemcpyHostToDevice, st));
+    RCG_HIP(hipStreamSynchronize(st));
+  } else {
+    RCG_HIP(hipMemcpyAsync(rows.data(), b->mem.p, bytes, hipMemcpyDeviceToHost, st));
+    RCG_HIP(hipStreamSynchronize(st));
+    for (int64_t c = 0; c < cols; ++c)
+      for (int64_t i = 0; i < b->n; ++i)
+        std::memcpy(h + es * (c * ld_host + i), &rows[es * (i * b->ld + c)], es);
+  }
+}
+}  // namespace
+
+int rcg_block_upload(rcg_block b, const void* host, int64_t ld_host, int64_t cols) {
+  return guarded([&] { transfer(b, const_cast<void*>(host), ld_host, cols, true); });
+}
+
+int rcg_block_download(rcg_block b, void* host, int64_t ld_host, int64_t cols) {
+  return guarded([&] { transfer(b, host, ld_host, cols, false); });
+}
+
+int rcg_block_copy(rcg_block dst, rcg_block src) {
+  return guarded([&] {
+    need(dst && src, "NULL argument");
+    need(dst->n == src->n && dst->ld == src->ld && dst->dtype == src->dtype,
+         "blocks differ in shape or type");
+    DeviceGuard g(dst->ctx->device);
+    RCG_HIP(hipMemcpyAsync(dst->mem.p, src->mem.p,
+                           rcg::dtype_size(src->dtype) * (size_t)src->n * (size_t)src->ld,
+                           hipMemcpyDeviceToDevice, dst->ctx->stream));
+    RCG_HIP(hipStreamSynchronize(dst->ctx->stream));
+  });
+}
+
+int rcg_block_randomize(rcg_block b, int64_t col0, int64_t ncols, uint64_t seed,
+                        uint64_t* counter) {
+  return guarded([&] {
+    need(b && counter, "NULL argument");
+    need(col0 >= 0 && ncols >= 0 && col0 + ncols <= b->cols, "column range exceeds the block");
+    DeviceGuard g(b->ctx->device);
+    if (b->dtype == RCG_F64)
+      rcg::launch_randomize_cols<double>(b->n, static_cast<double*>(b->d()), (int)b->ld, (int)col0,
+                                         (int)ncols, seed, *counter, b->ctx->stream);
+    else
+      rcg::launch_randomize_cols<float>(b->n, static_cast<float*>(b->d()), (int)b->ld, (int)col0,
+                                        (int)ncols, seed, *counter, b->ctx->stream);
+    *counter += (uint64_t)ncols * (uint64_t)b->n;
+    RCG_HIP(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int rcg_block_info(rcg_block b, int64_t* n, int64_t* cols, int64_t* ld, int* dtype,
+                   void** device_ptr) {
+  return guarded([&] {
+    need(b != nullptr, "block is NULL");
+    if (n) *n = b->n;
+    if (cols) *cols = b->cols;
+    if (ld) *ld = b->ld;
+    if (dtype) *dtype = b->dtype;
+    if (device_ptr) *device_ptr = b->d();
+  });
+}
+
+int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32_t* rowptr,
+                            const int32_t* col, const void* val, int val_dtype, int mem,
+                            rcg_operator* out) {
+  return guarded([&] {
+    need(ctx && rowptr && col && val && out, "NULL argument");
+    need(n > 0 && nnz >= 0, "bad operator size");
+    need(nnz <= INT32_MAX && n <= INT32_MAX, "sizes exceed the int32 index range");
+    need(val_dtype == RCG_F64, "CSR values must be FP64");
+    need(mem == RCG_MEM_HOST || mem == RCG_MEM_DEVICE, "bad memory flag");
+    DeviceGuard g(ctx->device);
+    auto o = std::make_unique<rcg_operator_s>();
+    o->ctx = ctx;
+    o->n = n;
+    o->nnz = nnz;
+    o->dtype = val_dtype;
+    if (mem == RCG_MEM_DEVICE) {
+      o->rowptr = rowptr;
+      o->col = col;
+      o->val = static_cast<const double*>(val);
+    } else {
+      // validated on the host before anything reaches a kernel
+      need(rowptr[0] == 0 && rowptr[n] == nnz, "rowptr must start at 0 and end at nnz");
+      for (int64_t i = 0; i < n; ++i) need(rowptr[i] <= rowptr[i + 1], "rowptr must be non-decreasing");
+      for (int64_t p = 0; p < nnz; ++p) need(col[p] >= 0 && col[p] < n, "column index out of range");
+      o->own_rowptr.ensure(sizeof(int) * (size_t)(n + 1));
+      o->own_col.ensure(sizeof(int) * (size_t)std::max<int64_t>(nnz, 1));
+      o->own_val.ensure(sizeof(double) * (size_t)std::max<int64_t>(nnz, 1));
+      RCG_HIP(hipMemcpy(o->own_rowptr.p, rowptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+      RCG_HIP(hipMemcpy(o->own_col.p, col, sizeof(int) * nnz, hipMemcpyHostToDevice));
+      RCG_HIP(hipMemcpy(o->own_val.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+      o->rowptr = o->own_rowptr.as<int>();
+      o->col = o->own_col.as<int>();
+      o->val = o->own_val.as<double>();
+    }
+    *out = o.release();
+  });
+}
+
+int rcg_operator_destroy(rcg_operator op) {
+  return guarded([&] {
+    if (!op) return;
+    DeviceGuard g(op->ctx->device);
+    (void)hipStreamSynchronize(op->ctx->stream);
+    delete op;
+  });
+}
+
+int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alpha, double beta,
+                       int64_t cols) {
+  return guarded([&] {
+    check_pair(op, in, out, cols);
+    need(in != out, "in and out must differ");
+    DeviceGuard g(op->ctx->device);
+    rcg::launch_csr_apply<double>(op->n, op->rowptr, op->col, op->val,
+                                  static_cast<const double*>(in->d()), static_cast<double*>(out->d()),
+                                  (int)in->ld, (int)cols, alpha, beta, op->ctx->stream);
+    RCG_HIP(hipStreamSynchronize(op->ctx->stream));
+  });
+}
+
+int rcg_cheb_filter(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols, int64_t degree,
+                    int64_t repetition, double lambda_max, double shift) {
+  return guarded([&] {
+    check_pair(op, X, tmp, cols);
+    need(X != tmp, "X and tmp must differ");
+    need(degree >= 1 && repetition >= 1, "degree and repetition must be >= 1");
+    DeviceGuard g(op->ctx->device);
+    Piece pc(X, tmp, op);
+    pc.eng.filter(cols, (long)degree, (long)repetition, lambda_max, shift);
+    pc.settle(X, cols);
+  });
+}
+
+int rcg_orthonormalize(rcg_block X, rcg_block tmp, int64_t cols) {
+  return guarded([&] {
+    need(X && tmp && X != tmp, "X and tmp must be distinct blocks");
+    need(X->n == tmp->n && X->ld == tmp->ld && X->ctx == tmp->ctx, "X and tmp differ in shape");
+    need(cols >= 1 && cols <= X->cols, "bad column count");
+    need(X->dtype == RCG_F64 && tmp->dtype == RCG_F64, "FP64 blocks expected");
+    DeviceGuard g(X->ctx->device);
+    Piece pc(X, tmp, nullptr);
+    pc.eng.orthonormalize(cols);
+    pc.settle(X, cols);
+  });
+}
+
+int rcg_deflate(rcg_block X, int64_t cols, rcg_block Y, int64_t found) {
+  return guarded([&] {
+    need(X && Y && X != Y, "bad arguments");
+    need(X->n == Y->n && X->ctx == Y->ctx, "X and Y differ in rows or context");
+    need(X->dtype == RCG_F64 && Y->dtype == RCG_F64, "FP64 blocks expected");
+    need(cols >= 1 && cols <= X->cols && found >= 0 && found <= Y->cols, "bad column counts");
+    DeviceGuard g(X->ctx->device);
+    Engine e(X->ctx, nullptr, X->n);
+    e.reserve(std::max(X->ld, Y->ld), std::max(X->ld, Y->ld));
+    e.attach(static_cast<double*>(X->d()), nullptr, nullptr, X->ld, static_cast<double*>(Y->d()), Y->ld);
+    e.deflate(cols, found);
+    e.normalize(cols);
+    e.sync();
+  });
+}
+
+int rcg_project(rcg_operator op, rcg_block V, rcg_block W, int64_t cols, double* H_host) {
+  return guarded([&] {
+    check_pair(op, V, W, cols);
+    need(V != W && H_host && cols >= 1, "bad arguments");
+    DeviceGuard g(op->ctx->device);
+    Engine e(op->ctx, op, op->n);
+    e.reserve(V->ld, 4);
+    e.attach(static_cast<double*>(V->d()), nullptr, static_cast<double*>(W->d()), V->ld, nullptr, 0);
+    std::vector<double> H;
+    e.project(cols, H);
+    std::memcpy(H_host, H.data(), sizeof(double) * cols * cols);
+  });
+}
+
+int rcg_rotate(rcg_block V, rcg_block tmp, int64_t cols, const double* Y_host) {
+  return guarded([&] {
+    need(V && tmp && V != tmp && Y_host, "bad arguments");
+    need(V->n == tmp->n && V->ld == tmp->ld && V->ctx == tmp->ctx, "shape mismatch");
+    need(cols >= 1 && cols <= V->cols, "bad column count");
+    DeviceGuard g(V->ctx->device);
+    Piece pc(V, tmp, nullptr);
+    pc.eng.right_multiply(cols, Y_host);
+    pc.eng.normalize(cols);
+    pc.settle(V, cols);
+  });
+}
+
+int rcg_small_eigensystem(int64_t n, const double* H, double* evals, double* evecs) {
+  return guarded([&] {
+    need(n >= 1 && H && evals && evecs, "bad arguments");
+    std::vector<double> w, Z;
+    rcg::dense::jacobi_eigensystem(H, n, w, Z);
+    std::memcpy(evals, w.data(), sizeof(double) * n);
+    std::memcpy(evecs, Z.data(), sizeof(double) * n * n);
+  });
+}
+
+int rcg_residuals(rcg_operator op, rcg_block V, rcg_block W, int64_t count, const double* theta,
+                  double* res_host) {
+  return guarded([&] {
+    check_pair(op, V, W, count);
+    need(V != W && theta && res_host, "bad arguments");
+    DeviceGuard g(op->ctx->device);
+    Engine e(op->ctx, op, op->n);
+    e.reserve(V->ld, 4);
+    e.attach(static_cast<double*>(V->d()), nullptr, static_cast<double*>(W->d()), V->ld, nullptr, 0);
+    std::vector<double> r;
+    e.residuals(count, theta, r);
+    std::memcpy(res_host, r.data(), sizeof(double) * count);
+  });
+}
+
+int rcg_lanczos_bounds(rcg_operator op, double tol, int stop_flags, int64_t itmax, uint64_t seed,
+                       uint64_t* counter, double* min_eig, double* max_eig, int64_t* iters) {
+  return guarded([&] {
+    need(op && counter && min_eig && max_eig && iters, "NULL argument");
+    need(itmax >= 5, "itmax must be >= 5");
+    DeviceGuard g(op->ctx->device);
+    Engine e(op->ctx, op, op->n);
+    e.reserve(4, 4);
+    double mn, mx;
+    long it;
+    e.lanczos(tol, (stop_flags & 1) != 0, (stop_flags & 2) != 0, (long)itmax, seed, counter, &mn,
+              &mx, &it);
+    *min_eig = mn;
+    *max_eig = mx;
+    *iters = it;
+  });
+}
+
+int rcg_solve_lowest(rcg_operator op, const rcg_solve_params* params, rcg_solve_result* result,
+                     double* evals, double* res, int64_t cap, rcg_block evecs, double* trace,
+                     int64_t trace_cap) {
+  return guarded([&] {
+    need(op && params && result && evals && res, "NULL argument");
+    need(params->k >= 1, "eigCount must be >= 1");
+    need(params->filter_dtype == RCG_F64, "only the FP64 filter is available in this build");
+    if (evecs) need(evecs->n == op->n && evecs->dtype == RCG_F64 && evecs->ctx == op->ctx, "evecs block mismatch");
+    DeviceGuard g(op->ctx->device);
+    std::memset(result, 0, sizeof(*result));
+    Solver s(op, *params);
+    if (params->nonrandom_start > 0) {
+      need(evecs != nullptr, "a non-random start needs the evecs block");
+      need(params->nonrandom_start <= evecs->cols, "more start vectors than evecs columns");
+      s.start_ = static_cast<const double*>(evecs->d());
+      s.start_ld_ = evecs->ld;
+      s.start_cols_ = params->nonrandom_start;
+    }
+    std::vector<double> ev, rr;
+    const long f = s.run(result, ev, rr);
+    result->trace_len = (int64_t)s.trace_.size();
+    result->rng_counter = s.counter_;
+    for (long i = 0; i < f && i < cap; ++i) {
+      evals[i] = ev[i];
+      res[i] = rr[i];
+    }
+    if (trace)
+      for (int64_t t = 0; t < (int64_t)s.trace_.size() && t < trace_cap; ++t)
+        std::memcpy(trace + 10 * t, s.trace_[t].v, sizeof(double) * 10);
+    if (evecs) s.export_vectors(evecs);
+  });
+}
+
+int64_t rcg_sizeof_solve_params(void) { return (int64_t)sizeof(rcg_solve_params); }
+int64_t rcg_sizeof_solve_result(void) { return (int64_t)sizeof(rcg_solve_result); }
+
+}  // extern "C"

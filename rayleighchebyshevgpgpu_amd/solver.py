@@ -1,0 +1,297 @@
+"""Python host side over the C ABI, mirroring the reference's interface.
+
+The reference is a C++ template library used as
+
+    RayleighChebyshevLM<Vtype, Otype, RandOp> rc;
+    rc.setStopCondition("RESIDUAL_ONLY");
+    long found = rc.getMinEigenSystem(eigCount, tol, inc, buf, vStart, oP, randOp,
+                                      eigValues, eigVectors);
+    rc.getEigVectorResiduals(); rc.getCountData(); rc.getFinalData();
+
+(RayleighChebyshevLM.h:281-472, 693-806).  ``RayleighChebyshevLM`` below has
+the same method names, argument meaning and error behaviour, over device
+blocks (``Block``, the AmatrixClassLMCuda role), a device CSR operator
+(``CsrOperator``, the user Otype) and ``RandomizeOp`` (ArandomizeOpClassLMCuda,
+with a seed).  Everything runs on the GPU through librcg.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check, lib
+
+
+class Context:
+    """Device + stream (rcg_context)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        h = C.c_void_p()
+        check(lib().rcg_context_create(device, C.c_void_p(stream) if stream else None, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        check(lib().rcg_context_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().rcg_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Block:
+    """N x cols device block (AmatrixClassLMCuda role); host side column-major."""
+
+    def __init__(self, n: int, cols: int, dtype=np.float64, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.n, self.cols = int(n), int(cols)
+        self.dtype = np.dtype(dtype)
+        code = capi.F64 if self.dtype == np.float64 else capi.F32
+        h = C.c_void_p()
+        check(lib().rcg_block_create(self.ctx.h, self.n, self.cols, code, C.byref(h)))
+        self.h = h
+        ld = C.c_int64()
+        check(lib().rcg_block_info(self.h, None, None, C.byref(ld), None, None))
+        self.ld = ld.value
+
+    @classmethod
+    def from_host(cls, X: np.ndarray, ctx: Context | None = None) -> "Block":
+        X = np.asfortranarray(X)
+        b = cls(X.shape[0], X.shape[1], X.dtype, ctx)
+        b.upload(X)
+        return b
+
+    def upload(self, X: np.ndarray):
+        X = np.asfortranarray(X, dtype=self.dtype)
+        check(lib().rcg_block_upload(self.h, X.ctypes.data_as(C.c_void_p), X.shape[0], X.shape[1]))
+
+    def download(self, cols: int | None = None) -> np.ndarray:
+        cols = self.cols if cols is None else cols
+        X = np.empty((self.n, cols), dtype=self.dtype, order="F")
+        check(lib().rcg_block_download(self.h, X.ctypes.data_as(C.c_void_p), self.n, cols))
+        return X
+
+    def randomize(self, col0: int, ncols: int, seed: int, counter: int) -> int:
+        c = C.c_uint64(counter)
+        check(lib().rcg_block_randomize(self.h, col0, ncols, seed, C.byref(c)))
+        return c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rcg_block_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CsrOperator:
+    """Device CSR operator (the reference's user Otype with
+    apply(in, out, alpha, beta), LanczosCpolyOperatorLMCuda.h:236,245)."""
+
+    def __init__(self, csr, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        if csr.is_complex:
+            raise NotImplementedError("complex CSR operators are not available in this build")
+        self.n, self.nnz = csr.n, csr.nnz
+        self._keep = (csr.rowptr, csr.col, csr.val)
+        h = C.c_void_p()
+        check(lib().rcg_operator_csr_create(
+            self.ctx.h, csr.n, csr.nnz, csr.rowptr.ctypes.data_as(C.c_void_p),
+            csr.col.ctypes.data_as(C.c_void_p), csr.val.ctypes.data_as(C.c_void_p), capi.F64,
+            capi.MEM_HOST, C.byref(h)))
+        self.h = h
+        self._keep = None
+
+    def apply(self, X: Block, Y: Block, alpha: float = 1.0, beta: float = 0.0, cols=None):
+        check(lib().rcg_operator_apply(self.h, X.h, Y.h, alpha, beta, X.cols if cols is None else cols))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rcg_operator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RandomizeOp:
+    """Start-vector stream (ArandomizeOpClassLMCuda role, seeded)."""
+
+    def __init__(self, seed: int = 0):
+        self.seed = int(seed)
+        self.counter = 0
+
+
+class RayleighChebyshevLM:
+    """RayleighChebyshevLM over librcg.so (RayleighChebyshevLM.h:213-472)."""
+
+    def __init__(self):
+        self.stopCondition = capi.STOP["COMBINATION"]
+        self.maxMinTol = 1.0e-6
+        self.maxInnerLoopCount = 10000
+        self.fixedIterationCount = False
+        self.nonRandomStartFlag = False
+        self.verboseFlag = False
+        self.minIntervalPolyDegreeMax = 200
+        self.filterRepetitionCount = 1
+        self.spectralBoundsIterMax = 10000
+        self.filterPrecision = np.float64
+        self.refine = False
+        self._result = capi.SolveResult()
+        self._residuals: list[float] = []
+        self.trace = np.zeros((0, 10))
+
+    # ---- setters (RayleighChebyshevLM.h:281-440)
+    def setStopCondition(self, cond):
+        if isinstance(cond, str):
+            key = cond.upper()
+            if key not in capi.STOP:
+                raise RuntimeError(
+                    "RayleighChebyshevLM Error : Stopping condition type specified not one of "
+                    f"DEFAULT, COMBINATION, EIGENVALUE_ONLY,or RESIDUAL_ONLY. Offending specification : {cond}")
+            cond = capi.STOP[key]
+        self.stopCondition = capi.STOP["COMBINATION"] if cond == 0 else int(cond)
+
+    def setMaxMinTolerance(self, val=1.0e-6):
+        self.maxMinTol = float(val)
+
+    def setMaxInnerLoopCount(self, val):
+        self.maxInnerLoopCount = int(val)
+
+    def resetMaxInnerLoopCount(self):
+        self.maxInnerLoopCount = 10000
+
+    def setFixedIterationCount(self, val):
+        self.fixedIterationCount = bool(val)
+
+    def setNonRandomStartFlag(self, val=True):
+        self.nonRandomStartFlag = bool(val)
+
+    def clearNonRandomStartFlag(self):
+        self.nonRandomStartFlag = False
+
+    def setVerboseFlag(self, val=True):
+        self.verboseFlag = bool(val)
+
+    def setMinIntervalPolyDegreeMax(self, val):
+        self.minIntervalPolyDegreeMax = int(val)
+
+    def setFilterRepetitionCount(self, val):
+        self.filterRepetitionCount = int(val)
+
+    def setMaxSpectralBoundsIter(self, val):
+        self.spectralBoundsIterMax = int(val)
+
+    # ---- getters
+    def getEigVectorResiduals(self):
+        return list(self._residuals)
+
+    def getCountData(self):
+        r = self._result
+        return {"ortho": r.count_ortho, "OpApply": r.count_apply, "eigenvalue": r.count_eig}
+
+    def getFinalData(self):
+        r = self._result
+        return {"maxResidual": r.final_maxres, "maxEigValueDifference": r.final_maxdiff,
+                "maxRelEigValueGap": r.final_maxgap}
+
+    def getSpectralBoundsIterCount(self):
+        return self._result.lanczos_iters
+
+    def getMinEigValueEst(self):
+        return self._result.min_est
+
+    def getMaxEigValueEst(self):
+        return self._result.max_est
+
+    def getGuardEigenvalue(self):
+        return self._result.guard
+
+    def getResult(self) -> capi.SolveResult:
+        return self._result
+
+    # ---- solves
+    def _solve(self, k, tol, inc, buf, oP, randOp, eigValues, eigVectors, interval=False,
+               lambda_max=0.0, bounds=None):
+        p = capi.SolveParams(
+            k=int(k), inc=int(inc), buf=int(buf), tol=float(tol), seed=randOp.seed,
+            stop=self.stopCondition, interval=int(interval),
+            max_inner=self.maxInnerLoopCount, deg_max=self.minIntervalPolyDegreeMax,
+            rep=self.filterRepetitionCount, lambda_max=float(lambda_max),
+            lanczos_itmax=self.spectralBoundsIterMax, maxmin_tol=self.maxMinTol,
+            fixed_iteration=int(self.fixedIterationCount),
+            filter_dtype=capi.F64 if self.filterPrecision == np.float64 else capi.F32,
+            refine=int(self.refine), verbose=int(self.verboseFlag),
+            use_bounds=0 if bounds is None else 1,
+            nonrandom_start=(min(getattr(eigVectors, "valid_cols", 0), eigVectors.cols)
+                             if (self.nonRandomStartFlag and eigVectors is not None) else 0),
+            min_eig=0.0 if bounds is None else float(bounds[0]),
+            max_eig=0.0 if bounds is None else float(bounds[1]))
+        cap = max(int(k), int(inc) + int(buf), 1)
+        ev = np.zeros(cap)
+        rr = np.zeros(cap)
+        trace = np.zeros((200000, 10))
+        r = capi.SolveResult()
+        check(lib().rcg_solve_lowest(
+            oP.h, C.byref(p), C.byref(r), ev.ctypes.data_as(C.c_void_p),
+            rr.ctypes.data_as(C.c_void_p), cap, eigVectors.h if eigVectors is not None else None,
+            trace.ctypes.data_as(C.c_void_p), trace.shape[0]))
+        self._result = r
+        f = int(r.found)
+        self._residuals = rr[:f].tolist()
+        self.trace = trace[: min(int(r.trace_len), trace.shape[0])].copy()
+        randOp.counter = int(r.rng_counter)
+        if eigVectors is not None:
+            eigVectors.valid_cols = min(f, eigVectors.cols)
+        if eigValues is not None:
+            eigValues.clear()
+            eigValues.extend(ev[:f].tolist())
+        return f
+
+    def getMinEigenSystem(self, eigCount, *args):
+        """getMinEigenSystem(eigCount, subspaceTol, inc, buf, vStart, oP, randOp,
+        eigValues, eigVectors) or getMinEigenSystem(eigCount, minEigValueEst,
+        maxEigValueBound, subspaceTol, inc, buf, vStart, oP, randOp, eigValues,
+        eigVectors) (RayleighChebyshevLM.h:693-738).  eigVectors is a Block
+        with at least eigCount columns (or None)."""
+        if len(args) == 8:
+            tol, inc, buf, _vStart, oP, randOp, eigValues, eigVectors = args
+            return self._solve(eigCount, tol, inc, buf, oP, randOp, eigValues, eigVectors)
+        if len(args) == 10:
+            mn, mx, tol, inc, buf, _vStart, oP, randOp, eigValues, eigVectors = args
+            return self._solve(eigCount, tol, inc, buf, oP, randOp, eigValues, eigVectors,
+                               bounds=(mn, mx))
+        raise TypeError("getMinEigenSystem takes 9 or 11 arguments")
+
+    def getMinIntervalEigenSystem(self, lambdaMax, subspaceTol, inc, buf, maxEigensystemDim,
+                                  vStart, oP, randOp, eigValues, eigVectors):
+        """All eigenpairs below lambdaMax, at most maxEigensystemDim
+        (RayleighChebyshevLM.h:745-782)."""
+        return self._solve(maxEigensystemDim, subspaceTol, inc, buf, oP, randOp, eigValues,
+                           eigVectors, interval=True, lambda_max=lambdaMax)

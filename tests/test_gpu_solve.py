@@ -1,0 +1,162 @@
+"""Whole solves on the GPU against the CPU oracle, the compiled reference's
+golden fixture and analytic spectra.
+
+Eigenvalues must agree with the FP64 oracle to a relative 1e-10 with the
+reference's floor max(|lambda|, 1e-11/tol) (RayleighChebyshevLM.h:271-278);
+each returned pair must satisfy the solve's own residual contract.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from rayleighchebyshevgpgpu_amd import capi
+from rayleighchebyshevgpgpu_amd import operators as ops
+from tests.golden.cases import CASES, build_operator
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_cases.json")))
+SOLVE_CASES = [n for n in CASES if n not in ("anderson3d32_k64",)]
+
+
+@pytest.fixture(scope="module")
+def rcg():
+    from rayleighchebyshevgpgpu_amd import solver
+    return solver
+
+
+def gpu_solve(rcg, A, c, want_vectors=False):
+    rc = rcg.RayleighChebyshevLM()
+    rc.setStopCondition(c.get("stop", "COMBINATION"))
+    if c.get("max_inner"):
+        rc.setMaxInnerLoopCount(c["max_inner"])
+    op = rcg.CsrOperator(A)
+    rnd = rcg.RandomizeOp(c.get("seed", 0))
+    ev = []
+    V = rcg.Block(A.n, max(c["k"], c["inc"] + c["buf"])) if want_vectors else None
+    if c.get("interval"):
+        f = rc.getMinIntervalEigenSystem(c["lambda_max"], c["tol"], c["inc"], c["buf"], c["k"],
+                                         None, op, rnd, ev, V)
+    else:
+        f = rc.getMinEigenSystem(c["k"], c["tol"], c["inc"], c["buf"], None, op, rnd, ev, V)
+    return rc, f, np.array(ev), V
+
+
+@pytest.mark.parametrize("name", SOLVE_CASES)
+def test_solve_matches_oracle_and_reference(rcg, orc, name):
+    c = CASES[name]
+    print(f"case: {name} {c}", flush=True)
+    A = build_operator(c)
+    rc, f, ev, _ = gpu_solve(rcg, A, c)
+    o = orc.solve(A, c["k"], c["tol"], c["inc"], c["buf"], seed=c.get("seed", 0),
+                  stop=c.get("stop", "COMBINATION"), interval=c.get("interval", False),
+                  lambda_max=c.get("lambda_max", 0.0), max_inner=c.get("max_inner", 0))
+    g = GOLDEN[name]
+    assert f == o["found"] == g["found"]
+    assert rc.getSpectralBoundsIterCount() == o["lanczos_iters"] == g["lanczos_iters"]
+    if name == "lap2d64_maxinner1":
+        # one iteration per pass: unconverged Ritz values, compared loosely
+        assert np.max(np.abs(ev - o["eigenvalues"])) < 1e-6
+        return
+    floor = np.maximum(np.abs(o["eigenvalues"]), 1e-11 / max(c["tol"], 1e-11))
+    tol_ev = 1e-10 if c.get("stop") == "RESIDUAL_ONLY" else max(10 * c["tol"], 1e-10)
+    assert np.all(np.abs(ev - o["eigenvalues"]) / floor <= tol_ev), (ev, o["eigenvalues"])
+    res = np.array(rc.getEigVectorResiduals())
+    assert res.shape[0] == f
+    if c.get("stop") == "RESIDUAL_ONLY":
+        # the oscillation guard may stop at the pass-locking floor (~ tol)
+        assert np.all(res / np.maximum(np.abs(ev), 1e-11 / c["tol"]) < 10 * c["tol"])
+
+
+def test_degree_sequence_matches_reference(rcg):
+    """Same filter-degree choices as the compiled reference (config 1)."""
+    name = "lap2d64_k16"
+    c = CASES[name]
+    print(f"case: degree sequence {name}", flush=True)
+    rc, f, ev, _ = gpu_solve(rcg, build_operator(c), c)
+    ref_deg = [int(t[1]) for t in GOLDEN[name]["trace"]]
+    assert rc.trace[:, 2].astype(int).tolist() == ref_deg
+    assert rc.getCountData() == GOLDEN[name]["count"]
+
+
+def test_config1_analytic_with_vectors(rcg):
+    """64x64 Dirichlet, lowest 16 vs 4 sin^2(pi i/130) + 4 sin^2(pi j/130); vectors."""
+    print("case: config-1 analytic, RESIDUAL_ONLY 1e-10, with eigenvectors", flush=True)
+    A = ops.laplacian_2d(64)
+    c = dict(k=16, inc=8, buf=8, tol=1e-10, seed=0, stop="RESIDUAL_ONLY")
+    rc, f, ev, V = gpu_solve(rcg, A, c, want_vectors=True)
+    exact = ops.laplacian_2d_spectrum(64)[:16]
+    assert f == 16
+    assert np.max(np.abs(ev - exact) / np.maximum(exact, 0.1)) < 1e-10
+    X = V.download(16)
+    S = A.to_scipy()
+    r = np.linalg.norm(S @ X - X * ev, axis=0)
+    normA = 8.0
+    assert np.max(r) / normA < 1e-10
+    assert np.max(np.abs(X.T @ X - np.eye(16))) < 1e-10
+
+
+def test_periodic3d_analytic_degenerate(rcg):
+    """W=0 periodic 16^3: 4[sin^2(pi a/16)+...]; the 6-fold cluster at index 1..6."""
+    print("case: 16^3 periodic Laplacian, k=7", flush=True)
+    A = ops.laplacian_3d(16, bc="periodic")
+    exact = ops.laplacian_3d_spectrum(16)
+    c = dict(k=7, inc=8, buf=8, tol=1e-10, seed=1, stop="RESIDUAL_ONLY")
+    rc, f, ev, V = gpu_solve(rcg, A, c, want_vectors=True)
+    assert f == 7
+    assert np.max(np.abs(ev - exact[:7])) < 1e-9
+
+
+def test_refinement_keeps_residuals(rcg):
+    """The final FP64 Rayleigh-Ritz over the found set keeps every pair."""
+    print("case: refine on 16^3 Anderson k=32", flush=True)
+    A = build_operator(CASES["anderson3d16_k32"])
+    rc = rcg.RayleighChebyshevLM()
+    rc.setStopCondition("RESIDUAL_ONLY")
+    rc.refine = True
+    op = rcg.CsrOperator(A)
+    ev = []
+    V = rcg.Block(A.n, 32)
+    f = rc.getMinEigenSystem(32, 1e-10, 16, 16, None, op, rcg.RandomizeOp(0), ev, V)
+    assert f == 32
+    ev = np.array(ev)
+    assert np.all(np.diff(ev) >= 0)
+    X = V.download(32)
+    r = np.linalg.norm(A.to_scipy() @ X - X * ev, axis=0)
+    assert np.max(r) / 14.0 < 1e-10
+    exact = np.linalg.eigvalsh(A.to_scipy().toarray())[:32]
+    assert np.max(np.abs(ev - exact) / np.maximum(np.abs(exact), 0.1)) < 1e-10
+
+
+def test_solve_errors_are_status_codes(rcg):
+    print("case: error paths", flush=True)
+    A = ops.laplacian_2d(8)
+    op = rcg.CsrOperator(A)
+    rc = rcg.RayleighChebyshevLM()
+    with pytest.raises(capi.RcgError):
+        rc.getMinEigenSystem(0, 1e-8, 4, 4, None, op, rcg.RandomizeOp(0), [], None)
+    with pytest.raises(capi.RcgError):
+        rc.getMinEigenSystem(4, 1e-8, 0, 0, None, op, rcg.RandomizeOp(0), [], None)
+    bad = ops.Csr(A.n, A.rowptr, A.col.copy(), A.val)
+    bad.col[3] = A.n + 5
+    with pytest.raises(capi.RcgError):
+        rcg.CsrOperator(bad)
+
+
+def test_warm_start_converges_immediately(rcg):
+    """setNonRandomStartFlag: starting from converged vectors needs few iterations."""
+    print("case: warm start", flush=True)
+    A = ops.laplacian_2d(40)
+    c = dict(k=8, inc=8, buf=8, tol=1e-10, seed=0, stop="RESIDUAL_ONLY")
+    rc, f, ev, V = gpu_solve(rcg, A, c, want_vectors=True)
+    cold_iters = rc.getCountData()["eigenvalue"]
+    rc2 = rcg.RayleighChebyshevLM()
+    rc2.setStopCondition("RESIDUAL_ONLY")
+    rc2.setNonRandomStartFlag(True)
+    ev2 = []
+    f2 = rc2.getMinEigenSystem(8, 1e-10, 8, 8, None, rcg.CsrOperator(A), rcg.RandomizeOp(0), ev2, V)
+    assert f2 == 8
+    assert np.max(np.abs(np.array(ev2) - ev)) < 1e-10
+    assert rc2.getCountData()["eigenvalue"] < cold_iters
