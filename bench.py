@@ -1,0 +1,252 @@
+"""Benchmark: converged eigenpairs/s of the lowest-eigenpair solve on one MI355X.
+
+Workload (one "step" = one whole solve, operator resident in HBM):
+  3-D 7-point Laplacian on a 128^3 periodic grid (n = 2,097,152, CSR int32)
+  plus an Anderson diagonal potential U[-2,2] from seed 1; lowest 64
+  eigenpairs, subspace increment 32, buffer 32, tolerance 1e-10 with the
+  residual-only stop; FP64 throughout (filter, orthonormalisation,
+  Rayleigh-Ritz); start vectors from seed 0.  Each step runs the Lanczos
+  bounds, all filter passes and returns eigenvalues + eigenvectors on device.
+
+value = converged eigenpairs (||Av - lambda v|| / ||A|| <= 1e-10, checked on
+the device after the timed region) per second of timed wall time.
+
+roofline: the fused Chebyshev step kernel (csr_block_kernel, mode kCheb),
+timed with HIP events on the library's stream around every filter
+application inside the timed solves; its algorithmic bytes per launch are
+nnz*(8+4) + 4*(n+1) + 3*n*b*8 (CSR values+columns+row pointers, one read of
+X_{j-1}, one read of X_{j-2}, one write of X_j).
+
+cpu_baseline: the CPU oracle (a restatement of the reference's
+RayleighChebyshevLM, OpenMP) timed on a bounded sample of the same solve (the
+Lanczos bounds and one inner iteration per outer pass), extrapolated to the
+whole solve with the GPU solve's own counts (filter degrees, orthogonalisations,
+Rayleigh-Ritz steps).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "converged eigenpairs/s @ resid≤1e-10, mixed vs FP64; p50/p99 solve latency"
+K, INC, BUF, TOL = 64, 32, 32, 1e-10
+GRID = 128
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_operator(grid=GRID):
+    from rayleighchebyshevgpgpu_amd import operators as ops
+    n = grid ** 3
+    pot = ops.anderson_potential(n, 4.0, 1)
+    return ops.laplacian_3d(grid, bc="periodic", potential=pot)
+
+
+def traffic_from_profiles():
+    """HBM bytes per Chebyshev-step launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "cheb_step_128cube_f64_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(A, gpu_counts, max_inner=1):
+    """Oracle on a bounded sample of the same solve, extrapolated with the GPU counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    r = orc.solve(A, K, TOL, INC, BUF, seed=0, stop="RESIDUAL_ONLY", max_inner=max_inner)
+    wall = time.time() - t0
+    t = r["timing"]
+    per_deg = t["filter"] / max(t["degree_sum"], 1)
+    per_ortho = t["ortho"] / max(t["n_ortho"], 1)
+    per_defl = t["deflate"] / max(t["n_deflate"], 1)
+    per_rr = t["rr"] / max(t["n_rr"], 1)
+    est = (t["lanczos"] + per_deg * gpu_counts["degrees"] + per_ortho * gpu_counts["ortho"]
+           + (per_defl + per_rr) * gpu_counts["rr"])
+    return {
+        "value": K / est,
+        "unit": "eigenpairs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"CPU oracle (restated RayleighChebyshevLM, OpenMP, {threads} threads) on the "
+                   f"same 128^3 operator: Lanczos bounds + {max_inner} inner iteration(s) per outer "
+                   f"pass, {wall:.1f} s wall; per-degree filter {per_deg:.3f} s, per-MGS "
+                   f"{per_ortho:.3f} s, per-deflation {per_defl:.3f} s, per-Rayleigh-Ritz "
+                   f"{per_rr:.3f} s, scaled by the GPU solve's {gpu_counts['degrees']} filter "
+                   f"degrees, {gpu_counts['ortho']} orthogonalisations and {gpu_counts['rr']} "
+                   f"Rayleigh-Ritz steps to an estimated {est:.1f} s solve"),
+    }
+
+
+def kernel_only(args):
+    """Filter steps alone (for rocprofv3 counter runs) plus a plain copy of
+    known size whose FETCH_SIZE/WRITE_SIZE calibrate the counters."""
+    import ctypes as C
+
+    import torch
+
+    from rayleighchebyshevgpgpu_amd import capi, solver
+    A = build_operator(args.grid)
+    ctx = solver.Context(0)
+    op = solver.CsrOperator(A, ctx)
+    X = solver.Block(A.n, K, ctx=ctx)
+    T = solver.Block(A.n, K, ctx=ctx)
+    X.randomize(0, K, 0, 0)
+    m = args.kernel_only + (args.kernel_only % 2)
+    capi.check(capi.lib().rcg_cheb_filter(op.h, X.h, T.h, K, m, 1, 14.2, 2.0))
+    ctx.synchronize()
+    src = torch.empty(2 ** 28, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    csr = A.nnz * 12 + 4 * (A.n + 1)
+    print(json.dumps({"mode": "kernel-only", "cheb_steps": m, "n": A.n, "nnz": A.nnz, "b": K,
+                      "alg_bytes_per_step": csr + 3 * A.n * K * 8,
+                      "alg_bytes_first_step": csr + 2 * A.n * K * 8,
+                      "copy_bytes_read": 2 ** 31, "copy_bytes_written": 2 ** 31}), flush=True)
+    del C
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=GRID)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--kernel-only", type=int, default=0, metavar="M",
+                    help="profiling mode: run M Chebyshev steps of the filter on the 64-column "
+                         "block and one 2 GiB device copy (the HBM calibration kernel), no solve")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve runs "
+              "on a single MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    from rayleighchebyshevgpgpu_amd import capi, solver
+
+    if args.kernel_only:
+        return kernel_only(args)
+
+    t0 = time.time()
+    A = build_operator(args.grid)
+    log(f"operator n={A.n} nnz={A.nnz} built in {time.time() - t0:.1f}s")
+    ctx = solver.Context(0)
+    op = solver.CsrOperator(A, ctx)
+    V = solver.Block(A.n, K, ctx=ctx)
+
+    def one_solve():
+        rc = solver.RayleighChebyshevLM()
+        rc.setStopCondition("RESIDUAL_ONLY")
+        ev = []
+        t = time.perf_counter()
+        f = rc.getMinEigenSystem(K, TOL, INC, BUF, None, op, solver.RandomizeOp(0), ev, V)
+        ctx.synchronize()
+        return time.perf_counter() - t, f, np.array(ev), rc
+
+    for w in range(args.warmup):
+        dt, f, _, _ = one_solve()
+        log(f"warmup {w}: {dt:.3f}s found={f}")
+
+    ctx.synchronize()
+    lat, results = [], []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        dt, f, ev, rc = one_solve()
+        lat.append(dt)
+        results.append((f, ev, rc.getResult()))
+        log(f"step {s}: {dt:.3f}s found={f} iters={rc.getResult().count_eig} "
+            f"degrees={rc.getResult().filter_degree_sum}")
+    ctx.synchronize()
+    total = time.perf_counter() - t_start
+
+    # residual check of the last solve's pairs on the device (outside the timed region)
+    f, ev, res = results[-1]
+    W = solver.Block(A.n, K, ctx=ctx)
+    import ctypes as C
+    r = np.zeros(f)
+    capi.check(capi.lib().rcg_operator_apply(op.h, V.h, W.h, 1.0, 0.0, f))
+    capi.check(capi.lib().rcg_residuals(op.h, V.h, W.h, f, ev.ctypes.data_as(C.c_void_p),
+                                        r.ctypes.data_as(C.c_void_p)))
+    normA = max(abs(res.min_est), abs(res.max_est))
+    rel = r / normA
+    converged_per_solve = int(np.sum(rel <= 1e-10))
+    converged_total = sum(int(x[0]) for x in results) * converged_per_solve / max(f, 1)
+    value = converged_total / total
+
+    # dominant kernel: the fused Chebyshev step, HIP-event timed inside the solves
+    launches = sum(x[2].filter_degree_sum for x in results)
+    t_dev = sum(x[2].t_filter_device for x in results)
+    alg_bytes = sum(x[2].filter_bytes for x in results)
+    avg_launch_s = t_dev / max(launches, 1)
+    bytes_per_launch = alg_bytes / max(launches, 1)
+    achieved = alg_bytes / t_dev / 1e9 if t_dev > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(),
+                "kernel": "csr_block_kernel<double,32,1,kCheb> (fused Chebyshev step)",
+                "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                "alg_bytes_per_launch": int(bytes_per_launch), "launches": int(launches)}
+
+    lat_ms = np.array(lat) * 1e3
+    last = results[-1][2]
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"lowest-eigenpair solve, 3-D 7-point periodic Laplacian {args.grid}^3 + "
+                         "Anderson U[-2,2] (seed 1), k=64 inc=32 buf=32 tol=1e-10 residual-only, "
+                         "FP64 filter"),
+            "n": A.n, "nnz": A.nnz, "k": K, "inc": INC, "buf": BUF, "tol": TOL,
+            "precision": "fp64",
+        },
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
+        "solve": {"found": int(f), "converged": converged_per_solve,
+                  "max_rel_residual": float(rel.max()) if f else None,
+                  "inner_iterations": int(last.count_eig), "filter_degrees": int(last.filter_degree_sum),
+                  "lanczos_iters": int(last.lanczos_iters),
+                  "t_lanczos_s": round(last.t_lanczos, 4), "t_filter_device_s": round(last.t_filter_device, 4),
+                  "t_ortho_s": round(last.t_ortho, 4), "t_rr_s": round(last.t_rr, 4),
+                  "lambda_min": float(ev[0]) if f else None, "lambda_k": float(ev[-1]) if f else None},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        log("cpu baseline sample ...")
+        counts = {"degrees": int(last.filter_degree_sum), "ortho": int(last.count_ortho),
+                  "rr": int(last.count_eig)}
+        line["cpu_baseline"] = cpu_baseline(A, counts, args.cpu_iters)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -185,8 +185,10 @@ typedef struct rcg_solve_result {
   double final_maxres, final_maxdiff, final_maxgap; /* getFinalData */
   int64_t trace_len;
   uint64_t rng_counter;
-  int64_t filter_degree_sum; /* sum of starDegree over filter applications */
-  double t_total, t_lanczos, t_filter, t_ortho, t_rr; /* seconds */
+  int64_t filter_degree_sum; /* Chebyshev step launches (sum of starDegree * rep) */
+  double t_total, t_lanczos, t_filter, t_ortho, t_rr; /* host wall seconds per phase */
+  double t_filter_device;    /* seconds between hipEvents around the filter steps */
+  double filter_bytes;       /* algorithmic bytes of those steps (CSR + 3 block passes) */
 } rcg_solve_result;
 
 /* RayleighChebyshevLM::getMinEigenSystem (RayleighChebyshevLM.h:693-738;

@@ -31,7 +31,10 @@ class Result(C.Structure):
         ("max_est", C.c_double), ("guard", C.c_double), ("count_ortho", C.c_long),
         ("count_apply", C.c_long), ("count_eig", C.c_long), ("final_maxres", C.c_double),
         ("final_maxdiff", C.c_double), ("final_maxgap", C.c_double), ("trace_len", C.c_long),
-        ("rng_counter", C.c_ulonglong),
+        ("rng_counter", C.c_ulonglong), ("t_lanczos", C.c_double), ("t_filter", C.c_double),
+        ("t_ortho", C.c_double), ("t_deflate", C.c_double), ("t_rr", C.c_double),
+        ("n_ortho", C.c_long), ("n_deflate", C.c_long), ("n_rr", C.c_long),
+        ("degree_sum", C.c_long),
     ]
 
 
@@ -235,6 +238,9 @@ def solve(A, k, tol, inc, buf, seed=0, stop="COMBINATION", interval=False, lambd
                   "maxRelEigValueGap": r.final_maxgap},
         "trace": trace[: min(r.trace_len, trace_cap)].copy(),
         "rng_counter": r.rng_counter,
+        "timing": {"lanczos": r.t_lanczos, "filter": r.t_filter, "ortho": r.t_ortho,
+                   "deflate": r.t_deflate, "rr": r.t_rr, "n_ortho": r.n_ortho,
+                   "n_deflate": r.n_deflate, "n_rr": r.n_rr, "degree_sum": r.degree_sum},
     }
     if want_vectors:
         out["eigenvectors"] = evecs[:, :f].copy()

@@ -822,6 +822,16 @@ class RCLM {
   std::map<std::string, double> finalData;
   std::map<std::string, long> countData;
   std::vector<TraceRow> trace;
+  // wall seconds per phase (timing of the CPU baseline; not part of the algorithm)
+  double tLanczos = 0.0, tFilter = 0.0, tOrtho = 0.0, tDeflate = 0.0, tRR = 0.0;
+  long nOrtho = 0, nDeflate = 0, nRR = 0, degreeSum = 0;
+  static double now() {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+  }
 
   static double relErrFactor(double val, double tol) {
     const double small = 1.0e-11;  // RayleighChebyshevLM.h:202
@@ -832,8 +842,10 @@ class RCLM {
   long getMinEigenSystem(long eigCount, double tol, long inc, long buf, std::vector<T>& vStart,
                          const Op& op, RandomStream& rnd, std::vector<double>& eigValues,
                          Block<T>& eigVectors) {
+    double t0 = now();
     LanczosResult lr =
         lanczos_min_max<T>(op, vStart, rnd, maxMinTol, true, false, lanczosIterMax);
+    tLanczos += now() - t0;
     lanczosIters = lr.iters;
     double minEig = lr.minEig, maxEig = lr.maxEig;
     minEigValueEst = minEig;
@@ -856,8 +868,10 @@ class RCLM {
   long getMinIntervalEigenSystem(double lambdaMax, double tol, long inc, long buf, long maxDim,
                                  std::vector<T>& vStart, const Op& op, RandomStream& rnd,
                                  std::vector<double>& eigValues, Block<T>& eigVectors) {
+    double t0 = now();
     LanczosResult lr =
         lanczos_min_max<T>(op, vStart, rnd, maxMinTol, true, true, lanczosIterMax);
+    tLanczos += now() - t0;
     lanczosIters = lr.iters;
     double minEig = lr.minEig, maxEig = lr.maxEig;
     minEigValueEst = minEig;
@@ -1040,8 +1054,11 @@ class RCLM {
       for (long k = buf; k < subspaceSize; ++k) oldEigs[k] = oldEigs[buf];
       if (applyCountCumulative > 0)
         for (long k = buf; k < subspaceSize; ++k) rnd.randomize(vArray[k]);
+      double tp = now();
       orthogonalize(vArray);
       orthogonalize(vArray);
+      tOrtho += now() - tp;
+      nOrtho += 2;
       countData["ortho"] += 2;
 
       lambdaStar = maxEigValue;
@@ -1082,17 +1099,28 @@ class RCLM {
         }
         Cpoly fp;
         fp.set(starDegree, filterRepetitionCount, starBound, shift);
-        if (!completedBasisFlag)
+        double tp = now();
+        if (!completedBasisFlag) {
           filter_apply<T>(op, vArray, starDegree, filterRepetitionCount, starBound, shift,
                           fp.UpperXStar);
+          degreeSum += starDegree * filterRepetitionCount;
+        }
+        tFilter += now() - tp;
         applyCount += 1;
         countData["OpApply"] += starDegree * subspaceSize;
 
+        tp = now();
         if (foundSize > 0) orthogonalizeAtoB(vArray, 0, subspaceSize - 1, eigVectors, 0, foundSize - 1);
         else orthogonalizeAtoB(vArray, 0, subspaceSize - 1, eigVectors, 0, -1);
+        tDeflate += now() - tp;
+        nDeflate += 1;
+        tp = now();
         orthogonalize(vArray);
+        tOrtho += now() - tp;
+        nOrtho += 1;
         countData["ortho"] += 1;
 
+        tp = now();
         formVtAV(op, vArray, H);
         countData["OpApply"] += subspaceSize;
         smallEig(H, subspaceSize, theta, Y);
@@ -1100,6 +1128,8 @@ class RCLM {
         residualCheckCount = (foundSize + subspaceSize < N) ? inc : subspaceSize;
         subspaceResiduals.clear();
         eigenVectorsAndResiduals(op, Y, theta, vArray, residualCheckCount, subspaceResiduals);
+        tRR += now() - tp;
+        nRR += 1;
         countData["OpApply"] += residualCheckCount;
 
         maxResidual = 0.0;

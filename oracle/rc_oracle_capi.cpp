@@ -75,6 +75,8 @@ struct orc_result {
   double final_maxgap;
   long trace_len;
   unsigned long long rng_counter;
+  double t_lanczos, t_filter, t_ortho, t_deflate, t_rr;  // wall seconds per phase
+  long n_ortho, n_deflate, n_rr, degree_sum;
 };
 
 const char* orc_last_error() { return g_err.c_str(); }
@@ -235,6 +237,15 @@ int solve_impl(const Op& op, long n, const orc_params* p, orc_result* r, double*
     r->final_maxgap = s.finalData["maxRelEigValueGap"];
     r->trace_len = (long)s.trace.size();
     r->rng_counter = rnd.counter;
+    r->t_lanczos = s.tLanczos;
+    r->t_filter = s.tFilter;
+    r->t_ortho = s.tOrtho;
+    r->t_deflate = s.tDeflate;
+    r->t_rr = s.tRR;
+    r->n_ortho = s.nOrtho;
+    r->n_deflate = s.nDeflate;
+    r->n_rr = s.nRR;
+    r->degree_sum = s.degreeSum;
     const long cap = p->out_cap;
     for (long i = 0; i < (long)ev.size() && i < cap; ++i) evals[i] = ev[i];
     if (resid)

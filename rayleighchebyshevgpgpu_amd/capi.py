@@ -37,7 +37,7 @@ class SolveResult(C.Structure):
         ("final_maxdiff", C.c_double), ("final_maxgap", C.c_double), ("trace_len", C.c_int64),
         ("rng_counter", C.c_uint64), ("filter_degree_sum", C.c_int64), ("t_total", C.c_double),
         ("t_lanczos", C.c_double), ("t_filter", C.c_double), ("t_ortho", C.c_double),
-        ("t_rr", C.c_double),
+        ("t_rr", C.c_double), ("t_filter_device", C.c_double), ("filter_bytes", C.c_double),
     ]
 
 

@@ -460,7 +460,16 @@ struct TraceRow {
 
 class Solver {
  public:
-  Solver(rcg_operator_s* op, const rcg_solve_params& p) : op_(op), p_(p), eng_(op->ctx, op, op->n) {}
+  Solver(rcg_operator_s* op, const rcg_solve_params& p) : op_(op), p_(p), eng_(op->ctx, op, op->n) {
+    RCG_HIP(hipEventCreate(&ev_[0]));
+    RCG_HIP(hipEventCreate(&ev_[1]));
+  }
+  ~Solver() {
+    (void)hipEventDestroy(ev_[0]);
+    (void)hipEventDestroy(ev_[1]);
+  }
+  Solver(const Solver&) = delete;
+  Solver& operator=(const Solver&) = delete;
 
   static double rel(double val, double tol) {
     return (std::abs(val) * tol > 1.0e-11) ? std::abs(val) : 1.0e-11 / tol;
@@ -618,18 +627,31 @@ class Solver {
         }
         ta = wall();
         if (!completedBasis) {
+          RCG_HIP(hipEventRecord(ev_[0], eng_.stream()));
           eng_.filter(subspaceSize, starDegree, rep, starBound, shift);
+          RCG_HIP(hipEventRecord(ev_[1], eng_.stream()));
           degree_sum_ += starDegree * rep;
+          // algorithmic bytes: CSR stream + X_{j-1} read + own-row reads/writes
+          const double csr = double(op_->nnz) * 12.0 + 4.0 * double(op_->n + 1);
+          const double blk = 8.0 * double(op_->n) * double(subspaceSize);
+          filter_bytes_ += double(rep) * (csr + 2.0 * blk + double(starDegree - 1) * (csr + 3.0 * blk));
+          pending_event_ = true;
         }
         applyCount += 1;
         count_apply_ += starDegree * subspaceSize;
-        t_filter_ += wall() - ta;  // enqueue time; the device time lands in the next sync
+        t_filter_ += wall() - ta;  // enqueue time; device time is read after the next sync
 
         ta = wall();
         eng_.deflate(subspaceSize, foundSize);
         eng_.orthonormalize(subspaceSize);
         count_ortho_ += 1;
         t_ortho_ += wall() - ta;
+        if (pending_event_) {  // orthonormalize synchronized the stream
+          float ms = 0.f;
+          RCG_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+          t_filter_dev_ += 1.0e-3 * ms;
+          pending_event_ = false;
+        }
 
         ta = wall();
         eng_.project(subspaceSize, H);
@@ -788,6 +810,8 @@ class Solver {
     R->t_ortho = t_ortho_;
     R->t_rr = t_rr_;
     R->filter_degree_sum = degree_sum_;
+    R->t_filter_device = t_filter_dev_;
+    R->filter_bytes = filter_bytes_;
     return foundSize;
   }
 
@@ -837,6 +861,9 @@ class Solver {
   long found_ = 0;
   long count_ortho_ = 0, count_apply_ = 0, count_eig_ = 0, degree_sum_ = 0;
   double guard_ = 0.0, t_filter_ = 0.0, t_ortho_ = 0.0, t_rr_ = 0.0;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  bool pending_event_ = false;
+  double t_filter_dev_ = 0.0, filter_bytes_ = 0.0;
 };
 
 void check_pair(rcg_operator op, rcg_block a, rcg_block b, int64_t cols) {
