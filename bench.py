@@ -96,12 +96,9 @@ def cpu_baseline(A, gpu_counts, max_inner=1):
 
 
 def kernel_only(args):
-    """Filter steps alone (for rocprofv3 counter runs) plus a plain copy of
-    known size whose FETCH_SIZE/WRITE_SIZE calibrate the counters."""
-    import ctypes as C
-
-    import torch
-
+    """Filter steps alone (for rocprofv3 counter runs) plus a device-to-device
+    copy of the 64-column block (a plain copy kernel of known size) whose
+    FETCH_SIZE/WRITE_SIZE calibrate the counters."""
     from rayleighchebyshevgpgpu_amd import capi, solver
     A = build_operator(args.grid)
     ctx = solver.Context(0)
@@ -111,17 +108,14 @@ def kernel_only(args):
     X.randomize(0, K, 0, 0)
     m = args.kernel_only + (args.kernel_only % 2)
     capi.check(capi.lib().rcg_cheb_filter(op.h, X.h, T.h, K, m, 1, 14.2, 2.0))
+    capi.check(capi.lib().rcg_block_copy(T.h, X.h))
     ctx.synchronize()
-    src = torch.empty(2 ** 28, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
     csr = A.nnz * 12 + 4 * (A.n + 1)
+    blk = A.n * X.ld * 8
     print(json.dumps({"mode": "kernel-only", "cheb_steps": m, "n": A.n, "nnz": A.nnz, "b": K,
                       "alg_bytes_per_step": csr + 3 * A.n * K * 8,
                       "alg_bytes_first_step": csr + 2 * A.n * K * 8,
-                      "copy_bytes_read": 2 ** 31, "copy_bytes_written": 2 ** 31}), flush=True)
-    del C
+                      "copy_bytes_read": blk, "copy_bytes_written": blk}), flush=True)
 
 
 def main():

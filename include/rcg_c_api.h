@@ -105,6 +105,11 @@ int rcg_block_info(rcg_block b, int64_t* n, int64_t* cols, int64_t* ld, int* dty
 int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32_t* rowptr,
                             const int32_t* col, const void* val, int val_dtype, int mem,
                             rcg_operator* out);
+/* No counterpart in the reference: a locality hint for operators on a
+ * structured nx x ny x nz grid whose rows are numbered x + nx*(y + ny*z).
+ * The CSR kernels may then visit x-lines in (y, z) tiles; results do not
+ * depend on it.  The visiting order is autotuned on first use. */
+int rcg_operator_set_grid(rcg_operator op, int64_t nx, int64_t ny, int64_t nz);
 int rcg_operator_destroy(rcg_operator op);
 /* Otype::apply(in, out, alpha, beta): out = alpha*A*in + beta*out over the
  * first `cols` columns (blocks share n and ld). */
@@ -189,6 +194,7 @@ typedef struct rcg_solve_result {
   double t_total, t_lanczos, t_filter, t_ortho, t_rr; /* host wall seconds per phase */
   double t_filter_device;    /* seconds between hipEvents around the filter steps */
   double filter_bytes;       /* algorithmic bytes of those steps (CSR + 3 block passes) */
+  int64_t filter_panel;      /* autotuned column-panel width of the filter step */
 } rcg_solve_result;
 
 /* RayleighChebyshevLM::getMinEigenSystem (RayleighChebyshevLM.h:693-738;

@@ -38,6 +38,7 @@ class SolveResult(C.Structure):
         ("rng_counter", C.c_uint64), ("filter_degree_sum", C.c_int64), ("t_total", C.c_double),
         ("t_lanczos", C.c_double), ("t_filter", C.c_double), ("t_ortho", C.c_double),
         ("t_rr", C.c_double), ("t_filter_device", C.c_double), ("filter_bytes", C.c_double),
+        ("filter_panel", C.c_int64),
     ]
 
 
@@ -76,6 +77,7 @@ def lib() -> C.CDLL:
         "rcg_block_info": ([vp, P(i64), P(i64), P(i64), P(i32), P(vp)], i32),
         "rcg_operator_csr_create": ([vp, i64, i64, vp, vp, vp, i32, i32, P(vp)], i32),
         "rcg_operator_destroy": ([vp], i32),
+        "rcg_operator_set_grid": ([vp, i64, i64, i64], i32),
         "rcg_operator_apply": ([vp, vp, vp, d, d, i64], i32),
         "rcg_cheb_filter": ([vp, vp, vp, i64, i64, i64, d, d], i32),
         "rcg_orthonormalize": ([vp, vp, i64], i32),

@@ -45,19 +45,31 @@ inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 // -------------------------------------------------------------- launchers
 // All pointers are device pointers; every launch goes on `st`.
 
+// Order in which the CSR kernels visit row blocks.  The rows computed and
+// their arithmetic do not depend on it; it only changes which rows are in
+// flight together (cache locality).  order == nullptr: contiguous blocks.
+struct CsrSched {
+  const int* order = nullptr;  // device: logical block -> row block
+  int64_t rows_per_blk = 0;    // rows per block when `order` is set
+  int64_t nblk = 0;            // number of row blocks when `order` is set
+  int remap = 1;               // 1: each XCD takes a contiguous range of logical blocks
+};
+
 // out = alpha*A*X + beta*out over `ncols` columns (ld shared by X and out).
 template <typename T>
 void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val, const T* X,
-                      T* out, int ld, int ncols, T alpha, T beta, hipStream_t st);
+                      T* out, int ld, int ncols, T alpha, T beta, hipStream_t st,
+                      const CsrSched* sched = nullptr);
 
 // One three-term Chebyshev step of the scaled C polynomial:
 //   first: out = s * g2*(g1*A*X1 + X1)
 //   else : out = s * (g2*(g1*A*X1 + X1) - X0)
 // `out` may alias X0 (each row reads its own X0 entries before writing them).
+// `panel` > 0 splits the columns into launches of that width (a multiple of 4).
 template <typename T>
 void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val, const T* X1,
                       const T* X0, T* out, int ld, int ncols, T g1, T g2, T s, bool first,
-                      hipStream_t st);
+                      hipStream_t st, int panel = 0, const CsrSched* sched = nullptr);
 
 // G (bx x by, column-major, ldg = bx) = X^H Y over n rows; ws holds partials.
 template <typename T>

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "rcg_internal.hpp"
 
@@ -29,37 +30,80 @@ constexpr int kThreads = 256;
 // hardware deals workgroups round-robin to the 8 XCDs; this mapping gives
 // XCD x the logical blocks [x*nb/8, (x+1)*nb/8), i.e. contiguous rows.
 __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
+#ifdef RCG_NO_XCD_REMAP
+  return bid;
+#else
   const int64_t per = nb / kXcds;
   return (bid % kXcds) * per + bid / kXcds;
+#endif
 }
+
+#ifdef RCG_NT_STREAMS
+constexpr bool kNtStreams = true;
+#else
+constexpr bool kNtStreams = false;
+#endif
 
 // 16-byte vector access to a row segment.
 template <typename T>
 struct V16;
+// ld/st go through the caches normally; ld_stream/st_stream mark single-use
+// data (the X_{j-2} read and the X_j write of a step) non-temporal when
+// RCG_NT_STREAMS is set, so L2 keeps the gathered X_{j-1} rows.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
 template <>
 struct V16<double> {
   static constexpr int W = 2;
   __device__ static void ld(const double* p, double (&o)[2]) {
-    const double2 t = *reinterpret_cast<const double2*>(p);
+    const dvec2 t = *reinterpret_cast<const dvec2*>(p);
     o[0] = t.x;
     o[1] = t.y;
   }
   __device__ static void st(double* p, const double (&o)[2]) {
-    *reinterpret_cast<double2*>(p) = make_double2(o[0], o[1]);
+    *reinterpret_cast<dvec2*>(p) = dvec2{o[0], o[1]};
+  }
+  __device__ static void ld_stream(const double* p, double (&o)[2]) {
+    if constexpr (kNtStreams) {
+      const dvec2 t = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p));
+      o[0] = t.x;
+      o[1] = t.y;
+    } else {
+      ld(p, o);
+    }
+  }
+  __device__ static void st_stream(double* p, const double (&o)[2]) {
+    if constexpr (kNtStreams) __builtin_nontemporal_store(dvec2{o[0], o[1]}, reinterpret_cast<dvec2*>(p));
+    else st(p, o);
   }
 };
 template <>
 struct V16<float> {
   static constexpr int W = 4;
   __device__ static void ld(const float* p, float (&o)[4]) {
-    const float4 t = *reinterpret_cast<const float4*>(p);
+    const fvec4 t = *reinterpret_cast<const fvec4*>(p);
     o[0] = t.x;
     o[1] = t.y;
     o[2] = t.z;
     o[3] = t.w;
   }
   __device__ static void st(float* p, const float (&o)[4]) {
-    *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+    *reinterpret_cast<fvec4*>(p) = fvec4{o[0], o[1], o[2], o[3]};
+  }
+  __device__ static void ld_stream(const float* p, float (&o)[4]) {
+    if constexpr (kNtStreams) {
+      const fvec4 t = __builtin_nontemporal_load(reinterpret_cast<const fvec4*>(p));
+      o[0] = t.x;
+      o[1] = t.y;
+      o[2] = t.z;
+      o[3] = t.w;
+    } else {
+      ld(p, o);
+    }
+  }
+  __device__ static void st_stream(float* p, const float (&o)[4]) {
+    if constexpr (kNtStreams) __builtin_nontemporal_store(fvec4{o[0], o[1], o[2], o[3]}, reinterpret_cast<fvec4*>(p));
+    else st(p, o);
   }
 };
 
@@ -72,12 +116,14 @@ template <typename T, int G, int NV, int MODE>
 __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
-    T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk) {
+    T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk, const int* __restrict__ order,
+    int remap) {
   using VT = V16<T>;
   constexpr int W = VT::W;
   constexpr int RS = kThreads / G;
-  const int64_t lb = xcd_remap(blockIdx.x, gridDim.x);
-  if (lb >= nblk) return;
+  const int64_t lq = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  if (lq >= nblk) return;
+  const int64_t lb = order ? (int64_t)order[lq] : lq;
   const int g = threadIdx.x % G;
   const int r = threadIdx.x / G;
   const int64_t i0 = lb * rows_per_blk;
@@ -90,38 +136,40 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
       for (int w = 0; w < W; ++w) acc[k][w] = T(0);
     const int pb = rowptr[i];
     const int pe = rowptr[i + 1];
-    int p = pb;
-    for (; p + 1 < pe; p += 2) {
-      const int j0 = col[p];
-      const int j1 = col[p + 1];
-      const T a0 = val[p];
-      const T a1 = val[p + 1];
-      const T* x0 = X + (int64_t)j0 * ld;
-      const T* x1 = X + (int64_t)j1 * ld;
+    // Chunks of 4 nonzeros: the gathers of a chunk are issued before its FMAs.
+    for (int p = pb; p < pe; p += 4) {
+      int jj[4];
+      T aa[4];
 #pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        const int cv = g + k * G;
-        if (cv < ncv) {
-          T u0[W], u1[W];
-          VT::ld(x0 + cv * W, u0);
-          VT::ld(x1 + cv * W, u1);
-#pragma unroll
-          for (int w = 0; w < W; ++w) acc[k][w] = fma(a1, u1[w], fma(a0, u0[w], acc[k][w]));
+      for (int q = 0; q < 4; ++q) {
+        const bool live = p + q < pe;
+        if constexpr (kNtStreams) {
+          jj[q] = live ? __builtin_nontemporal_load(col + p + q) : 0;
+          aa[q] = live ? __builtin_nontemporal_load(val + p + q) : T(0);
+        } else {
+          jj[q] = live ? col[p + q] : 0;
+          aa[q] = live ? val[p + q] : T(0);
         }
       }
-    }
-    if (p < pe) {
-      const int j0 = col[p];
-      const T a0 = val[p];
-      const T* x0 = X + (int64_t)j0 * ld;
+      T u[4][NV][W];
 #pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        const int cv = g + k * G;
-        if (cv < ncv) {
-          T u0[W];
-          VT::ld(x0 + cv * W, u0);
+      for (int q = 0; q < 4; ++q) {
+        if (p + q < pe) {
+          const T* xr = X + (int64_t)jj[q] * ld;
 #pragma unroll
-          for (int w = 0; w < W; ++w) acc[k][w] = fma(a0, u0[w], acc[k][w]);
+          for (int k = 0; k < NV; ++k) {
+            const int cv = g + k * G;
+            if (cv < ncv) VT::ld(xr + cv * W, u[q][k]);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (p + q < pe) {
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int w = 0; w < W; ++w) acc[k][w] = fma(aa[q], u[q][k][w], acc[k][w]);
         }
       }
     }
@@ -147,47 +195,60 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
       } else {
         T xs[W], zs[W];
         VT::ld(X + off, xs);
-        VT::ld(Z + off, zs);
+        VT::ld_stream(Z + off, zs);
 #pragma unroll
         for (int w = 0; w < W; ++w) o[w] = p2 * fma(p1, fma(p0, acc[k][w], xs[w]), -zs[w]);
       }
-      VT::st(out + off, o);
+      VT::st_stream(out + off, o);
     }
   }
 }
 
 template <typename T, int MODE, int G, int NV>
 void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
-                  T* out, int ld, int ncv, T p0, T p1, T p2, hipStream_t st) {
+                  T* out, int ld, int ncv, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc) {
   constexpr int RS = kThreads / G;
-  // ~16 row-passes per workgroup keeps the grid >> 256 CUs at large n.
-  int64_t rows_per_blk = (int64_t)RS * 16;
-  int64_t nblk = (n + rows_per_blk - 1) / rows_per_blk;
-  if (nblk < 2048) {
-    rows_per_blk = RS;
+  int64_t rows_per_blk, nblk;
+  const int* order = nullptr;
+  if (sc && sc->order) {
+    rows_per_blk = sc->rows_per_blk;
+    nblk = sc->nblk;
+    order = sc->order;
+  } else {
+    // ~16 row-passes per workgroup keeps the grid >> 256 CUs at large n.
+    rows_per_blk = (int64_t)RS * 16;
     nblk = (n + rows_per_blk - 1) / rows_per_blk;
+    if (nblk < 2048) {
+      rows_per_blk = RS;
+      nblk = (n + rows_per_blk - 1) / rows_per_blk;
+    }
   }
+  const int remap = sc ? sc->remap : 1;
   const int64_t grid = round_up(nblk, kXcds);
   hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE>), dim3((unsigned)grid), dim3(kThreads), 0,
-                     st, n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, rows_per_blk, nblk);
+                     st, n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, rows_per_blk, nblk, order,
+                     remap);
   RCG_HIP(hipGetLastError());
 }
 
 template <typename T, int MODE>
 void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
-                T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st) {
+                T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc) {
   constexpr int W = V16<T>::W;
   const int ncv = (ncols + W - 1) / W;
   if (ld % 4 != 0) throw Error(1, "block leading dimension must be a multiple of 4");
-  if (ncv <= 1) return csr_launch_g<T, MODE, 1, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
-  if (ncv <= 2) return csr_launch_g<T, MODE, 2, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
-  if (ncv <= 4) return csr_launch_g<T, MODE, 4, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
-  if (ncv <= 8) return csr_launch_g<T, MODE, 8, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
-  if (ncv <= 16) return csr_launch_g<T, MODE, 16, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
-  if (ncv <= 32) return csr_launch_g<T, MODE, 32, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
-  if (ncv <= 64) return csr_launch_g<T, MODE, 64, 1>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
-  if (ncv <= 128) return csr_launch_g<T, MODE, 64, 2>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
-  if (ncv <= 256) return csr_launch_g<T, MODE, 64, 4>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st);
+#define RCG_CSR_CASE(LIM, GG, NN) \
+  if (ncv <= LIM) return csr_launch_g<T, MODE, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st, sc);
+  RCG_CSR_CASE(1, 1, 1)
+  RCG_CSR_CASE(2, 2, 1)
+  RCG_CSR_CASE(4, 4, 1)
+  RCG_CSR_CASE(8, 8, 1)
+  RCG_CSR_CASE(16, 16, 1)
+  RCG_CSR_CASE(32, 32, 1)
+  RCG_CSR_CASE(64, 64, 1)
+  RCG_CSR_CASE(128, 64, 2)
+  RCG_CSR_CASE(256, 64, 4)
+#undef RCG_CSR_CASE
   throw Error(6, "CSR block kernels support at most 256 16-byte column vectors per row");
 }
 
@@ -253,17 +314,34 @@ __global__ __launch_bounds__(kThreads) void gram_partial_kernel(
     }
 }
 
-// out[o] = sum_b P[b*m + o] in block order.
+// out[o] = sum_b P[b*m + o].  A workgroup owns 64 outputs; its 4 waves take
+// the partial blocks b = s (mod 4) with four interleaved accumulators each and
+// are combined in a fixed order, so the result does not depend on scheduling.
 template <typename T, typename TO>
 __global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const T* __restrict__ P,
                                                                    int64_t nblk, int64_t m,
                                                                    TO* __restrict__ out) {
-  const int64_t o = blockIdx.x * (int64_t)kThreads + threadIdx.x;
-  if (o >= m) return;
-  T s = T(0);
-  for (int64_t b = 0; b < nblk; ++b) s += P[b * m + o];
-  out[o] = TO(s);
+  __shared__ T red[4][64];
+  const int ol = threadIdx.x % 64;
+  const int sl = threadIdx.x / 64;
+  const int64_t o = blockIdx.x * (int64_t)64 + ol;
+  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+  if (o < m) {
+    int64_t b = sl;
+    for (; b + 12 < nblk; b += 16) {
+      a0 += P[b * m + o];
+      a1 += P[(b + 4) * m + o];
+      a2 += P[(b + 8) * m + o];
+      a3 += P[(b + 12) * m + o];
+    }
+    for (; b < nblk; b += 4) a0 += P[b * m + o];
+  }
+  red[sl][ol] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  if (sl == 0 && o < m) out[o] = TO((red[0][ol] + red[1][ol]) + (red[2][ol] + red[3][ol]));
 }
+
+inline unsigned reduce_grid(int64_t m) { return (unsigned)((m + 63) / 64); }
 
 template <typename T>
 int64_t gram_blocks(int64_t n, int bx, int by, int64_t* rows_per_blk) {
@@ -332,6 +410,154 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_kernel(
       const int64_t off = i * ldo + c;
       T v = alpha * acc[p][q];
       if (beta != T(0)) v = fma(beta, Z[off], v);
+      out[off] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------- FP64 MFMA versions
+// v_mfma_f64_16x16x4f64 computes D(16x16) += A(16x4) B(4x16); lane l supplies
+// A[l%16][l/16] and B[l/16][l%16] and holds D[4j + l/16][l%16] in register j.
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4v mfma16x16x4(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Partial X^T Y of one 64x64 output tile over a row range.  Wave w takes the
+// 4-row groups i0 + 4w + 16t; lane (kr = l/16, cc = l%16) loads
+// X[row+kr][a0+16p+cc] and Y[row+kr][b0+16q+cc] -- 128-byte row segments --
+// which are exactly the A and B fragments of the 16 MFMA subtiles.  SYM
+// (X == Y, one tile) multiplies only subtiles p <= q and mirrors the rest.
+template <bool SYM>
+__global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
+    int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ Y, int ldy,
+    int by, double* __restrict__ P, int64_t rows_per_blk, int nta) {
+  __shared__ double red[kTile * kTile];
+  const int tile = blockIdx.y;
+  const int a0 = (tile % nta) * kTile;
+  const int b0 = (tile / nta) * kTile;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kr = lane >> 4;
+  const int cc = lane & 15;
+  const int64_t i0 = blockIdx.x * rows_per_blk;
+  const int64_t i1 = min(n, i0 + rows_per_blk);
+  d4v acc[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = i0 + 4 * wave; i < i1; i += 16) {
+    const int64_t row = i + kr;
+    const bool ok = row < i1;
+    double xa[4], yb[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c = a0 + 16 * t + cc;
+      xa[t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
+    }
+    if constexpr (SYM) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) yb[t] = xa[t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int c = b0 + 16 * t + cc;
+        yb[t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (!SYM || p <= q) acc[p][q] = mfma16x16x4(xa[p], yb[q], acc[p][q]);
+  }
+  // the four waves' tiles are summed in LDS in wave order
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (SYM && p > q) continue;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int idx = (16 * p + 4 * j + kr) + kTile * (16 * q + cc);
+            red[idx] = (w == 0) ? acc[p][q][j] : red[idx] + acc[p][q][j];
+          }
+        }
+    }
+    __syncthreads();
+  }
+  double* Pb = P + blockIdx.x * (int64_t)bx * by;
+  for (int e = threadIdx.x; e < kTile * kTile; e += kThreads) {
+    const int al = e % kTile;
+    const int bl = e / kTile;
+    const int a = a0 + al;
+    const int b = b0 + bl;
+    if (a >= bx || b >= by) continue;
+    double v;
+    if constexpr (SYM) v = (al / 16 <= bl / 16) ? red[al + kTile * bl] : red[bl + kTile * al];
+    else v = red[al + kTile * bl];
+    Pb[a + (int64_t)b * bx] = v;
+  }
+}
+
+// out = alpha X C + beta Z (X: n x bx, C: bx x by column-major) computed as
+// out^T = C^T X^T: A fragments come from C staged in LDS, B fragments from
+// the wave's 16 rows of X staged in LDS; K runs over bx in chunks of 64.
+// Workgroup = 4 waves x 16 rows; blockIdx.y selects 64 output columns.
+__global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
+    int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ C,
+    int ldc, int by, double alpha, double beta, const double* Z, double* out, int ldo,
+    int64_t nrb) {
+  __shared__ double cs[kTile][kTile + 1];
+  __shared__ double xs[4][16][kTile + 1];
+  const int64_t rb = xcd_remap(blockIdx.x, gridDim.x);
+  if (rb >= nrb) return;
+  const int c0 = blockIdx.y * kTile;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kr = lane >> 4;
+  const int cc = lane & 15;
+  const int64_t ibase = rb * kTile + wave * 16;
+  d4v acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = d4v{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < bx; k0 += kTile) {
+    if (k0 > 0) __syncthreads();
+    for (int e = threadIdx.x; e < kTile * kTile; e += kThreads) {
+      const int k = e % kTile;
+      const int c = e / kTile;
+      cs[k][c] = (k0 + k < bx && c0 + c < by) ? C[(k0 + k) + (int64_t)(c0 + c) * ldc] : 0.0;
+    }
+    for (int e = lane; e < 16 * kTile; e += 64) {
+      const int r = e / kTile;
+      const int k = e % kTile;
+      const int64_t i = ibase + r;
+      xs[wave][r][k] = (i < n && k0 + k < bx) ? X[i * ldx + k0 + k] : 0.0;
+    }
+    __syncthreads();
+    const int kn = min(kTile, bx - k0);
+    for (int ks = 0; ks < kn; ks += 4) {
+      const int k = ks + kr;
+      const double bf = xs[wave][cc][k];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = mfma16x16x4(cs[k][16 * t + cc], bf, acc[t]);
+    }
+  }
+  const int64_t i = ibase + cc;
+  if (i >= n) return;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = c0 + 16 * t + 4 * j + kr;
+      if (c >= by) continue;
+      const int64_t off = i * ldo + c;
+      double v = alpha * acc[t][j];
+      if (beta != 0.0) v = fma(beta, Z[off], v);
       out[off] = v;
     }
   }
@@ -531,23 +757,34 @@ inline unsigned grid_for(int64_t work) {
 // ============================================================== launchers
 template <typename T>
 void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val, const T* X,
-                      T* out, int ld, int ncols, T alpha, T beta, hipStream_t st) {
+                      T* out, int ld, int ncols, T alpha, T beta, hipStream_t st,
+                      const CsrSched* sc) {
   if (n <= 0 || ncols <= 0) return;
   if (beta == T(0))
-    csr_launch<T, kApply>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, T(0), T(0), st);
+    csr_launch<T, kApply>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, T(0), T(0), st, sc);
   else
-    csr_launch<T, kApplyBeta>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, beta, T(0), st);
+    csr_launch<T, kApplyBeta>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, beta, T(0), st, sc);
 }
 
 template <typename T>
 void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val, const T* X1,
                       const T* X0, T* out, int ld, int ncols, T g1, T g2, T s, bool first,
-                      hipStream_t st) {
+                      hipStream_t st, int panel, const CsrSched* sc) {
   if (n <= 0 || ncols <= 0) return;
-  if (first)
-    csr_launch<T, kChebFirst>(n, rowptr, col, val, X1, nullptr, out, ld, ncols, g1, g2, s, st);
-  else
-    csr_launch<T, kCheb>(n, rowptr, col, val, X1, X0, out, ld, ncols, g1, g2, s, st);
+  // Column panels: each launch streams the CSR arrays once and touches only
+  // `panel` columns of every row, which shrinks the per-plane footprint of
+  // the gathered block so that neighbour rows stay in L2 between uses.
+  if (panel <= 0 || panel >= ncols) panel = ncols;
+  if (panel % 4 != 0) panel = round_up(panel, 4);
+  for (int c0 = 0; c0 < ncols; c0 += panel) {
+    const int pc = std::min(panel, ncols - c0);
+    if (first)
+      csr_launch<T, kChebFirst>(n, rowptr, col, val, X1 + c0, nullptr, out + c0, ld, pc, g1, g2, s,
+                                st, sc);
+    else
+      csr_launch<T, kCheb>(n, rowptr, col, val, X1 + c0, X0 + c0, out + c0, ld, pc, g1, g2, s, st,
+                           sc);
+  }
 }
 
 template <typename T>
@@ -578,12 +815,22 @@ void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, in
   int64_t rpb;
   const int64_t nblk = gram_blocks<T>(n, bx, by, &rpb);
   if ((size_t)nblk * bx * by > ws_elems) throw Error(3, "gram workspace too small");
-  const int ntiles = ((bx + kTile - 1) / kTile) * ((by + kTile - 1) / kTile);
-  hipLaunchKernelGGL(gram_partial_kernel<T>, dim3((unsigned)nblk, ntiles), dim3(kThreads), 0, st, n,
-                     X, ldx, bx, Y, ldy, by, ws, rpb, ntiles);
+  const int nta = (bx + kTile - 1) / kTile;
+  const int ntiles = nta * ((by + kTile - 1) / kTile);
+  if constexpr (std::is_same<T, double>::value) {
+    if (X == Y && bx == by && ldx == ldy && bx <= kTile)
+      hipLaunchKernelGGL(gram_mfma_kernel<true>, dim3((unsigned)nblk, 1), dim3(kThreads), 0, st, n,
+                         X, ldx, bx, Y, ldy, by, ws, rpb, nta);
+    else
+      hipLaunchKernelGGL(gram_mfma_kernel<false>, dim3((unsigned)nblk, ntiles), dim3(kThreads), 0,
+                         st, n, X, ldx, bx, Y, ldy, by, ws, rpb, nta);
+  } else {
+    hipLaunchKernelGGL(gram_partial_kernel<T>, dim3((unsigned)nblk, ntiles), dim3(kThreads), 0, st,
+                       n, X, ldx, bx, Y, ldy, by, ws, rpb, ntiles);
+  }
   RCG_HIP(hipGetLastError());
   const int64_t m = (int64_t)bx * by;
-  hipLaunchKernelGGL((reduce_partials_kernel<T, T>), dim3(grid_for(m)), dim3(kThreads), 0, st, ws,
+  hipLaunchKernelGGL((reduce_partials_kernel<T, T>), dim3(reduce_grid(m)), dim3(kThreads), 0, st, ws,
                      nblk, m, G);
   RCG_HIP(hipGetLastError());
 }
@@ -595,8 +842,13 @@ void launch_tall_gemm(int64_t n, const T* X, int ldx, int bx, const T* C, int ld
   const int64_t nrb = (n + kTile - 1) / kTile;
   const int64_t grid = round_up(nrb, kXcds);
   const unsigned ct = (unsigned)((by + kTile - 1) / kTile);
-  hipLaunchKernelGGL(tall_gemm_kernel<T>, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
-                     ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);
+  if constexpr (std::is_same<T, double>::value) {
+    hipLaunchKernelGGL(tall_gemm_mfma_kernel, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
+                       ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);
+  } else {
+    hipLaunchKernelGGL(tall_gemm_kernel<T>, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
+                       ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);
+  }
   RCG_HIP(hipGetLastError());
 }
 
@@ -613,7 +865,7 @@ void launch_col_sumsq(int64_t n, const T* X, int ld, int nc, double* sums, doubl
   hipLaunchKernelGGL((col_sumsq_kernel<T, false>), dim3((unsigned)nblk), dim3(kThreads), 0, st, n,
                      X, (const T*)nullptr, ld, nc, (const double*)nullptr, ws);
   RCG_HIP(hipGetLastError());
-  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(grid_for(nc)), dim3(kThreads),
+  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(reduce_grid(nc)), dim3(kThreads),
                      0, st, ws, nblk, (int64_t)nc, sums);
   RCG_HIP(hipGetLastError());
 }
@@ -627,7 +879,7 @@ void launch_residual_sumsq(int64_t n, const T* V, const T* W, int ld, int nc, co
   hipLaunchKernelGGL((col_sumsq_kernel<T, true>), dim3((unsigned)nblk), dim3(kThreads), 0, st, n, V,
                      W, ld, nc, theta, ws);
   RCG_HIP(hipGetLastError());
-  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(grid_for(nc)), dim3(kThreads),
+  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(reduce_grid(nc)), dim3(kThreads),
                      0, st, ws, nblk, (int64_t)nc, sums);
   RCG_HIP(hipGetLastError());
 }
@@ -710,9 +962,10 @@ void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* al
 // ------------------------------------------------------- instantiations
 #define RCG_INST(T)                                                                              \
   template void launch_csr_apply<T>(int64_t, const int*, const int*, const T*, const T*, T*, int, \
-                                    int, T, T, hipStream_t);                                      \
+                                    int, T, T, hipStream_t, const CsrSched*);                     \
   template void launch_cheb_step<T>(int64_t, const int*, const int*, const T*, const T*,          \
-                                    const T*, T*, int, int, T, T, T, bool, hipStream_t);          \
+                                    const T*, T*, int, int, T, T, T, bool, hipStream_t, int,      \
+                                    const CsrSched*);                                             \
   template size_t gram_ws_elems<T>(int64_t, int, int);                                           \
   template void launch_gram<T>(int64_t, const T*, int, int, const T*, int, int, T*, T*, size_t,   \
                                hipStream_t);                                                      \

@@ -15,7 +15,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -96,6 +98,13 @@ struct rcg_block_s {
   void* d() const { return mem.p; }
 };
 
+// A candidate visiting order of the CSR kernels' row blocks.
+struct SchedCand {
+  std::string name;
+  DevBuf order;
+  rcg::CsrSched sc;
+};
+
 struct rcg_operator_s {
   rcg_context_s* ctx = nullptr;
   int64_t n = 0, nnz = 0;
@@ -104,6 +113,13 @@ struct rcg_operator_s {
   const int* col = nullptr;
   const double* val = nullptr;
   DevBuf own_rowptr, own_col, own_val;
+  int64_t gx = 0, gy = 0, gz = 0;  // structured-grid hint (rows x + gx(y + gy z))
+  std::vector<std::unique_ptr<SchedCand>> scheds;
+  struct Tuned {
+    int panel;
+    int sched;
+  };
+  std::map<int64_t, Tuned> tuned;  // autotuned filter shape per (ld, columns)
 };
 
 namespace {
@@ -155,7 +171,7 @@ double wall() {
 // W (A V, ld) and Y (found vectors, ldy).
 class Engine {
  public:
-  Engine(rcg_context_s* ctx, const rcg_operator_s* op, int64_t n) : ctx_(ctx), op_(op), n_(n) {
+  Engine(rcg_context_s* ctx, rcg_operator_s* op, int64_t n) : ctx_(ctx), op_(op), n_(n) {
     st_ = ctx->stream;
   }
 
@@ -208,25 +224,134 @@ class Engine {
   double* hlz() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + 3 * bb_; }
   double* ws() { return ctx_->ws.as<double>(); }
 
+  // Candidate row-block orders: contiguous blocks with and without the XCD
+  // remap and, for operators with a structured-grid hint, x-lines visited in
+  // (y, z) tiles so that the lines a stencil row gathers from are in flight
+  // together.
+  void build_schedules() {
+    if (!op_->scheds.empty()) return;
+    auto add = [&](const std::string& name, std::vector<int> order, int64_t rows, int remap) {
+      auto c = std::make_unique<SchedCand>();
+      c->name = name;
+      c->sc.remap = remap;
+      if (!order.empty()) {
+        c->order.ensure(sizeof(int) * order.size());
+        RCG_HIP(hipMemcpy(c->order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+        c->sc.order = c->order.as<int>();
+        c->sc.rows_per_blk = rows;
+        c->sc.nblk = (int64_t)order.size();
+      }
+      op_->scheds.push_back(std::move(c));
+    };
+    add("rows", {}, 0, 1);
+    add("rows-noremap", {}, 0, 0);
+    const int64_t gx = op_->gx, gy = op_->gy, gz = op_->gz;
+    if (gx >= 64 && gy >= 4 && gz >= 4 && gx * gy * gz == n_) {
+      const int tiles[][2] = {{8, 8}, {16, 16}, {32, 8}, {16, 4}, {4, 16}};
+      for (const auto& t : tiles) {
+        const int ty = t[0], tz = t[1];
+        std::vector<int> order;
+        order.reserve(gy * gz);
+        for (int64_t z0 = 0; z0 < gz; z0 += tz)
+          for (int64_t y0 = 0; y0 < gy; y0 += ty)
+            for (int64_t z = z0; z < std::min(gz, z0 + tz); ++z)
+              for (int64_t y = y0; y < std::min(gy, y0 + ty); ++y) order.push_back((int)(y + gy * z));
+        for (int remap = 0; remap < 2; ++remap)
+          add("tile" + std::to_string(ty) + "x" + std::to_string(tz) + (remap ? "" : "-noremap"),
+              order, gx, remap);
+      }
+    }
+  }
+
+  // Filter shape for b columns: the fastest row-block order, then the fastest
+  // column-panel width {b, 32, 16} under it, timed with hipEvents on this
+  // operator (3 Chebyshev steps after a warm-up step) and cached on it.
+  // RCG_FILTER_PANEL / RCG_FILTER_SCHED override the choice.
+  rcg_operator_s::Tuned tuned_for(int64_t b) {
+    build_schedules();
+    const int64_t key = ld_ * 100000 + b;
+    auto it = op_->tuned.find(key);
+    rcg_operator_s::Tuned t{(int)b, 0};
+    if (it != op_->tuned.end()) {
+      t = it->second;
+    } else if (n_ * b >= (int64_t)1 << 22) {  // only large blocks are worth timing
+      hipEvent_t e0, e1;
+      RCG_HIP(hipEventCreate(&e0));
+      RCG_HIP(hipEventCreate(&e1));
+      auto time_it = [&](int panel, int sched) {
+        const rcg::CsrSched* sc = &op_->scheds[sched]->sc;
+        rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, V(), T(), T(), (int)ld_,
+                                      (int)b, 0.0, 1.0, 0.0, false, st_, panel, sc);
+        RCG_HIP(hipEventRecord(e0, st_));
+        for (int r = 0; r < 3; ++r)
+          rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, V(), T(), T(),
+                                        (int)ld_, (int)b, 0.0, 1.0, 0.0, false, st_, panel, sc);
+        RCG_HIP(hipEventRecord(e1, st_));
+        RCG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        RCG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+      };
+      float best = 1e30f;
+      for (int sidx = 0; sidx < (int)op_->scheds.size(); ++sidx) {
+        const float ms = time_it((int)b, sidx);
+        if (ms < best) {
+          best = ms;
+          t = {(int)b, sidx};
+        }
+      }
+      for (int cand : {32, 16}) {
+        if (cand >= b) continue;
+        const float ms = time_it(cand, t.sched);
+        if (ms < best) {
+          best = ms;
+          t.panel = cand;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      op_->tuned[key] = t;
+    }
+    if (const char* env = std::getenv("RCG_FILTER_PANEL")) {
+      const int v = std::atoi(env);
+      if (v > 0) t.panel = v;
+    }
+    if (const char* env = std::getenv("RCG_FILTER_SCHED")) {
+      for (int sidx = 0; sidx < (int)op_->scheds.size(); ++sidx)
+        if (op_->scheds[sidx]->name == env) t.sched = sidx;
+    }
+    return t;
+  }
+  const rcg::CsrSched* sched_for(int64_t b) {
+    if (!op_ || n_ * b < (int64_t)1 << 22) return nullptr;
+    return &op_->scheds[tuned_for(b).sched]->sc;
+  }
+
   // V <- P_m(A')^rep V over b columns (LanczosCpolyOperatorLM.h:192-304):
   // X1 = g2(g1 A X0 + X0), Xj = g2(g1 A X_{j-1} + X_{j-1}) - X_{j-2}, out = Xm/(m+1).
   void filter(int64_t b, long m, long rep, double lmax, double shift) {
     const auto c = rcg::cpoly::coefficients(m, lmax, shift);
+    const auto tn = tuned_for(b);
+    const rcg::CsrSched* sc = &op_->scheds[tn.sched]->sc;
+    last_panel_ = tn.panel;
+    last_sched_ = op_->scheds[tn.sched]->name;
     for (long r = 0; r < rep; ++r) {
       double* X0 = V();
       double* X1 = T();
       const double s1 = (m == 1) ? 0.5 : 1.0;
       rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, X0, nullptr, X1,
-                                    (int)ld_, (int)b, c.g1, c.g2, s1, true, st_);
+                                    (int)ld_, (int)b, c.g1, c.g2, s1, true, st_, tn.panel, sc);
       for (long j = 2; j <= m; ++j) {
         const double s = (j == m) ? 1.0 / double(m + 1) : 1.0;
         rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, X1, X0, X0,
-                                      (int)ld_, (int)b, c.g1, c.g2, s, false, st_);
+                                      (int)ld_, (int)b, c.g1, c.g2, s, false, st_, tn.panel, sc);
         std::swap(X0, X1);
       }
       if (X1 != V()) swapVT();
     }
   }
+  int last_panel() const { return last_panel_; }
+  const std::string& last_sched() const { return last_sched_; }
 
   // host[bx x by] = X^H Y (column-major).
   void gram_to_host(const double* X, int64_t ldx, int64_t bx, const double* Yp, int64_t ldy,
@@ -309,7 +434,7 @@ class Engine {
   // for j >= i, mirrored.
   void project(int64_t b, std::vector<double>& H) {
     rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, V(), W_, (int)ld_, (int)b,
-                                  1.0, 0.0, st_);
+                                  1.0, 0.0, st_, sched_for(b));
     H.assign(b * b, 0.0);
     gram_to_host(V(), ld_, b, W_, ld_, b, H.data());
     for (int64_t i = 0; i < b; ++i)
@@ -329,7 +454,7 @@ class Engine {
     res.assign(count, 0.0);
     if (count <= 0) return;
     rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, V(), W_, (int)ld_,
-                                  (int)count, 1.0, 0.0, st_);
+                                  (int)count, 1.0, 0.0, st_, sched_for(count));
     std::memcpy(hvecUp(), th, sizeof(double) * count);
     RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * count, hipMemcpyHostToDevice, st_));
     rcg::launch_residual_sumsq<double>(n_, V(), W_, (int)ld_, (int)count, dtheta(), dsums(), ws(),
@@ -442,7 +567,7 @@ class Engine {
 
  private:
   rcg_context_s* ctx_;
-  const rcg_operator_s* op_;
+  rcg_operator_s* op_;
   int64_t n_;
   hipStream_t st_;
   double* buf_[2] = {nullptr, nullptr};
@@ -451,6 +576,8 @@ class Engine {
   double* Y_ = nullptr;
   int64_t ld_ = 0, ldy_ = 0, bb_ = 4;
   size_t ws_elems_ = 0;
+  int last_panel_ = 0;
+  std::string last_sched_;
 };
 
 // ------------------------------------------------------------- the solver
@@ -812,6 +939,7 @@ class Solver {
     R->filter_degree_sum = degree_sum_;
     R->t_filter_device = t_filter_dev_;
     R->filter_bytes = filter_bytes_;
+    R->filter_panel = eng_.last_panel();
     return foundSize;
   }
 
@@ -878,7 +1006,7 @@ void check_pair(rcg_operator op, rcg_block a, rcg_block b, int64_t cols) {
 // Engine over caller blocks for the piecewise entry points.
 struct Piece {
   Engine eng;
-  Piece(rcg_block V, rcg_block T, const rcg_operator_s* op, int64_t kmax = 4)
+  Piece(rcg_block V, rcg_block T, rcg_operator_s* op, int64_t kmax = 4)
       : eng(V->ctx, op, V->n) {
     eng.reserve(V->ld, std::max<int64_t>(kmax, 4));
     eng.attach(static_cast<double*>(V->d()), T ? static_cast<double*>(T->d()) : nullptr, nullptr,
@@ -1094,6 +1222,20 @@ int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32
       o->val = o->own_val.as<double>();
     }
     *out = o.release();
+  });
+}
+
+int rcg_operator_set_grid(rcg_operator op, int64_t nx, int64_t ny, int64_t nz) {
+  return guarded([&] {
+    need(op != nullptr, "operator is NULL");
+    need(nx >= 1 && ny >= 1 && nz >= 1 && nx * ny * nz == op->n, "grid does not match the operator size");
+    DeviceGuard g(op->ctx->device);
+    RCG_HIP(hipStreamSynchronize(op->ctx->stream));
+    op->gx = nx;
+    op->gy = ny;
+    op->gz = nz;
+    op->scheds.clear();
+    op->tuned.clear();
   });
 }
 

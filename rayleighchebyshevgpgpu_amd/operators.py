@@ -137,7 +137,9 @@ def _laplacian(shape, bc, potential):
     rows = np.concatenate([np.arange(n), r])
     cols = np.concatenate([np.arange(n), c])
     vals = np.concatenate([diag, -np.ones(r.shape[0])])
-    return _from_coo(n, rows, cols, vals)
+    A = _from_coo(n, rows, cols, vals)
+    A.grid = tuple(shape)  # locality hint for the device kernels (row order x fastest)
+    return A
 
 
 def laplacian_2d(nx, ny=None, bc="dirichlet", potential=None) -> Csr:

@@ -111,7 +111,7 @@ class CsrOperator:
     """Device CSR operator (the reference's user Otype with
     apply(in, out, alpha, beta), LanczosCpolyOperatorLMCuda.h:236,245)."""
 
-    def __init__(self, csr, ctx: Context | None = None):
+    def __init__(self, csr, ctx: Context | None = None, grid=None):
         self.ctx = ctx or default_context()
         if csr.is_complex:
             raise NotImplementedError("complex CSR operators are not available in this build")
@@ -124,6 +124,10 @@ class CsrOperator:
             capi.MEM_HOST, C.byref(h)))
         self.h = h
         self._keep = None
+        grid = grid if grid is not None else getattr(csr, "grid", None)
+        if grid is not None:
+            nx, ny, nz = (tuple(grid) + (1, 1))[:3]
+            check(lib().rcg_operator_set_grid(self.h, nx, ny, nz))
 
     def apply(self, X: Block, Y: Block, alpha: float = 1.0, beta: float = 0.0, cols=None):
         check(lib().rcg_operator_apply(self.h, X.h, Y.h, alpha, beta, X.cols if cols is None else cols))
