@@ -109,6 +109,10 @@ def kernel_only(args):
     m = args.kernel_only + (args.kernel_only % 2)
     capi.check(capi.lib().rcg_cheb_filter(op.h, X.h, T.h, K, m, 1, 14.2, 2.0))
     capi.check(capi.lib().rcg_block_copy(T.h, X.h))
+    # column norms + normalisation: col_sumsq reads the block once (n*ld*8
+    # bytes), normalize_cols reads and writes it once -- known-size kernels of
+    # the same access type as the filter step's own-row streams
+    capi.check(capi.lib().rcg_deflate(X.h, K, T.h, 0))
     ctx.synchronize()
     csr = A.nnz * 12 + 4 * (A.n + 1)
     blk = A.n * X.ld * 8

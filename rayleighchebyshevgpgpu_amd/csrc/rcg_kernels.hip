@@ -110,9 +110,16 @@ struct V16<float> {
 // ------------------------------------------------------------ CSR x block
 enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3 };
 
+// Rows per workgroup block (cap) and nonzeros staged in LDS per block.
+constexpr int kMaxRpb = 512;
+constexpr int kStageNz = 1024;
+
 // G lanes own one row; each lane handles NV 16-byte column vectors at
-// cv = g + k*G.  256/G rows are in flight per workgroup.
-template <typename T, int G, int NV, int MODE>
+// cv = g + k*G.  256/G rows are in flight per workgroup.  The block's row
+// pointers (and, when they fit, its column indices and values) are staged in
+// LDS by one coalesced load, and a row's own-row operands are loaded before
+// its gathers, so that computing a row costs a single trip to memory.
+template <typename T, int G, int NV, int MODE, int RPI>
 __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
@@ -121,88 +128,146 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
   using VT = V16<T>;
   constexpr int W = VT::W;
   constexpr int RS = kThreads / G;
+  __shared__ int s_rp[kMaxRpb + 1];
+  __shared__ int s_col[kStageNz];
+  __shared__ T s_val[kStageNz];
   const int64_t lq = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-  if (lq >= nblk) return;
+  if (lq >= nblk) return;  // uniform across the workgroup
   const int64_t lb = order ? (int64_t)order[lq] : lq;
   const int g = threadIdx.x % G;
   const int r = threadIdx.x / G;
   const int64_t i0 = lb * rows_per_blk;
   const int64_t i1 = min(n, i0 + rows_per_blk);
-  for (int64_t i = i0 + r; i < i1; i += RS) {
-    T acc[NV][W];
+  const int nr = (int)(i1 - i0);
+  for (int t = threadIdx.x; t <= nr; t += kThreads) s_rp[t] = rowptr[i0 + t];
+  __syncthreads();
+  const int base = s_rp[0];
+  const int nz = s_rp[nr] - base;
+  const bool staged = nz <= kStageNz;
+  if (staged)
+    for (int t = threadIdx.x; t < nz; t += kThreads) {
+      s_col[t] = col[base + t];
+      s_val[t] = val[base + t];
+    }
+  __syncthreads();
+  // RPI rows per lane iteration: rows li and li + RS, ... so that each wave
+  // has RPI rows' gathers in flight at once.
+  for (int li0 = r; li0 < nr; li0 += RS * RPI) {
+    int pb[RPI], pe[RPI];
+    bool live_row[RPI];
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
+    for (int q = 0; q < RPI; ++q) {
+      const int li = li0 + q * RS;
+      live_row[q] = li < nr;
+      pb[q] = live_row[q] ? s_rp[li] : 0;
+      pe[q] = live_row[q] ? s_rp[li + 1] : 0;
+    }
+    // own-row operands first: they do not depend on the gathers
+    T xs[RPI][NV][W], zs[RPI][NV][W];
 #pragma unroll
-      for (int w = 0; w < W; ++w) acc[k][w] = T(0);
-    const int pb = rowptr[i];
-    const int pe = rowptr[i + 1];
-    // Chunks of 4 nonzeros: the gathers of a chunk are issued before its FMAs.
-    for (int p = pb; p < pe; p += 4) {
-      int jj[4];
-      T aa[4];
+    for (int q = 0; q < RPI; ++q) {
+      if (!live_row[q]) continue;
+      const int64_t rowoff = (i0 + li0 + q * RS) * (int64_t)ld;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const bool live = p + q < pe;
-        if constexpr (kNtStreams) {
-          jj[q] = live ? __builtin_nontemporal_load(col + p + q) : 0;
-          aa[q] = live ? __builtin_nontemporal_load(val + p + q) : T(0);
-        } else {
-          jj[q] = live ? col[p + q] : 0;
-          aa[q] = live ? val[p + q] : T(0);
-        }
-      }
-      T u[4][NV][W];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (p + q < pe) {
-          const T* xr = X + (int64_t)jj[q] * ld;
-#pragma unroll
-          for (int k = 0; k < NV; ++k) {
-            const int cv = g + k * G;
-            if (cv < ncv) VT::ld(xr + cv * W, u[q][k]);
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (p + q < pe) {
-#pragma unroll
-          for (int k = 0; k < NV; ++k)
-#pragma unroll
-            for (int w = 0; w < W; ++w) acc[k][w] = fma(aa[q], u[q][k][w], acc[k][w]);
+      for (int k = 0; k < NV; ++k) {
+        const int cv = g + k * G;
+        if (cv < ncv) {
+          if constexpr (MODE == kChebFirst || MODE == kCheb) VT::ld(X + rowoff + cv * W, xs[q][k]);
+          if constexpr (MODE == kCheb) VT::ld_stream(Z + rowoff + cv * W, zs[q][k]);
+          if constexpr (MODE == kApplyBeta) VT::ld(out + rowoff + cv * W, zs[q][k]);
         }
       }
     }
+    T acc[RPI][NV][W];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      const int cv = g + k * G;
-      if (cv >= ncv) continue;
-      const int64_t off = i * (int64_t)ld + cv * W;
-      T o[W];
-      if constexpr (MODE == kApply) {
+    for (int q = 0; q < RPI; ++q)
 #pragma unroll
-        for (int w = 0; w < W; ++w) o[w] = p0 * acc[k][w];
-      } else if constexpr (MODE == kApplyBeta) {
-        T y[W];
-        VT::ld(out + off, y);
+      for (int k = 0; k < NV; ++k)
 #pragma unroll
-        for (int w = 0; w < W; ++w) o[w] = fma(p0, acc[k][w], p1 * y[w]);
-      } else if constexpr (MODE == kChebFirst) {
-        T xs[W];
-        VT::ld(X + off, xs);
+        for (int w = 0; w < W; ++w) acc[q][k][w] = T(0);
+    int maxlen = 0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) o[w] = p2 * (p1 * fma(p0, acc[k][w], xs[w]));
-      } else {
-        T xs[W], zs[W];
-        VT::ld(X + off, xs);
-        VT::ld_stream(Z + off, zs);
+    for (int q = 0; q < RPI; ++q) maxlen = max(maxlen, pe[q] - pb[q]);
+    // chunks of 4 nonzeros per row: every gather of a chunk (over the RPI
+    // rows) is issued before its FMAs
+    for (int c = 0; c < maxlen; c += 4) {
+      int jj[RPI][4];
+      T aa[RPI][4];
 #pragma unroll
-        for (int w = 0; w < W; ++w) o[w] = p2 * fma(p1, fma(p0, acc[k][w], xs[w]), -zs[w]);
+      for (int q = 0; q < RPI; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int p = pb[q] + c + e;
+          const bool live = p < pe[q];
+          if (staged) {
+            jj[q][e] = live ? s_col[p - base] : 0;
+            aa[q][e] = live ? s_val[p - base] : T(0);
+          } else {
+            jj[q][e] = live ? col[p] : 0;
+            aa[q][e] = live ? val[p] : T(0);
+          }
+        }
+      T u[RPI][4][NV][W];
+#pragma unroll
+      for (int q = 0; q < RPI; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (pb[q] + c + e < pe[q]) {
+            const T* xr = X + (int64_t)jj[q][e] * ld;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+              const int cv = g + k * G;
+              if (cv < ncv) VT::ld(xr + cv * W, u[q][e][k]);
+            }
+          }
+        }
+#pragma unroll
+      for (int q = 0; q < RPI; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (pb[q] + c + e < pe[q]) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+#pragma unroll
+              for (int w = 0; w < W; ++w) acc[q][k][w] = fma(aa[q][e], u[q][e][k][w], acc[q][k][w]);
+          }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPI; ++q) {
+      if (!live_row[q]) continue;
+      const int64_t rowoff = (i0 + li0 + q * RS) * (int64_t)ld;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const int cv = g + k * G;
+        if (cv >= ncv) continue;
+        T o[W];
+        if constexpr (MODE == kApply) {
+#pragma unroll
+          for (int w = 0; w < W; ++w) o[w] = p0 * acc[q][k][w];
+        } else if constexpr (MODE == kApplyBeta) {
+#pragma unroll
+          for (int w = 0; w < W; ++w) o[w] = fma(p0, acc[q][k][w], p1 * zs[q][k][w]);
+        } else if constexpr (MODE == kChebFirst) {
+#pragma unroll
+          for (int w = 0; w < W; ++w) o[w] = p2 * (p1 * fma(p0, acc[q][k][w], xs[q][k][w]));
+        } else {
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            o[w] = p2 * fma(p1, fma(p0, acc[q][k][w], xs[q][k][w]), -zs[q][k][w]);
+        }
+        VT::st_stream(out + rowoff + cv * W, o);
       }
-      VT::st_stream(out + off, o);
     }
   }
 }
+
+#ifndef RCG_CSR_RPI
+#define RCG_CSR_RPI 1
+#endif
+#ifndef RCG_CSR_GDIV
+#define RCG_CSR_GDIV 1
+#endif
 
 template <typename T, int MODE, int G, int NV>
 void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
@@ -215,17 +280,19 @@ void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T*
     nblk = sc->nblk;
     order = sc->order;
   } else {
-    // ~16 row-passes per workgroup keeps the grid >> 256 CUs at large n.
-    rows_per_blk = (int64_t)RS * 16;
+    // ~128-row blocks (their nonzeros fit the LDS stage for stencils) keep
+    // the grid >> 256 CUs at large n; small problems use one row pass.
+    rows_per_blk = std::min<int64_t>(std::max<int64_t>(RS, 128), kMaxRpb);
     nblk = (n + rows_per_blk - 1) / rows_per_blk;
     if (nblk < 2048) {
       rows_per_blk = RS;
       nblk = (n + rows_per_blk - 1) / rows_per_blk;
     }
   }
+  if (rows_per_blk > kMaxRpb) throw Error(1, "CSR row block exceeds the staged row-pointer range");
   const int remap = sc ? sc->remap : 1;
   const int64_t grid = round_up(nblk, kXcds);
-  hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE>), dim3((unsigned)grid), dim3(kThreads), 0,
+  hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE, RCG_CSR_RPI>), dim3((unsigned)grid), dim3(kThreads), 0,
                      st, n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, rows_per_blk, nblk, order,
                      remap);
   RCG_HIP(hipGetLastError());
@@ -243,9 +310,9 @@ void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X
   RCG_CSR_CASE(2, 2, 1)
   RCG_CSR_CASE(4, 4, 1)
   RCG_CSR_CASE(8, 8, 1)
-  RCG_CSR_CASE(16, 16, 1)
-  RCG_CSR_CASE(32, 32, 1)
-  RCG_CSR_CASE(64, 64, 1)
+  RCG_CSR_CASE(16, 16 / RCG_CSR_GDIV, RCG_CSR_GDIV)
+  RCG_CSR_CASE(32, 32 / RCG_CSR_GDIV, RCG_CSR_GDIV)
+  RCG_CSR_CASE(64, 64 / RCG_CSR_GDIV, RCG_CSR_GDIV)
   RCG_CSR_CASE(128, 64, 2)
   RCG_CSR_CASE(256, 64, 4)
 #undef RCG_CSR_CASE

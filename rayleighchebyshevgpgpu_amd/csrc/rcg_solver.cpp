@@ -246,7 +246,7 @@ class Engine {
     add("rows", {}, 0, 1);
     add("rows-noremap", {}, 0, 0);
     const int64_t gx = op_->gx, gy = op_->gy, gz = op_->gz;
-    if (gx >= 64 && gy >= 4 && gz >= 4 && gx * gy * gz == n_) {
+    if (gx >= 64 && gx <= 512 && gy >= 4 && gz >= 4 && gx * gy * gz == n_) {
       const int tiles[][2] = {{8, 8}, {16, 16}, {32, 8}, {16, 4}, {4, 16}};
       for (const auto& t : tiles) {
         const int ty = t[0], tz = t[1];
