@@ -40,7 +40,7 @@ def build(verbose: bool = True, out: str = LIB, defines=()) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = out + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-soname,librcg.so", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
