@@ -9,7 +9,12 @@ Workload (one "step" = one whole solve, operator resident in HBM):
   bounds, all filter passes and returns eigenvalues + eigenvectors on device.
 
 value = converged eigenpairs (||Av - lambda v|| / ||A|| <= 1e-10, checked on
-the device after the timed region) per second of timed wall time.
+the device after the timed region) per second of timed wall time, FP64
+filter.  The same solve with the mixed-precision filter (FP32 Chebyshev
+steps on each Ritz column's correction, FP64 for everything else and a final
+FP64 Rayleigh-Ritz refinement over the found pairs) is timed the same way
+and reported under "mixed", with its speed-up and its largest eigenvalue
+difference from the FP64 solve.
 
 roofline: the fused Chebyshev step kernel (csr_block_kernel, mode kCheb),
 timed with HIP events on the library's stream around every filter
@@ -55,9 +60,10 @@ def build_operator(grid=GRID):
     return ops.laplacian_3d(grid, bc="periodic", potential=pot)
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(precision="fp64"):
     """HBM bytes per Chebyshev-step launch from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(ROOT, "profiles", "cheb_step_128cube_f64_traffic.json")
+    name = "cheb_step_128cube_f32src" if precision == "mixed" else "cheb_step_128cube_f64"
+    path = os.path.join(ROOT, "profiles", name + "_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -107,7 +113,14 @@ def kernel_only(args):
     T = solver.Block(A.n, K, ctx=ctx)
     X.randomize(0, K, 0, 0)
     m = args.kernel_only + (args.kernel_only % 2)
-    capi.check(capi.lib().rcg_cheb_filter(op.h, X.h, T.h, K, m, 1, 14.2, 2.0))
+    if args.kernel_mixed:
+        # all columns as Ritz columns: the FP32 correction step with its source block
+        theta = np.full(K, -0.3)
+        ritz = np.ones(K, dtype=np.int32)
+        capi.check(capi.lib().rcg_cheb_filter_mixed(op.h, X.h, T.h, K, m, 1, 14.2, 2.0,
+                                                    theta.ctypes.data, ritz.ctypes.data))
+    else:
+        capi.check(capi.lib().rcg_cheb_filter(op.h, X.h, T.h, K, m, 1, 14.2, 2.0))
     capi.check(capi.lib().rcg_block_copy(T.h, X.h))
     # column norms + normalisation: col_sumsq reads the block once (n*ld*8
     # bytes), normalize_cols reads and writes it once -- known-size kernels of
@@ -116,8 +129,13 @@ def kernel_only(args):
     ctx.synchronize()
     csr = A.nnz * 12 + 4 * (A.n + 1)
     blk = A.n * X.ld * 8
-    print(json.dumps({"mode": "kernel-only", "cheb_steps": m, "n": A.n, "nnz": A.nnz, "b": K,
-                      "alg_bytes_per_step": csr + 3 * A.n * K * 8,
+    if args.kernel_mixed:
+        step = A.nnz * 8 + 4 * (A.n + 1) + 4 * A.n * K * 4
+    else:
+        step = csr + 3 * A.n * K * 8
+    print(json.dumps({"mode": "kernel-only", "precision": "mixed" if args.kernel_mixed else "fp64",
+                      "cheb_steps": m, "n": A.n, "nnz": A.nnz, "b": K,
+                      "alg_bytes_per_step": step,
                       "alg_bytes_first_step": csr + 2 * A.n * K * 8,
                       "copy_bytes_read": blk, "copy_bytes_written": blk}), flush=True)
 
@@ -130,9 +148,12 @@ def main():
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solves")
     ap.add_argument("--kernel-only", type=int, default=0, metavar="M",
                     help="profiling mode: run M Chebyshev steps of the filter on the 64-column "
                          "block and one 2 GiB device copy (the HBM calibration kernel), no solve")
+    ap.add_argument("--kernel-mixed", action="store_true",
+                    help="with --kernel-only: the FP32 correction steps of the mixed filter")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve runs "
@@ -151,61 +172,89 @@ def main():
     op = solver.CsrOperator(A, ctx)
     V = solver.Block(A.n, K, ctx=ctx)
 
-    def one_solve():
-        rc = solver.RayleighChebyshevLM()
-        rc.setStopCondition("RESIDUAL_ONLY")
-        ev = []
-        t = time.perf_counter()
-        f = rc.getMinEigenSystem(K, TOL, INC, BUF, None, op, solver.RandomizeOp(0), ev, V)
+    def run(precision, steps, warmup):
+        """warmup + timed solves at one filter precision; returns the measurements."""
+        def one_solve():
+            rc = solver.RayleighChebyshevLM()
+            rc.setStopCondition("RESIDUAL_ONLY")
+            rc.filterPrecision = np.float32 if precision == "mixed" else np.float64
+            ev = []
+            t = time.perf_counter()
+            f = rc.getMinEigenSystem(K, TOL, INC, BUF, None, op, solver.RandomizeOp(0), ev, V)
+            ctx.synchronize()
+            return time.perf_counter() - t, f, np.array(ev), rc
+
+        for w in range(warmup):
+            dt, f, _, _ = one_solve()
+            log(f"{precision} warmup {w}: {dt:.3f}s found={f}")
         ctx.synchronize()
-        return time.perf_counter() - t, f, np.array(ev), rc
+        lat, results = [], []
+        t_start = time.perf_counter()
+        for s in range(steps):
+            dt, f, ev, rc = one_solve()
+            lat.append(dt)
+            results.append((f, ev, rc.getResult()))
+            log(f"{precision} step {s}: {dt:.3f}s found={f} iters={rc.getResult().count_eig} "
+                f"degrees={rc.getResult().filter_degree_sum}")
+        ctx.synchronize()
+        total = time.perf_counter() - t_start
 
-    for w in range(args.warmup):
-        dt, f, _, _ = one_solve()
-        log(f"warmup {w}: {dt:.3f}s found={f}")
+        # residual check of the last solve's pairs on the device (outside the timed region)
+        f, ev, res = results[-1]
+        r = np.zeros(f)
+        capi.check(capi.lib().rcg_operator_apply(op.h, V.h, Wb.h, 1.0, 0.0, f))
+        capi.check(capi.lib().rcg_residuals(op.h, V.h, Wb.h, f, ev.ctypes.data_as(C.c_void_p),
+                                            r.ctypes.data_as(C.c_void_p)))
+        normA = max(abs(res.min_est), abs(res.max_est))
+        rel = r / normA
+        converged_per_solve = int(np.sum(rel <= 1e-10))
+        converged_total = sum(int(x[0]) for x in results) * converged_per_solve / max(f, 1)
 
-    ctx.synchronize()
-    lat, results = [], []
-    t_start = time.perf_counter()
-    for s in range(args.steps):
-        dt, f, ev, rc = one_solve()
-        lat.append(dt)
-        results.append((f, ev, rc.getResult()))
-        log(f"step {s}: {dt:.3f}s found={f} iters={rc.getResult().count_eig} "
-            f"degrees={rc.getResult().filter_degree_sum}")
-    ctx.synchronize()
-    total = time.perf_counter() - t_start
+        # dominant kernel: the Chebyshev step, HIP-event timed inside the solves
+        launches = sum(x[2].filter_launches for x in results)
+        t_dev = sum(x[2].t_filter_device for x in results)
+        alg_bytes = sum(x[2].filter_bytes for x in results)
+        achieved = alg_bytes / t_dev / 1e9 if t_dev > 0 else 0.0
+        kern = ("csr_block_kernel<float,16,1,kChebSrc> (FP32 correction step)" if precision == "mixed"
+                else "csr_block_kernel<double,32,1,kCheb> (fused FP64 Chebyshev step)")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profiles(precision), "kernel": kern,
+                    "avg_launch_us": round(t_dev / max(launches, 1) * 1e6, 2),
+                    "alg_bytes_per_launch": int(alg_bytes / max(launches, 1)),
+                    "launches": int(launches)}
+        last = results[-1][2]
+        lat_ms = np.array(lat) * 1e3
+        return {
+            "value": converged_total / total, "total": total, "lat_ms": lat_ms, "f": f, "ev": ev,
+            "last": last, "roofline": roofline,
+            "solve": {"found": int(f), "converged": converged_per_solve,
+                      "max_rel_residual": float(rel.max()) if f else None,
+                      "inner_iterations": int(last.count_eig),
+                      "filter_degrees": int(last.filter_degree_sum),
+                      "lanczos_iters": int(last.lanczos_iters),
+                      "t_lanczos_s": round(last.t_lanczos, 4),
+                      "t_filter_steps_device_s": round(last.t_filter_device, 4),
+                      "t_ortho_s": round(last.t_ortho, 4), "t_rr_s": round(last.t_rr, 4),
+                      "lambda_min": float(ev[0]) if f else None,
+                      "lambda_k": float(ev[-1]) if f else None},
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
+        }
 
-    # residual check of the last solve's pairs on the device (outside the timed region)
-    f, ev, res = results[-1]
-    W = solver.Block(A.n, K, ctx=ctx)
     import ctypes as C
-    r = np.zeros(f)
-    capi.check(capi.lib().rcg_operator_apply(op.h, V.h, W.h, 1.0, 0.0, f))
-    capi.check(capi.lib().rcg_residuals(op.h, V.h, W.h, f, ev.ctypes.data_as(C.c_void_p),
-                                        r.ctypes.data_as(C.c_void_p)))
-    normA = max(abs(res.min_est), abs(res.max_est))
-    rel = r / normA
-    converged_per_solve = int(np.sum(rel <= 1e-10))
-    converged_total = sum(int(x[0]) for x in results) * converged_per_solve / max(f, 1)
-    value = converged_total / total
+    Wb = solver.Block(A.n, K, ctx=ctx)
+    fp64 = run("fp64", args.steps, args.warmup)
+    mixed = None if args.no_mixed else run("mixed", args.steps, args.warmup)
+    if mixed is not None:
+        ev64 = fp64["ev"]
+        floor = np.maximum(np.abs(ev64), 1e-11 / TOL)
+        n_cmp = min(len(ev64), len(mixed["ev"]))
+        mixed["max_rel_eig_diff_vs_fp64"] = float(np.max(
+            np.abs(mixed["ev"][:n_cmp] - ev64[:n_cmp]) / floor[:n_cmp])) if n_cmp else None
 
-    # dominant kernel: the fused Chebyshev step, HIP-event timed inside the solves
-    launches = sum(x[2].filter_degree_sum for x in results)
-    t_dev = sum(x[2].t_filter_device for x in results)
-    alg_bytes = sum(x[2].filter_bytes for x in results)
-    avg_launch_s = t_dev / max(launches, 1)
-    bytes_per_launch = alg_bytes / max(launches, 1)
-    achieved = alg_bytes / t_dev / 1e9 if t_dev > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(),
-                "kernel": "csr_block_kernel<double,32,1,kCheb> (fused Chebyshev step)",
-                "avg_launch_us": round(avg_launch_s * 1e6, 2),
-                "alg_bytes_per_launch": int(bytes_per_launch), "launches": int(launches)}
-
-    lat_ms = np.array(lat) * 1e3
-    last = results[-1][2]
+    total, f, ev, last = fp64["total"], fp64["f"], fp64["ev"], fp64["last"]
+    value, roofline = fp64["value"], fp64["roofline"]
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -226,16 +275,19 @@ def main():
             "n": A.n, "nnz": A.nnz, "k": K, "inc": INC, "buf": BUF, "tol": TOL,
             "precision": "fp64",
         },
-        "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
-        "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
-        "solve": {"found": int(f), "converged": converged_per_solve,
-                  "max_rel_residual": float(rel.max()) if f else None,
-                  "inner_iterations": int(last.count_eig), "filter_degrees": int(last.filter_degree_sum),
-                  "lanczos_iters": int(last.lanczos_iters),
-                  "t_lanczos_s": round(last.t_lanczos, 4), "t_filter_device_s": round(last.t_filter_device, 4),
-                  "t_ortho_s": round(last.t_ortho, 4), "t_rr_s": round(last.t_rr, 4),
-                  "lambda_min": float(ev[0]) if f else None, "lambda_k": float(ev[-1]) if f else None},
+        "p50_ms": fp64["p50_ms"],
+        "p99_ms": fp64["p99_ms"],
+        "solve": fp64["solve"],
         "roofline": roofline,
+        "mixed": None if mixed is None else {
+            "value": round(mixed["value"], 3), "unit": "eigenpairs/s",
+            "speedup_vs_fp64": round(mixed["value"] / value, 3) if value > 0 else None,
+            "ms_per_step": round(mixed["total"] / args.steps * 1e3, 2),
+            "p50_ms": mixed["p50_ms"], "p99_ms": mixed["p99_ms"],
+            "max_rel_eig_diff_vs_fp64": mixed["max_rel_eig_diff_vs_fp64"],
+            "dtype": "f32 filter steps, f64 start/recombination, orthonormalisation, "
+                     "Rayleigh-Ritz and refinement",
+            "solve": mixed["solve"], "roofline": mixed["roofline"]},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -45,6 +45,9 @@ inline void check(int status) {
 class Context {
  public:
   explicit Context(int device = 0, void* hip_stream = nullptr) {
+    if (rcg_sizeof_solve_params() != (int64_t)sizeof(rcg_solve_params) ||
+        rcg_sizeof_solve_result() != (int64_t)sizeof(rcg_solve_result))
+      throw std::runtime_error("rcg: librcg.so and rcg_c_api.h disagree on the solve structs");
     check(rcg_context_create(device, hip_stream, &h_));
   }
   ~Context() { rcg_context_destroy(h_); }

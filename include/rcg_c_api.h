@@ -124,6 +124,16 @@ int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alph
  * used as the second recurrence buffer. */
 int rcg_cheb_filter(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols, int64_t degree,
                     int64_t repetition, double lambda_max, double shift);
+/* No counterpart in the reference (its filter is FP64 only): the same filter
+ * with FP32 arithmetic.  Columns with ritz[c] != 0 are split as
+ * X_j = u_j(theta[c]) x + E_j (u_j the filter's scalar recurrence at the
+ * Ritz value theta[c]); E_j runs in FP32 from the FP64 residual
+ * A x - theta[c] x, so the FP32 rounding error scales with that residual.
+ * Other columns (all when ritz is NULL) are filtered directly in FP32.  The
+ * start and the final recombination are FP64; X stays FP64. */
+int rcg_cheb_filter_mixed(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols,
+                          int64_t degree, int64_t repetition, double lambda_max, double shift,
+                          const double* theta, const int32_t* ritz);
 /* RayleighChebyshevLM::orthogonalize (RayleighChebyshevLM.h:1782-1837) /
  * cuSOLVER geqrf+orgqr (RayleighChebyshevLMCuda.h:1979-2107): X <- Q with
  * X = QR, R upper triangular with positive diagonal (CholQR2, shifted when
@@ -171,7 +181,10 @@ typedef struct rcg_solve_params {
   int64_t lanczos_itmax;  /* setMaxSpectralBoundsIter, 0 = 10000 */
   double maxmin_tol;      /* setMaxMinTolerance, 0 = 1e-6 */
   int32_t fixed_iteration;/* setFixedIterationCount */
-  int32_t filter_dtype;   /* RCG_F64 (plain) or RCG_F32 (mixed precision filter) */
+  int32_t filter_dtype;   /* RCG_F64 (plain) or RCG_F32: mixed precision -- FP32
+                             Chebyshev steps on the correction X_j - u_j(theta) v
+                             of each Ritz column, FP64 everywhere else, and the
+                             final FP64 refinement sweep */
   int32_t refine;         /* 1: final FP64 Rayleigh-Ritz over all found vectors */
   int32_t verbose;        /* setVerboseFlag: one line per inner iteration on stdout */
   int32_t use_bounds;     /* 1: getMinEigenSystem(k, min_eig, max_eig, ...) (no Lanczos) */
@@ -192,9 +205,12 @@ typedef struct rcg_solve_result {
   uint64_t rng_counter;
   int64_t filter_degree_sum; /* Chebyshev step launches (sum of starDegree * rep) */
   double t_total, t_lanczos, t_filter, t_ortho, t_rr; /* host wall seconds per phase */
-  double t_filter_device;    /* seconds between hipEvents around the filter steps */
-  double filter_bytes;       /* algorithmic bytes of those steps (CSR + 3 block passes) */
+  double t_filter_device;    /* seconds between hipEvents around the Chebyshev step
+                                launches (FP64 steps, or the FP32 correction steps) */
+  double filter_bytes;       /* algorithmic bytes of those steps: CSR + 3 block passes
+                                (FP64) or CSR(FP32 values) + 4 FP32 block passes */
   int64_t filter_panel;      /* autotuned column-panel width of the filter step */
+  int64_t filter_launches;   /* step kernel launches inside t_filter_device */
 } rcg_solve_result;
 
 /* RayleighChebyshevLM::getMinEigenSystem (RayleighChebyshevLM.h:693-738;

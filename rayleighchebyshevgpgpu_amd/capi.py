@@ -38,7 +38,7 @@ class SolveResult(C.Structure):
         ("rng_counter", C.c_uint64), ("filter_degree_sum", C.c_int64), ("t_total", C.c_double),
         ("t_lanczos", C.c_double), ("t_filter", C.c_double), ("t_ortho", C.c_double),
         ("t_rr", C.c_double), ("t_filter_device", C.c_double), ("filter_bytes", C.c_double),
-        ("filter_panel", C.c_int64),
+        ("filter_panel", C.c_int64), ("filter_launches", C.c_int64),
     ]
 
 
@@ -80,6 +80,7 @@ def lib() -> C.CDLL:
         "rcg_operator_set_grid": ([vp, i64, i64, i64], i32),
         "rcg_operator_apply": ([vp, vp, vp, d, d, i64], i32),
         "rcg_cheb_filter": ([vp, vp, vp, i64, i64, i64, d, d], i32),
+        "rcg_cheb_filter_mixed": ([vp, vp, vp, i64, i64, i64, d, d, vp, vp], i32),
         "rcg_orthonormalize": ([vp, vp, i64], i32),
         "rcg_deflate": ([vp, i64, vp, i64], i32),
         "rcg_project": ([vp, vp, vp, i64, vp], i32),

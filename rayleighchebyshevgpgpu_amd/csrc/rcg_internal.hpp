@@ -71,6 +71,26 @@ void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val
                       const T* X0, T* out, int ld, int ncols, T g1, T g2, T s, bool first,
                       hipStream_t st, int panel = 0, const CsrSched* sched = nullptr);
 
+// Mixed-precision filter.  The FP64 filter output s*X_m of X_0 = v is split
+// per column as X_j = u_j(theta_c) v + E_j, u_j the scalar recurrence of the
+// step at the Ritz value theta_c; E_j (FP32) then obeys
+//   E_j = g2*(g1*A*E_{j-1} + E_{j-1}) - E_{j-2} + kap_j[c] * R,
+//   kap_j[c] = g1*g2*u_{j-1}(theta_c), R = A v - theta_c v (FP64 -> FP32),
+// so FP32 rounding is relative to the residual, not to v.
+void launch_cheb_step_src(int64_t n, const int* rowptr, const int* col, const float* val,
+                          const float* X1, const float* X0, float* out, const float* R,
+                          const float* kap, int ld, int ncols, float g1, float g2, float s,
+                          hipStream_t st, int panel = 0, const CsrSched* sched = nullptr);
+// E0, E1, R from V and AV (see the kernel); cp[c] = theta_c, cp[nc+c] = 1 for
+// Ritz columns, 0 for plain ones (E_j = X_j, R = 0).
+void launch_mixed_start(int64_t n, const double* V, const double* AV, int ld, int nc,
+                        const double* cp, double e1, double g1, double g2, float* E0, float* E1,
+                        float* R, hipStream_t st);
+// V <- s * (um[c] V + E).
+void launch_mixed_combine(int64_t n, double* V, int ld, int nc, const double* um, double s,
+                          const float* E, hipStream_t st);
+void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st);
+
 // G (bx x by, column-major, ldg = bx) = X^H Y over n rows; ws holds partials.
 template <typename T>
 void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, int by, T* G,

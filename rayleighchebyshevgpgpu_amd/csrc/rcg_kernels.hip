@@ -108,7 +108,10 @@ struct V16<float> {
 };
 
 // ------------------------------------------------------------ CSR x block
-enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3 };
+// kChebSrc: kCheb plus a per-column multiple of a source block,
+//   out = p2 * (p1*(p0*A*X + X) - Z + kap[c] * R),
+// the correction recurrence of the mixed-precision filter.
+enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3, kChebSrc = 4 };
 
 // Rows per workgroup block (cap) and nonzeros staged in LDS per block.
 constexpr int kMaxRpb = 512;
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
     T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk, const int* __restrict__ order,
-    int remap) {
+    int remap, const T* __restrict__ R, const T* __restrict__ kap) {
   using VT = V16<T>;
   constexpr int W = VT::W;
   constexpr int RS = kThreads / G;
@@ -164,6 +167,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     }
     // own-row operands first: they do not depend on the gathers
     T xs[RPI][NV][W], zs[RPI][NV][W];
+    T rs[MODE == kChebSrc ? RPI : 1][NV][W];
 #pragma unroll
     for (int q = 0; q < RPI; ++q) {
       if (!live_row[q]) continue;
@@ -172,8 +176,10 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
       for (int k = 0; k < NV; ++k) {
         const int cv = g + k * G;
         if (cv < ncv) {
-          if constexpr (MODE == kChebFirst || MODE == kCheb) VT::ld(X + rowoff + cv * W, xs[q][k]);
-          if constexpr (MODE == kCheb) VT::ld_stream(Z + rowoff + cv * W, zs[q][k]);
+          if constexpr (MODE == kChebFirst || MODE == kCheb || MODE == kChebSrc)
+            VT::ld(X + rowoff + cv * W, xs[q][k]);
+          if constexpr (MODE == kCheb || MODE == kChebSrc) VT::ld_stream(Z + rowoff + cv * W, zs[q][k]);
+          if constexpr (MODE == kChebSrc) VT::ld_stream(R + rowoff + cv * W, rs[q][k]);
           if constexpr (MODE == kApplyBeta) VT::ld(out + rowoff + cv * W, zs[q][k]);
         }
       }
@@ -251,6 +257,11 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
         } else if constexpr (MODE == kChebFirst) {
 #pragma unroll
           for (int w = 0; w < W; ++w) o[w] = p2 * (p1 * fma(p0, acc[q][k][w], xs[q][k][w]));
+        } else if constexpr (MODE == kChebSrc) {
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            o[w] = p2 * fma(kap[cv * W + w], rs[q][k][w],
+                            fma(p1, fma(p0, acc[q][k][w], xs[q][k][w]), -zs[q][k][w]));
         } else {
 #pragma unroll
           for (int w = 0; w < W; ++w)
@@ -271,7 +282,8 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
 
 template <typename T, int MODE, int G, int NV>
 void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
-                  T* out, int ld, int ncv, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc) {
+                  T* out, int ld, int ncv, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
+                  const T* R, const T* kap) {
   constexpr int RS = kThreads / G;
   int64_t rows_per_blk, nblk;
   const int* order = nullptr;
@@ -294,18 +306,19 @@ void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T*
   const int64_t grid = round_up(nblk, kXcds);
   hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE, RCG_CSR_RPI>), dim3((unsigned)grid), dim3(kThreads), 0,
                      st, n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, rows_per_blk, nblk, order,
-                     remap);
+                     remap, R, kap);
   RCG_HIP(hipGetLastError());
 }
 
 template <typename T, int MODE>
 void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
-                T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc) {
+                T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
+                const T* R = nullptr, const T* kap = nullptr) {
   constexpr int W = V16<T>::W;
   const int ncv = (ncols + W - 1) / W;
   if (ld % 4 != 0) throw Error(1, "block leading dimension must be a multiple of 4");
 #define RCG_CSR_CASE(LIM, GG, NN) \
-  if (ncv <= LIM) return csr_launch_g<T, MODE, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st, sc);
+  if (ncv <= LIM) return csr_launch_g<T, MODE, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st, sc, R, kap);
   RCG_CSR_CASE(1, 1, 1)
   RCG_CSR_CASE(2, 2, 1)
   RCG_CSR_CASE(4, 4, 1)
@@ -515,30 +528,37 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
-  for (int64_t i = i0 + 4 * wave; i < i1; i += 16) {
-    const int64_t row = i + kr;
-    const bool ok = row < i1;
-    double xa[4], yb[4];
+  // four 16-row groups per pass: all of their loads are issued before the
+  // MFMAs that consume them
+  constexpr int U = 4;
+  for (int64_t ib = i0 + 4 * wave; ib < i1; ib += 16 * U) {
+    double xa[U][4], yb[U][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int c = a0 + 16 * t + cc;
-      xa[t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
-    }
-    if constexpr (SYM) {
-#pragma unroll
-      for (int t = 0; t < 4; ++t) yb[t] = xa[t];
-    } else {
+    for (int u = 0; u < U; ++u) {
+      const int64_t row = ib + 16 * u + kr;
+      const bool ok = row < i1;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const int c = b0 + 16 * t + cc;
-        yb[t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
+        const int c = a0 + 16 * t + cc;
+        xa[u][t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
+      }
+      if constexpr (!SYM) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int c = b0 + 16 * t + cc;
+          yb[u][t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
+        }
       }
     }
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (!SYM || p <= q) acc[p][q] = mfma16x16x4(xa[p], yb[q], acc[p][q]);
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (SYM && p > q) continue;
+          acc[p][q] = mfma16x16x4(xa[u][p], SYM ? xa[u][q] : yb[u][q], acc[p][q]);
+        }
   }
   // the four waves' tiles are summed in LDS in wave order
   for (int w = 0; w < 4; ++w) {
@@ -571,61 +591,83 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   }
 }
 
-// out = alpha X C + beta Z (X: n x bx, C: bx x by column-major) computed as
-// out^T = C^T X^T: A fragments come from C staged in LDS, B fragments from
-// the wave's 16 rows of X staged in LDS; K runs over bx in chunks of 64.
-// Workgroup = 4 waves x 16 rows; blockIdx.y selects 64 output columns.
+// out = alpha X C + beta Z (X: n x bx, C: bx x by column-major).  A wave owns
+// 16 rows of a 64-row tile; lane (kk = l/16, r = l%16) loads the 16
+// consecutive entries [16kk, 16kk+16) of row r of a 64-wide k chunk with
+// 16-byte loads, and MFMA step s contracts k = 16kk + s: A[r][kk] =
+// X[r][16kk+s], B[kk][c] = C[16kk+s][c] (LDS).  D[4j+kk][c] lands in register
+// j, so 16 lanes store 16 consecutive columns of one row.  Workgroups loop
+// over row tiles; C is staged once when bx <= 64.  blockIdx.y selects 64
+// output columns.
 __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ C,
     int ldc, int by, double alpha, double beta, const double* Z, double* out, int ldo,
-    int64_t nrb) {
-  __shared__ double cs[kTile][kTile + 1];
-  __shared__ double xs[4][16][kTile + 1];
-  const int64_t rb = xcd_remap(blockIdx.x, gridDim.x);
-  if (rb >= nrb) return;
+    int64_t ntiles) {
+  __shared__ double cs[kTile][kTile + 1];  // [k][c]
   const int c0 = blockIdx.y * kTile;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int kr = lane >> 4;
-  const int cc = lane & 15;
-  const int64_t ibase = rb * kTile + wave * 16;
-  d4v acc[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = d4v{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < bx; k0 += kTile) {
-    if (k0 > 0) __syncthreads();
+  const int kk = lane >> 4;
+  const int r = lane & 15;
+  const int nk = (bx + kTile - 1) / kTile;
+  const bool vec_ok = ((reinterpret_cast<uintptr_t>(X) & 15) == 0) && (ldx % 2 == 0);
+  auto stage = [&](int k0) {
     for (int e = threadIdx.x; e < kTile * kTile; e += kThreads) {
       const int k = e % kTile;
       const int c = e / kTile;
       cs[k][c] = (k0 + k < bx && c0 + c < by) ? C[(k0 + k) + (int64_t)(c0 + c) * ldc] : 0.0;
     }
-    for (int e = lane; e < 16 * kTile; e += 64) {
-      const int r = e / kTile;
-      const int k = e % kTile;
-      const int64_t i = ibase + r;
-      xs[wave][r][k] = (i < n && k0 + k < bx) ? X[i * ldx + k0 + k] : 0.0;
-    }
+  };
+  if (nk == 1) {
+    stage(0);
     __syncthreads();
-    const int kn = min(kTile, bx - k0);
-    for (int ks = 0; ks < kn; ks += 4) {
-      const int k = ks + kr;
-      const double bf = xs[wave][cc][k];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) acc[t] = mfma16x16x4(cs[k][16 * t + cc], bf, acc[t]);
-    }
   }
-  const int64_t i = ibase + cc;
-  if (i >= n) return;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t row = tile * kTile + wave * 16 + r;
+    d4v acc[4];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < 4; ++t) acc[t] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int ci = 0; ci < nk; ++ci) {
+      const int k0 = ci * kTile;
+      if (nk > 1) {
+        __syncthreads();
+        stage(k0);
+        __syncthreads();
+      }
+      double xa[16];
+      const int kb = k0 + 16 * kk;
+      if (row < n && kb + 16 <= bx && vec_ok) {
+        const double* xp = X + row * ldx + kb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const dvec2 t = *reinterpret_cast<const dvec2*>(xp + 2 * q);
+          xa[2 * q] = t.x;
+          xa[2 * q + 1] = t.y;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xa[q] = (row < n && kb + q < bx) ? X[row * ldx + kb + q] : 0.0;
+      }
+#pragma unroll
+      for (int sidx = 0; sidx < 16; ++sidx) {
+        const int k = 16 * kk + sidx;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = mfma16x16x4(xa[sidx], cs[k][16 * t + r], acc[t]);
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int c = c0 + 16 * t + 4 * j + kr;
-      if (c >= by) continue;
-      const int64_t off = i * ldo + c;
-      double v = alpha * acc[t][j];
-      if (beta != 0.0) v = fma(beta, Z[off], v);
-      out[off] = v;
+      const int64_t ro = tile * kTile + wave * 16 + 4 * j + kk;
+      if (ro >= n) continue;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int c = c0 + 16 * t + r;
+        if (c >= by) continue;
+        const int64_t off = ro * ldo + c;
+        double v = alpha * acc[t][j];
+        if (beta != 0.0) v = fma(beta, Z[off], v);
+        out[off] = v;
+      }
     }
   }
 }
@@ -721,6 +763,53 @@ __global__ __launch_bounds__(kThreads) void fill_cols_kernel(int64_t n, T* X, in
   const int j = (int)(e % cnt);
   if (i >= n) return;
   X[i * ld + c0 + j] = value;
+}
+
+// ------------------------------------------------ mixed-precision filter
+// Start of the correction recurrence X_j = u_j(theta_c) v + E_j: from
+// V (FP64) and AV (FP64), per column c < nc with cp[c] = theta_c and
+// cp[nc + c] = 1 (Ritz column) or 0 (plain column):
+//   Ritz : R = AV - theta V, E0 = 0, E1 = e1 * R
+//   plain: R = 0,            E0 = V, E1 = g2*(g1*AV + V)
+// E0, E1, R are FP32 blocks of the same ld.
+__global__ __launch_bounds__(kThreads) void mixed_start_kernel(
+    int64_t n, const double* __restrict__ V, const double* __restrict__ AV, int ld, int nc,
+    const double* __restrict__ cp, double e1, double g1, double g2, float* __restrict__ E0,
+    float* __restrict__ E1, float* __restrict__ R) {
+  const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  const int64_t i = e / nc;
+  const int c = (int)(e % nc);
+  if (i >= n) return;
+  const int64_t k = i * ld + c;
+  const double v = V[k], w = AV[k];
+  if (cp[nc + c] != 0.0) {
+    const double r = w - cp[c] * v;
+    R[k] = float(r);
+    E0[k] = 0.0f;
+    E1[k] = float(e1 * r);
+  } else {
+    R[k] = 0.0f;
+    E0[k] = float(v);
+    E1[k] = float(g2 * fma(g1, w, v));
+  }
+}
+
+// V <- s * (um[c] * V + E), c < nc (E FP32).
+__global__ __launch_bounds__(kThreads) void mixed_combine_kernel(int64_t n, double* V, int ld,
+                                                                 int nc, const double* __restrict__ um,
+                                                                 double s, const float* __restrict__ E) {
+  const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  const int64_t i = e / nc;
+  const int c = (int)(e % nc);
+  if (i >= n) return;
+  const int64_t k = i * ld + c;
+  V[k] = s * fma(um[c], V[k], double(E[k]));
+}
+
+__global__ __launch_bounds__(kThreads) void to_f32_kernel(int64_t n, const double* __restrict__ x,
+                                                          float* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  if (i < n) y[i] = float(x[i]);
 }
 
 // ----------------------------------------------------------------- Lanczos
@@ -854,6 +943,46 @@ void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val
   }
 }
 
+void launch_cheb_step_src(int64_t n, const int* rowptr, const int* col, const float* val,
+                          const float* X1, const float* X0, float* out, const float* R,
+                          const float* kap, int ld, int ncols, float g1, float g2, float s,
+                          hipStream_t st, int panel, const CsrSched* sc) {
+  if (n <= 0 || ncols <= 0) return;
+  if (panel <= 0 || panel >= ncols) panel = ncols;
+  if (panel % 4 != 0) panel = round_up(panel, 4);
+  for (int c0 = 0; c0 < ncols; c0 += panel) {
+    const int pc = std::min(panel, ncols - c0);
+    csr_launch<float, kChebSrc>(n, rowptr, col, val, X1 + c0, X0 + c0, out + c0, ld, pc, g1, g2, s,
+                                st, sc, R + c0, kap + c0);
+  }
+}
+
+void launch_mixed_start(int64_t n, const double* V, const double* AV, int ld, int nc,
+                        const double* cp, double e1, double g1, double g2, float* E0, float* E1,
+                        float* R, hipStream_t st) {
+  if (n <= 0 || nc <= 0) return;
+  const int64_t tot = n * nc;
+  hipLaunchKernelGGL(mixed_start_kernel, dim3((unsigned)((tot + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, n, V, AV, ld, nc, cp, e1, g1, g2, E0, E1, R);
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_mixed_combine(int64_t n, double* V, int ld, int nc, const double* um, double s,
+                          const float* E, hipStream_t st) {
+  if (n <= 0 || nc <= 0) return;
+  const int64_t tot = n * nc;
+  hipLaunchKernelGGL(mixed_combine_kernel, dim3((unsigned)((tot + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, n, V, ld, nc, um, s, E);
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads),
+                     0, st, n, x, y);
+  RCG_HIP(hipGetLastError());
+}
+
 template <typename T>
 size_t gram_ws_elems(int64_t n, int bx, int by) {
   int64_t rpb;
@@ -910,8 +1039,10 @@ void launch_tall_gemm(int64_t n, const T* X, int ldx, int bx, const T* C, int ld
   const int64_t grid = round_up(nrb, kXcds);
   const unsigned ct = (unsigned)((by + kTile - 1) / kTile);
   if constexpr (std::is_same<T, double>::value) {
-    hipLaunchKernelGGL(tall_gemm_mfma_kernel, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
-                       ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);
+    // persistent over row tiles: 4 workgroups per CU of a 256-CU part
+    const int64_t pgrid = std::min<int64_t>(nrb, 1024);
+    hipLaunchKernelGGL(tall_gemm_mfma_kernel, dim3((unsigned)pgrid, ct), dim3(kThreads), 0, st, n,
+                       X, ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);
   } else {
     hipLaunchKernelGGL(tall_gemm_kernel<T>, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
                        ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);

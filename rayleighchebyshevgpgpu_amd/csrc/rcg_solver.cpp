@@ -119,7 +119,8 @@ struct rcg_operator_s {
     int panel;
     int sched;
   };
-  std::map<int64_t, Tuned> tuned;  // autotuned filter shape per (ld, columns)
+  std::map<int64_t, Tuned> tuned;  // autotuned filter shape per (kind, ld, columns)
+  DevBuf val32;                    // FP32 copy of val for the mixed-precision filter
 };
 
 namespace {
@@ -265,11 +266,14 @@ class Engine {
 
   // Filter shape for b columns: the fastest row-block order, then the fastest
   // column-panel width {b, 32, 16} under it, timed with hipEvents on this
-  // operator (3 Chebyshev steps after a warm-up step) and cached on it.
-  // RCG_FILTER_PANEL / RCG_FILTER_SCHED override the choice.
-  rcg_operator_s::Tuned tuned_for(int64_t b) {
+  // operator (3 Chebyshev steps after a warm-up step) and cached on it per
+  // step kind (0: FP64 step, 1: FP32 correction step).  `launch(panel, sched)`
+  // runs one step on scratch buffers.  RCG_FILTER_PANEL / RCG_FILTER_SCHED
+  // override the choice.
+  template <class Launch>
+  rcg_operator_s::Tuned tuned_for(int64_t b, int kind, Launch&& launch) {
     build_schedules();
-    const int64_t key = ld_ * 100000 + b;
+    const int64_t key = (int64_t)kind * 1000000000000LL + ld_ * 100000 + b;
     auto it = op_->tuned.find(key);
     rcg_operator_s::Tuned t{(int)b, 0};
     if (it != op_->tuned.end()) {
@@ -280,12 +284,9 @@ class Engine {
       RCG_HIP(hipEventCreate(&e1));
       auto time_it = [&](int panel, int sched) {
         const rcg::CsrSched* sc = &op_->scheds[sched]->sc;
-        rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, V(), T(), T(), (int)ld_,
-                                      (int)b, 0.0, 1.0, 0.0, false, st_, panel, sc);
+        launch(panel, sc);
         RCG_HIP(hipEventRecord(e0, st_));
-        for (int r = 0; r < 3; ++r)
-          rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, V(), T(), T(),
-                                        (int)ld_, (int)b, 0.0, 1.0, 0.0, false, st_, panel, sc);
+        for (int r = 0; r < 3; ++r) launch(panel, sc);
         RCG_HIP(hipEventRecord(e1, st_));
         RCG_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -322,6 +323,12 @@ class Engine {
     }
     return t;
   }
+  rcg_operator_s::Tuned tuned_for(int64_t b) {
+    return tuned_for(b, 0, [&](int panel, const rcg::CsrSched* sc) {
+      rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, V(), T(), T(), (int)ld_,
+                                    (int)b, 0.0, 1.0, 0.0, false, st_, panel, sc);
+    });
+  }
   const rcg::CsrSched* sched_for(int64_t b) {
     if (!op_ || n_ * b < (int64_t)1 << 22) return nullptr;
     return &op_->scheds[tuned_for(b).sched]->sc;
@@ -335,6 +342,10 @@ class Engine {
     const rcg::CsrSched* sc = &op_->scheds[tn.sched]->sc;
     last_panel_ = tn.panel;
     last_sched_ = op_->scheds[tn.sched]->name;
+    const double csr = double(op_->nnz) * 12.0 + 4.0 * double(n_ + 1);
+    const double blk = 8.0 * double(n_) * double(b);
+    step_begin(rep * m * ((b + tn.panel - 1) / tn.panel),
+               double(rep) * (csr + 2.0 * blk + double(m - 1) * (csr + 3.0 * blk)));
     for (long r = 0; r < rep; ++r) {
       double* X0 = V();
       double* X1 = T();
@@ -349,6 +360,124 @@ class Engine {
       }
       if (X1 != V()) swapVT();
     }
+    step_end();
+  }
+
+  // Mixed-precision filter (the correction recurrence of
+  // launch_cheb_step_src): V <- P_m(A')^rep V over b columns with FP32 steps
+  // and FP32 CSR values.  theta[c] / ritz[c]: the Ritz value of column c and
+  // whether column c is a Ritz vector (others are filtered directly in FP32).
+  // Needs m >= 2 and the FP32 buffers attached with attach_mixed.
+  void filter_mixed(int64_t b, long m, long rep, double lmax, double shift,
+                    const std::vector<double>& theta, const std::vector<char>& ritz) {
+    if (m < 2) return filter(b, m, rep, lmax, shift);
+    if (!E_[0]) throw Error(RCG_ERR_INVALID, "mixed-precision buffers are not attached");
+    const auto c = rcg::cpoly::coefficients(m, lmax, shift);
+    const double g12 = c.g1 * c.g2;
+    // coefficient table: cp (theta, flag) [2b doubles], um [ld doubles],
+    // kap rows j = 0..m [ld floats each]
+    const size_t nd = 2 * (size_t)b + (size_t)ld_;
+    const size_t bytes = sizeof(double) * nd + sizeof(float) * (size_t)(m + 1) * (size_t)ld_;
+    sync();  // the pinned table of the previous call is no longer in flight
+    coef_.ensure(bytes);
+    hcoef_.ensure(bytes);
+    double* hd = hcoef_.as<double>();
+    float* hk = reinterpret_cast<float*>(hd + nd);
+    std::fill(hd, hd + nd, 0.0);
+    std::fill(hk, hk + (size_t)(m + 1) * (size_t)ld_, 0.0f);
+    for (int64_t col = 0; col < b; ++col) {
+      if (!ritz[col]) continue;
+      hd[col] = theta[col];
+      hd[b + col] = 1.0;
+      const double l = c.g2 * (1.0 + c.g1 * theta[col]);
+      double u0 = 1.0, u1 = l;  // u_{j-2}, u_{j-1}
+      for (long j = 2; j <= m; ++j) {
+        hk[(size_t)j * ld_ + col] = float(g12 * u1);
+        const double u = l * u1 - u0;
+        u0 = u1;
+        u1 = u;
+      }
+      hd[2 * b + col] = u1;  // u_m
+    }
+    RCG_HIP(hipMemcpyAsync(coef_.p, hcoef_.p, bytes, hipMemcpyHostToDevice, st_));
+    const double* dcp = coef_.as<double>();
+    const double* dum = dcp + 2 * b;
+    const float* dk = reinterpret_cast<const float*>(dcp + nd);
+    const float* val32 = values_f32();
+    const float g1f = float(c.g1), g2f = float(c.g2);
+    const auto tn = tuned_for(b, 1, [&](int panel, const rcg::CsrSched* sc) {
+      rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, E_[0], E_[1], E_[1], R32_,
+                                dk + 2 * ld_, (int)ld_, (int)b, g1f, g2f, 1.0f, st_, panel, sc);
+    });
+    const rcg::CsrSched* sc = &op_->scheds[tn.sched]->sc;
+    last_panel_ = tn.panel;
+    last_sched_ = op_->scheds[tn.sched]->name;
+    const double csr = double(op_->nnz) * 8.0 + 4.0 * double(n_ + 1);
+    const double blk = 4.0 * double(n_) * double(b);
+    for (long r = 0; r < rep; ++r) {
+      rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, V(), W_, (int)ld_, (int)b,
+                                    1.0, 0.0, st_, sched_for(b));
+      rcg::launch_mixed_start(n_, V(), W_, (int)ld_, (int)b, dcp, g12, c.g1, c.g2, E_[0], E_[1],
+                              R32_, st_);
+      float* X0 = E_[0];
+      float* X1 = E_[1];
+      step_begin((m - 1) * ((b + tn.panel - 1) / tn.panel), double(m - 1) * (csr + 4.0 * blk));
+      for (long j = 2; j <= m; ++j) {
+        rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, X1, X0, X0, R32_,
+                                  dk + (size_t)j * ld_, (int)ld_, (int)b, g1f, g2f, 1.0f, st_,
+                                  tn.panel, sc);
+        std::swap(X0, X1);
+      }
+      step_end();
+      rcg::launch_mixed_combine(n_, V(), (int)ld_, (int)b, dum, 1.0 / double(m + 1), X1, st_);
+    }
+  }
+  void attach_mixed(float* E0, float* E1, float* R) {
+    E_[0] = E0;
+    E_[1] = E1;
+    R32_ = R;
+  }
+  const float* values_f32() {
+    if (!op_->val32.p) {
+      op_->val32.ensure(sizeof(float) * (size_t)std::max<int64_t>(op_->nnz, 1));
+      rcg::launch_to_f32(op_->nnz, op_->val, op_->val32.as<float>(), st_);
+    }
+    return op_->val32.as<float>();
+  }
+
+  // Device time of the Chebyshev step launches (hipEvents around each run
+  // of steps), their count and algorithmic bytes; read once the stream has
+  // passed the end event.
+  void step_begin(long launches, double bytes) {
+    if (!fev_[0]) {
+      RCG_HIP(hipEventCreate(&fev_[0]));
+      RCG_HIP(hipEventCreate(&fev_[1]));
+    }
+    collect_step_time();
+    RCG_HIP(hipEventRecord(fev_[0], st_));
+    pend_launches_ = launches;
+    pend_bytes_ = bytes;
+  }
+  void step_end() {
+    RCG_HIP(hipEventRecord(fev_[1], st_));
+    step_pending_ = true;
+  }
+  void collect_step_time() {
+    if (!step_pending_) return;
+    RCG_HIP(hipEventSynchronize(fev_[1]));
+    float ms = 0.f;
+    RCG_HIP(hipEventElapsedTime(&ms, fev_[0], fev_[1]));
+    step_seconds_ += 1.0e-3 * ms;
+    step_launches_ += pend_launches_;
+    step_bytes_ += pend_bytes_;
+    step_pending_ = false;
+  }
+  double step_seconds() { collect_step_time(); return step_seconds_; }
+  double step_bytes() { collect_step_time(); return step_bytes_; }
+  long step_launches() { collect_step_time(); return step_launches_; }
+  ~Engine() {
+    for (auto& e : fev_)
+      if (e) (void)hipEventDestroy(e);
   }
   int last_panel() const { return last_panel_; }
   const std::string& last_sched() const { return last_sched_; }
@@ -578,6 +707,14 @@ class Engine {
   size_t ws_elems_ = 0;
   int last_panel_ = 0;
   std::string last_sched_;
+  float* E_[2] = {nullptr, nullptr};  // FP32 correction blocks (mixed filter)
+  float* R32_ = nullptr;              // FP32 residual block (mixed filter)
+  DevBuf coef_;                       // mixed filter coefficient table
+  HostBuf hcoef_;
+  hipEvent_t fev_[2] = {nullptr, nullptr};
+  bool step_pending_ = false;
+  long pend_launches_ = 0, step_launches_ = 0;
+  double pend_bytes_ = 0.0, step_bytes_ = 0.0, step_seconds_ = 0.0;
 };
 
 // ------------------------------------------------------------- the solver
@@ -587,14 +724,7 @@ struct TraceRow {
 
 class Solver {
  public:
-  Solver(rcg_operator_s* op, const rcg_solve_params& p) : op_(op), p_(p), eng_(op->ctx, op, op->n) {
-    RCG_HIP(hipEventCreate(&ev_[0]));
-    RCG_HIP(hipEventCreate(&ev_[1]));
-  }
-  ~Solver() {
-    (void)hipEventDestroy(ev_[0]);
-    (void)hipEventDestroy(ev_[1]);
-  }
+  Solver(rcg_operator_s* op, const rcg_solve_params& p) : op_(op), p_(p), eng_(op->ctx, op, op->n) {}
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
 
@@ -640,6 +770,14 @@ class Solver {
     RCG_HIP(hipMemsetAsync(Wb_.p, 0, blk, st));
     eng_.reserve(ld, ldy);
     eng_.attach(Vb_.as<double>(), Tb_.as<double>(), Wb_.as<double>(), ld, Yb_.as<double>(), ldy);
+    const bool mixed = p_.filter_dtype == RCG_F32;
+    if (mixed) {  // E_{j-1}, E_{j-2} and R of the FP32 correction recurrence
+      const size_t fb = sizeof(float) * (size_t)n * (size_t)ld;
+      Mb_.ensure(3 * fb);
+      RCG_HIP(hipMemsetAsync(Mb_.p, 0, 3 * fb, st));
+      float* m0 = Mb_.as<float>();
+      eng_.attach_mixed(m0, m0 + (size_t)n * ld, m0 + 2 * (size_t)n * ld);
+    }
 
     const long maxInner = p_.fixed_iteration ? 1 : (p_.max_inner > 0 ? (long)p_.max_inner : 10000);
     const long degMax = p_.deg_max > 0 ? (long)p_.deg_max : 200;
@@ -678,6 +816,10 @@ class Solver {
     const bool intervalFlag = p_.interval != 0;
 
     long foundSize = 0;
+    // Ritz value of each block column at filter time, and whether the column
+    // is a Ritz vector (the mixed filter's correction needs both)
+    std::vector<double> fTheta(subspaceSize, 0.0);
+    std::vector<char> fRitz(subspaceSize, 0);
     std::vector<double> theta(subspaceSize, 0.0), oldEigs(subspaceSize, 0.0),
         eigDiffs(subspaceSize, 1.0), oldEigDiffs(subspaceSize, 1.0), subspaceResiduals,
         residualHistory, H, Yh;
@@ -713,7 +855,10 @@ class Solver {
 
     while (!exitFlag) {
       for (long k = buf; k < subspaceSize; ++k) oldEigs[k] = oldEigs[buf];
-      if (applyCountCumulative > 0) eng_.randomize(buf, subspaceSize - buf, p_.seed, &counter_);
+      if (applyCountCumulative > 0) {
+        eng_.randomize(buf, subspaceSize - buf, p_.seed, &counter_);
+        for (long k = buf; k < subspaceSize; ++k) fRitz[k] = 0;
+      }
       double ta = wall();
       eng_.orthonormalize(subspaceSize);
       count_ortho_ += 2;
@@ -754,15 +899,11 @@ class Solver {
         }
         ta = wall();
         if (!completedBasis) {
-          RCG_HIP(hipEventRecord(ev_[0], eng_.stream()));
-          eng_.filter(subspaceSize, starDegree, rep, starBound, shift);
-          RCG_HIP(hipEventRecord(ev_[1], eng_.stream()));
+          if (mixed)
+            eng_.filter_mixed(subspaceSize, starDegree, rep, starBound, shift, fTheta, fRitz);
+          else
+            eng_.filter(subspaceSize, starDegree, rep, starBound, shift);
           degree_sum_ += starDegree * rep;
-          // algorithmic bytes: CSR stream + X_{j-1} read + own-row reads/writes
-          const double csr = double(op_->nnz) * 12.0 + 4.0 * double(op_->n + 1);
-          const double blk = 8.0 * double(op_->n) * double(subspaceSize);
-          filter_bytes_ += double(rep) * (csr + 2.0 * blk + double(starDegree - 1) * (csr + 3.0 * blk));
-          pending_event_ = true;
         }
         applyCount += 1;
         count_apply_ += starDegree * subspaceSize;
@@ -773,12 +914,6 @@ class Solver {
         eng_.orthonormalize(subspaceSize);
         count_ortho_ += 1;
         t_ortho_ += wall() - ta;
-        if (pending_event_) {  // orthonormalize synchronized the stream
-          float ms = 0.f;
-          RCG_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-          t_filter_dev_ += 1.0e-3 * ms;
-          pending_event_ = false;
-        }
 
         ta = wall();
         eng_.project(subspaceSize, H);
@@ -786,6 +921,8 @@ class Solver {
         rcg::dense::jacobi_eigensystem(H.data(), subspaceSize, theta, Yh);
         const long residualCheckCount = (foundSize + subspaceSize < N) ? inc : subspaceSize;
         eng_.rotate_residuals(subspaceSize, Yh, theta, residualCheckCount, subspaceResiduals);
+        fTheta.assign(theta.begin(), theta.end());
+        fRitz.assign(subspaceSize, 1);
         count_apply_ += residualCheckCount;
         count_eig_ += 1;
         t_rr_ += wall() - ta;
@@ -874,6 +1011,11 @@ class Solver {
         rcg::launch_copy_cols<double>(n, eng_.V(), (int)ld, (int)foundCount, eng_.T(), (int)ld, 0,
                                       (int)(subspaceSize - foundCount), st);
         eng_.swapVT();
+        for (long i = 0; i < subspaceSize; ++i) {
+          const bool kept = i + foundCount < subspaceSize;
+          fTheta[i] = kept ? theta[i + foundCount] : 0.0;
+          fRitz[i] = kept ? fRitz[i + foundCount] : 0;
+        }
       }
       const double gv = buf > 0 ? theta[subspaceSize - buf] : lambdaMax;  // guard (LM:1559-1587)
       guard_ = gv;
@@ -892,11 +1034,14 @@ class Solver {
           subspaceSize = inc;
           completedBasis = true;
           eng_.randomize(0, subspaceSize, p_.seed, &counter_);
+          fRitz.assign(subspaceSize, 0);
         } else {
           buf = N - (foundSize + inc);
           subspaceSize = inc + buf;
         }
         theta.resize(subspaceSize, 0.0);
+        fTheta.resize(subspaceSize, 0.0);
+        fRitz.resize(subspaceSize, 0);
       }
       ++pass;
     }
@@ -905,7 +1050,7 @@ class Solver {
     bool nonMonotone = false;
     for (long i = 0; i + 1 < foundSize; ++i)
       if (eigValues[i] > eigValues[i + 1]) nonMonotone = true;
-    if (nonMonotone || (p_.refine && foundSize > 0)) {
+    if (nonMonotone || ((p_.refine || mixed) && foundSize > 0)) {
       const double tr = wall();
       repair(foundSize, eigValues, eigRes);
       count_apply_ += 2 * foundSize;
@@ -937,8 +1082,9 @@ class Solver {
     R->t_ortho = t_ortho_;
     R->t_rr = t_rr_;
     R->filter_degree_sum = degree_sum_;
-    R->t_filter_device = t_filter_dev_;
-    R->filter_bytes = filter_bytes_;
+    R->t_filter_device = eng_.step_seconds();
+    R->filter_bytes = eng_.step_bytes();
+    R->filter_launches = eng_.step_launches();
     R->filter_panel = eng_.last_panel();
     return foundSize;
   }
@@ -985,13 +1131,10 @@ class Solver {
   rcg_operator_s* op_;
   rcg_solve_params p_;
   Engine eng_;
-  DevBuf Vb_, Tb_, Wb_, Yb_;
+  DevBuf Vb_, Tb_, Wb_, Yb_, Mb_;
   long found_ = 0;
   long count_ortho_ = 0, count_apply_ = 0, count_eig_ = 0, degree_sum_ = 0;
   double guard_ = 0.0, t_filter_ = 0.0, t_ortho_ = 0.0, t_rr_ = 0.0;
-  hipEvent_t ev_[2] = {nullptr, nullptr};
-  bool pending_event_ = false;
-  double t_filter_dev_ = 0.0, filter_bytes_ = 0.0;
 };
 
 void check_pair(rcg_operator op, rcg_block a, rcg_block b, int64_t cols) {
@@ -1274,6 +1417,36 @@ int rcg_cheb_filter(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols, i
   });
 }
 
+int rcg_cheb_filter_mixed(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols,
+                          int64_t degree, int64_t repetition, double lambda_max, double shift,
+                          const double* theta, const int32_t* ritz) {
+  return guarded([&] {
+    check_pair(op, X, tmp, cols);
+    need(X != tmp, "X and tmp must differ");
+    need(degree >= 1 && repetition >= 1, "degree and repetition must be >= 1");
+    need(!ritz || theta, "Ritz flags need Ritz values");
+    DeviceGuard g(op->ctx->device);
+    Piece pc(X, tmp, op);
+    // tmp doubles as the FP64 A*X block of the correction start
+    pc.eng.attach(static_cast<double*>(X->d()), static_cast<double*>(tmp->d()),
+                  static_cast<double*>(tmp->d()), X->ld, nullptr, 0);
+    const size_t fb = sizeof(float) * (size_t)X->n * (size_t)X->ld;
+    DevBuf mb;
+    mb.ensure(3 * fb);
+    RCG_HIP(hipMemsetAsync(mb.p, 0, 3 * fb, pc.eng.stream()));
+    float* m0 = mb.as<float>();
+    pc.eng.attach_mixed(m0, m0 + (size_t)X->n * X->ld, m0 + 2 * (size_t)X->n * X->ld);
+    std::vector<double> th(cols, 0.0);
+    std::vector<char> rz(cols, 0);
+    for (int64_t c = 0; c < cols; ++c) {
+      if (theta) th[c] = theta[c];
+      if (ritz) rz[c] = ritz[c] != 0;
+    }
+    pc.eng.filter_mixed(cols, (long)degree, (long)repetition, lambda_max, shift, th, rz);
+    pc.settle(X, cols);
+  });
+}
+
 int rcg_orthonormalize(rcg_block X, rcg_block tmp, int64_t cols) {
   return guarded([&] {
     need(X && tmp && X != tmp, "X and tmp must be distinct blocks");
@@ -1379,7 +1552,8 @@ int rcg_solve_lowest(rcg_operator op, const rcg_solve_params* params, rcg_solve_
   return guarded([&] {
     need(op && params && result && evals && res, "NULL argument");
     need(params->k >= 1, "eigCount must be >= 1");
-    need(params->filter_dtype == RCG_F64, "only the FP64 filter is available in this build");
+    need(params->filter_dtype == RCG_F64 || params->filter_dtype == RCG_F32,
+         "filter_dtype must be RCG_F64 or RCG_F32");
     if (evecs) need(evecs->n == op->n && evecs->dtype == RCG_F64 && evecs->ctx == op->ctx, "evecs block mismatch");
     DeviceGuard g(op->ctx->device);
     std::memset(result, 0, sizeof(*result));

@@ -33,8 +33,10 @@ def test_cpp_caller_links(tmp_path):
 def test_cpp_caller_solves():
     import __graft_entry__ as ge
     exe = ge.CPP_EXAMPLE
-    if not os.path.exists(exe):
-        ge.build_cpp_example()
+    deps = [SRC, LIB, os.path.join(ROOT, "include", "rcg_c_api.h"),
+            os.path.join(ROOT, "include", "rcg", "RayleighChebyshevLM.hpp")]
+    if not os.path.exists(exe) or os.path.getmtime(exe) < max(os.path.getmtime(d) for d in deps):
+        ge.build_cpp_example()  # stale against the header or the library
     print("cpp caller: 24x24 Dirichlet Laplacian, k=10", flush=True)
     p = subprocess.run([exe, "24", "10", "1e-10"], capture_output=True, text=True, timeout=120)
     print(p.stdout, p.stderr)
