@@ -73,7 +73,7 @@ int rcg_context_synchronize(rcg_context ctx);
 /* ------------------------------------------------------------ blocks */
 /* AmatrixClassLMCuda(m, n) / resize / create_cuda_memory
  * (AmatrixClassLMCuda.h:21-80, 251-266): an N x cols device block,
- * zero-filled. */
+ * zero-filled; dtype RCG_F64, RCG_F32 or RCG_C128 (ld counts elements). */
 int rcg_block_create(rcg_context ctx, int64_t n, int64_t cols, int dtype, rcg_block* out);
 int rcg_block_destroy(rcg_block b);
 /* host_to_device (AmatrixClassLMCuda.h:109-117): host column-major, leading
@@ -100,8 +100,11 @@ int rcg_block_info(rcg_block b, int64_t* n, int64_t* cols, int64_t* ld, int* dty
  * :236,245, RayleighChebyshevLMCuda.h:2135,2235): an N x N CSR matrix with
  * int32 indices and sorted columns.  Arrays are copied to the device
  * (mem = RCG_MEM_HOST) or referenced (RCG_MEM_DEVICE: the caller keeps them
- * alive).  val_dtype: RCG_F64 (the FP32 copy for mixed-precision filtering is
- * made on demand). */
+ * alive).  val_dtype: RCG_F64, or RCG_C128 for a Hermitian operator (values
+ * as interleaved re, im pairs, the layout of double _Complex); the FP32 /
+ * complex64 copy for mixed-precision filtering is made on demand.  Complex
+ * operators work with rcg_operator_apply, rcg_residuals, rcg_lanczos_bounds
+ * and rcg_solve_lowest; the other piecewise entry points are real only. */
 int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32_t* rowptr,
                             const int32_t* col, const void* val, int val_dtype, int mem,
                             rcg_operator* out);
@@ -112,7 +115,8 @@ int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32
 int rcg_operator_set_grid(rcg_operator op, int64_t nx, int64_t ny, int64_t nz);
 int rcg_operator_destroy(rcg_operator op);
 /* Otype::apply(in, out, alpha, beta): out = alpha*A*in + beta*out over the
- * first `cols` columns (blocks share n and ld). */
+ * first `cols` columns (blocks share n and ld; C128 blocks for a C128
+ * operator). */
 int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alpha, double beta,
                        int64_t cols);
 
@@ -156,7 +160,7 @@ int rcg_rotate(rcg_block V, rcg_block tmp, int64_t cols, const double* Y_host);
 int rcg_small_eigensystem(int64_t n, const double* H, double* evals, double* evecs);
 /* cuda_kernels::residuals (cuda_kernels.cu:307-382) with the apply at
  * RayleighChebyshevLMCuda.h:2235: W = A V, r_c = ||theta_c V_c - W_c|| for
- * c < count. */
+ * c < count (real theta; C128 blocks for a C128 operator). */
 int rcg_residuals(rcg_operator op, rcg_block V, rcg_block W, int64_t count, const double* theta,
                   double* res_host);
 /* LanczosMaxMinFinder::getMinMaxEigenvalues (LanczosMaxMinFinder.h:248-544)

@@ -56,10 +56,13 @@ struct CsrSched {
 };
 
 // out = alpha*A*X + beta*out over `ncols` columns (ld shared by X and out).
+// Complex operators (cx): val holds interleaved (re, im) pairs, the blocks
+// interleave each complex column as two real columns, and ncols counts real
+// columns.
 template <typename T>
 void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val, const T* X,
                       T* out, int ld, int ncols, T alpha, T beta, hipStream_t st,
-                      const CsrSched* sched = nullptr);
+                      const CsrSched* sched = nullptr, bool cx = false);
 
 // One three-term Chebyshev step of the scaled C polynomial:
 //   first: out = s * g2*(g1*A*X1 + X1)
@@ -69,7 +72,8 @@ void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val
 template <typename T>
 void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val, const T* X1,
                       const T* X0, T* out, int ld, int ncols, T g1, T g2, T s, bool first,
-                      hipStream_t st, int panel = 0, const CsrSched* sched = nullptr);
+                      hipStream_t st, int panel = 0, const CsrSched* sched = nullptr,
+                      bool cx = false);
 
 // Mixed-precision filter.  The FP64 filter output s*X_m of X_0 = v is split
 // per column as X_j = u_j(theta_c) v + E_j, u_j the scalar recurrence of the
@@ -80,7 +84,8 @@ void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val
 void launch_cheb_step_src(int64_t n, const int* rowptr, const int* col, const float* val,
                           const float* X1, const float* X0, float* out, const float* R,
                           const float* kap, int ld, int ncols, float g1, float g2, float s,
-                          hipStream_t st, int panel = 0, const CsrSched* sched = nullptr);
+                          hipStream_t st, int panel = 0, const CsrSched* sched = nullptr,
+                          bool cx = false);
 // E0, E1, R from V and AV (see the kernel); cp[c] = theta_c, cp[nc+c] = 1 for
 // Ritz columns, 0 for plain ones (E_j = X_j, R = 0).
 void launch_mixed_start(int64_t n, const double* V, const double* AV, int ld, int nc,
@@ -142,5 +147,19 @@ void launch_lz_apply_dot(int64_t n, const int* rowptr, const int* col, const dou
 void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* alpha_k,
                          double* beta_k, double* ws, size_t ws_elems, hipStream_t st);
 size_t lz_ws_elems(int64_t n);
+// complex (interleaved) v += A w; alpha_k = Re sum conj(w) v, alpha_im_k = Im.
+void launch_lz_apply_dot_c(int64_t n, const int* rowptr, const int* col, const double* val,
+                           const double* w, double* v, double* alpha_k, double* alpha_im_k,
+                           double* ws, size_t ws_elems, hipStream_t st);
+
+// ---- complex helpers
+// complex columns [c0, c0+cnt) of an interleaved block: (re, im) of row i,
+// column c0+j from draws counter + 2(j*n + i) + {0, 1}.
+void launch_randomize_cplx(int64_t n, double* X, int ld, int c0, int cnt, uint64_t seed,
+                           uint64_t counter, hipStream_t st);
+// s[2c] = s[2c+1] = s[2c] + s[2c+1], c < nc.
+void launch_pair_sums(double* s, int nc, hipStream_t st);
+// real Gram of interleaved blocks (2f x 2b) -> real embedding of Y^H V, in place.
+void launch_herm_embed(double* G, int f, int b, hipStream_t st);
 
 }  // namespace rcg

@@ -122,7 +122,9 @@ constexpr int kStageNz = 1024;
 // pointers (and, when they fit, its column indices and values) are staged in
 // LDS by one coalesced load, and a row's own-row operands are loaded before
 // its gathers, so that computing a row costs a single trip to memory.
-template <typename T, int G, int NV, int MODE, int RPI>
+// CX: complex operator and block (interleaved re, im): each 16-byte vector
+// holds W/2 complex entries and the gather multiplies complex values.
+template <typename T, int G, int NV, int MODE, int RPI, bool CX>
 __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
   constexpr int RS = kThreads / G;
   __shared__ int s_rp[kMaxRpb + 1];
   __shared__ int s_col[kStageNz];
-  __shared__ T s_val[kStageNz];
+  __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
   const int64_t lq = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
   if (lq >= nblk) return;  // uniform across the workgroup
   const int64_t lb = order ? (int64_t)order[lq] : lq;
@@ -150,7 +152,12 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
   if (staged)
     for (int t = threadIdx.x; t < nz; t += kThreads) {
       s_col[t] = col[base + t];
-      s_val[t] = val[base + t];
+      if constexpr (CX) {
+        s_val[2 * t] = val[2 * (int64_t)(base + t)];
+        s_val[2 * t + 1] = val[2 * (int64_t)(base + t) + 1];
+      } else {
+        s_val[t] = val[base + t];
+      }
     }
   __syncthreads();
   // RPI rows per lane iteration: rows li and li + RS, ... so that each wave
@@ -198,19 +205,31 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     // rows) is issued before its FMAs
     for (int c = 0; c < maxlen; c += 4) {
       int jj[RPI][4];
-      T aa[RPI][4];
+      T aa[RPI][4], ai[RPI][4];
 #pragma unroll
       for (int q = 0; q < RPI; ++q)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int p = pb[q] + c + e;
           const bool live = p < pe[q];
-          if (staged) {
-            jj[q][e] = live ? s_col[p - base] : 0;
-            aa[q][e] = live ? s_val[p - base] : T(0);
+          if constexpr (CX) {
+            if (staged) {
+              jj[q][e] = live ? s_col[p - base] : 0;
+              aa[q][e] = live ? s_val[2 * (p - base)] : T(0);
+              ai[q][e] = live ? s_val[2 * (p - base) + 1] : T(0);
+            } else {
+              jj[q][e] = live ? col[p] : 0;
+              aa[q][e] = live ? val[2 * (int64_t)p] : T(0);
+              ai[q][e] = live ? val[2 * (int64_t)p + 1] : T(0);
+            }
           } else {
-            jj[q][e] = live ? col[p] : 0;
-            aa[q][e] = live ? val[p] : T(0);
+            if (staged) {
+              jj[q][e] = live ? s_col[p - base] : 0;
+              aa[q][e] = live ? s_val[p - base] : T(0);
+            } else {
+              jj[q][e] = live ? col[p] : 0;
+              aa[q][e] = live ? val[p] : T(0);
+            }
           }
         }
       T u[RPI][4][NV][W];
@@ -233,9 +252,20 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
         for (int e = 0; e < 4; ++e) {
           if (pb[q] + c + e < pe[q]) {
 #pragma unroll
-            for (int k = 0; k < NV; ++k)
+            for (int k = 0; k < NV; ++k) {
+              if constexpr (CX) {
 #pragma unroll
-              for (int w = 0; w < W; ++w) acc[q][k][w] = fma(aa[q][e], u[q][e][k][w], acc[q][k][w]);
+                for (int w = 0; w < W; w += 2) {
+                  acc[q][k][w] = fma(aa[q][e], u[q][e][k][w], acc[q][k][w]);
+                  acc[q][k][w] = fma(-ai[q][e], u[q][e][k][w + 1], acc[q][k][w]);
+                  acc[q][k][w + 1] = fma(aa[q][e], u[q][e][k][w + 1], acc[q][k][w + 1]);
+                  acc[q][k][w + 1] = fma(ai[q][e], u[q][e][k][w], acc[q][k][w + 1]);
+                }
+              } else {
+#pragma unroll
+                for (int w = 0; w < W; ++w) acc[q][k][w] = fma(aa[q][e], u[q][e][k][w], acc[q][k][w]);
+              }
+            }
           }
         }
     }
@@ -280,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
 #define RCG_CSR_GDIV 1
 #endif
 
-template <typename T, int MODE, int G, int NV>
+template <typename T, int MODE, bool CX, int G, int NV>
 void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
                   T* out, int ld, int ncv, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
                   const T* R, const T* kap) {
@@ -304,21 +334,21 @@ void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T*
   if (rows_per_blk > kMaxRpb) throw Error(1, "CSR row block exceeds the staged row-pointer range");
   const int remap = sc ? sc->remap : 1;
   const int64_t grid = round_up(nblk, kXcds);
-  hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE, RCG_CSR_RPI>), dim3((unsigned)grid), dim3(kThreads), 0,
+  hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE, RCG_CSR_RPI, CX>), dim3((unsigned)grid), dim3(kThreads), 0,
                      st, n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, rows_per_blk, nblk, order,
                      remap, R, kap);
   RCG_HIP(hipGetLastError());
 }
 
-template <typename T, int MODE>
-void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
-                T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
-                const T* R = nullptr, const T* kap = nullptr) {
+template <typename T, int MODE, bool CX>
+void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
+                   T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
+                   const T* R, const T* kap) {
   constexpr int W = V16<T>::W;
   const int ncv = (ncols + W - 1) / W;
   if (ld % 4 != 0) throw Error(1, "block leading dimension must be a multiple of 4");
 #define RCG_CSR_CASE(LIM, GG, NN) \
-  if (ncv <= LIM) return csr_launch_g<T, MODE, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st, sc, R, kap);
+  if (ncv <= LIM) return csr_launch_g<T, MODE, CX, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st, sc, R, kap);
   RCG_CSR_CASE(1, 1, 1)
   RCG_CSR_CASE(2, 2, 1)
   RCG_CSR_CASE(4, 4, 1)
@@ -330,6 +360,19 @@ void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X
   RCG_CSR_CASE(256, 64, 4)
 #undef RCG_CSR_CASE
   throw Error(6, "CSR block kernels support at most 256 16-byte column vectors per row");
+}
+
+// ncols counts real columns (2 per complex column when cx).
+template <typename T, int MODE>
+void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
+                T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
+                const T* R = nullptr, const T* kap = nullptr, bool cx = false) {
+  if (cx) {
+    if (ncols % 2 != 0) throw Error(1, "complex blocks have an even number of real columns");
+    csr_launch_cx<T, MODE, true>(n, rp, ci, va, X, Z, out, ld, ncols, p0, p1, p2, st, sc, R, kap);
+  } else {
+    csr_launch_cx<T, MODE, false>(n, rp, ci, va, X, Z, out, ld, ncols, p0, p1, p2, st, sc, R, kap);
+  }
 }
 
 // ------------------------------------------------------ tall-skinny Gram
@@ -812,6 +855,55 @@ __global__ __launch_bounds__(kThreads) void to_f32_kernel(int64_t n, const doubl
   if (i < n) y[i] = float(x[i]);
 }
 
+// ------------------------------------------------------- complex helpers
+// Complex block columns are pairs of real columns (re, im).  Start vectors:
+// complex column c0+j, row i takes draws counter + 2(j*n + i) + {0 re, 1 im}.
+__global__ __launch_bounds__(kThreads) void randomize_cplx_kernel(int64_t n, double* X, int ld,
+                                                                  int c0, int cnt, uint64_t key,
+                                                                  uint64_t counter) {
+  const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+  const int64_t i = e / (2 * cnt);
+  const int r = (int)(e % (2 * cnt));
+  if (i >= n) return;
+  const int j = r >> 1;
+  const int part = r & 1;
+  X[i * ld + 2 * (c0 + j) + part] =
+      sym_draw(key, counter + 2 * ((uint64_t)j * (uint64_t)n + (uint64_t)i) + (uint64_t)part);
+}
+
+// s[2c] = s[2c+1] = s[2c] + s[2c+1], c < nc: squared norms of complex columns
+// from those of their real and imaginary parts.
+__global__ void pair_sums_kernel(double* s, int nc) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double t = s[2 * c] + s[2 * c + 1];
+  s[2 * c] = t;
+  s[2 * c + 1] = t;
+}
+
+// In place, per 2x2 block: the real Gram G~ = Y~^T V~ of interleaved complex
+// blocks (2f x 2b, column-major, ld 2f) becomes the real embedding of
+// C = Y^H V, so that V~ C~ is the interleaved form of V C:
+//   C = (G~[2a,2c] + G~[2a+1,2c+1]) + i (G~[2a,2c+1] - G~[2a+1,2c]),
+//   C~[2a,2c] = C~[2a+1,2c+1] = Re C, C~[2a,2c+1] = Im C, C~[2a+1,2c] = -Im C.
+__global__ void herm_embed_kernel(double* G, int f, int b) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= f * b) return;
+  const int a = e % f;
+  const int c = e / f;
+  const int64_t ld = 2 * f;
+  double* g00 = G + 2 * a + (int64_t)(2 * c) * ld;
+  double* g10 = g00 + 1;
+  double* g01 = G + 2 * a + (int64_t)(2 * c + 1) * ld;
+  double* g11 = g01 + 1;
+  const double re = *g00 + *g11;
+  const double im = *g01 - *g10;
+  *g00 = re;
+  *g11 = re;
+  *g01 = im;
+  *g10 = -im;
+}
+
 // ----------------------------------------------------------------- Lanczos
 constexpr int kLzRows = 2048;  // rows per partial block
 
@@ -886,6 +978,38 @@ __global__ __launch_bounds__(kThreads) void lz_apply_dot_kernel(int64_t n,
   s = block_sum(s, red);
   if (threadIdx.x == 0) P[blockIdx.x] = s;
 }
+// complex: v += A w (interleaved re, im), partials of Re and Im of sum conj(w) v
+__global__ __launch_bounds__(kThreads) void lz_apply_dot_c_kernel(
+    int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const double* __restrict__ val, const double* __restrict__ w, double* v, double* P,
+    int64_t nblk) {
+  __shared__ double red[kThreads];
+  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
+  const int64_t i1 = min(n, i0 + kLzRows);
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    double ar = 0.0, ai = 0.0;
+    for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const double vr = val[2 * (int64_t)p], vi = val[2 * (int64_t)p + 1];
+      const double xr = w[2 * (int64_t)col[p]], xi = w[2 * (int64_t)col[p] + 1];
+      ar = fma(vr, xr, fma(-vi, xi, ar));
+      ai = fma(vr, xi, fma(vi, xr, ai));
+    }
+    const double nr = v[2 * i] + ar, ni = v[2 * i + 1] + ai;
+    v[2 * i] = nr;
+    v[2 * i + 1] = ni;
+    const double wr = w[2 * i], wi = w[2 * i + 1];
+    sr = fma(wr, nr, fma(wi, ni, sr));
+    si = fma(wr, ni, fma(-wi, nr, si));
+  }
+  sr = block_sum(sr, red);
+  si = block_sum(si, red);
+  if (threadIdx.x == 0) {
+    P[blockIdx.x] = sr;
+    P[nblk + blockIdx.x] = si;
+  }
+}
+
 // v += (-alpha) w; partial |v|^2
 __global__ __launch_bounds__(kThreads) void lz_axpy_norm_kernel(int64_t n, const double* w,
                                                                 double* v, const double* alpha,
@@ -914,18 +1038,20 @@ inline unsigned grid_for(int64_t work) {
 template <typename T>
 void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val, const T* X,
                       T* out, int ld, int ncols, T alpha, T beta, hipStream_t st,
-                      const CsrSched* sc) {
+                      const CsrSched* sc, bool cx) {
   if (n <= 0 || ncols <= 0) return;
   if (beta == T(0))
-    csr_launch<T, kApply>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, T(0), T(0), st, sc);
+    csr_launch<T, kApply>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, T(0), T(0), st, sc,
+                          nullptr, nullptr, cx);
   else
-    csr_launch<T, kApplyBeta>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, beta, T(0), st, sc);
+    csr_launch<T, kApplyBeta>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, beta, T(0), st,
+                              sc, nullptr, nullptr, cx);
 }
 
 template <typename T>
 void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val, const T* X1,
                       const T* X0, T* out, int ld, int ncols, T g1, T g2, T s, bool first,
-                      hipStream_t st, int panel, const CsrSched* sc) {
+                      hipStream_t st, int panel, const CsrSched* sc, bool cx) {
   if (n <= 0 || ncols <= 0) return;
   // Column panels: each launch streams the CSR arrays once and touches only
   // `panel` columns of every row, which shrinks the per-plane footprint of
@@ -936,24 +1062,24 @@ void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val
     const int pc = std::min(panel, ncols - c0);
     if (first)
       csr_launch<T, kChebFirst>(n, rowptr, col, val, X1 + c0, nullptr, out + c0, ld, pc, g1, g2, s,
-                                st, sc);
+                                st, sc, nullptr, nullptr, cx);
     else
       csr_launch<T, kCheb>(n, rowptr, col, val, X1 + c0, X0 + c0, out + c0, ld, pc, g1, g2, s, st,
-                           sc);
+                           sc, nullptr, nullptr, cx);
   }
 }
 
 void launch_cheb_step_src(int64_t n, const int* rowptr, const int* col, const float* val,
                           const float* X1, const float* X0, float* out, const float* R,
                           const float* kap, int ld, int ncols, float g1, float g2, float s,
-                          hipStream_t st, int panel, const CsrSched* sc) {
+                          hipStream_t st, int panel, const CsrSched* sc, bool cx) {
   if (n <= 0 || ncols <= 0) return;
   if (panel <= 0 || panel >= ncols) panel = ncols;
   if (panel % 4 != 0) panel = round_up(panel, 4);
   for (int c0 = 0; c0 < ncols; c0 += panel) {
     const int pc = std::min(panel, ncols - c0);
     csr_launch<float, kChebSrc>(n, rowptr, col, val, X1 + c0, X0 + c0, out + c0, ld, pc, g1, g2, s,
-                                st, sc, R + c0, kap + c0);
+                                st, sc, R + c0, kap + c0, cx);
   }
 }
 
@@ -1122,7 +1248,47 @@ void launch_fill_cols(int64_t n, T* X, int ld, int c0, int cnt, T value, hipStre
   RCG_HIP(hipGetLastError());
 }
 
-size_t lz_ws_elems(int64_t n) { return (size_t)((n + kLzRows - 1) / kLzRows) + 4; }
+size_t lz_ws_elems(int64_t n) { return 2 * (size_t)((n + kLzRows - 1) / kLzRows) + 4; }
+
+void launch_randomize_cplx(int64_t n, double* X, int ld, int c0, int cnt, uint64_t seed,
+                           uint64_t counter, hipStream_t st) {
+  if (n <= 0 || cnt <= 0) return;
+  const uint64_t key = [&] {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }();
+  hipLaunchKernelGGL(randomize_cplx_kernel, dim3(grid_for(n * 2 * cnt)), dim3(kThreads), 0, st, n, X,
+                     ld, c0, cnt, key, counter);
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_pair_sums(double* s, int nc, hipStream_t st) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(pair_sums_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, s, nc);
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_herm_embed(double* G, int f, int b, hipStream_t st) {
+  if (f <= 0 || b <= 0) return;
+  hipLaunchKernelGGL(herm_embed_kernel, dim3((f * b + 255) / 256), dim3(256), 0, st, G, f, b);
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_lz_apply_dot_c(int64_t n, const int* rowptr, const int* col, const double* val,
+                           const double* w, double* v, double* alpha_k, double* alpha_im_k,
+                           double* ws, size_t ws_elems, hipStream_t st) {
+  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  if ((size_t)(2 * nblk) > ws_elems) throw Error(3, "Lanczos workspace too small");
+  hipLaunchKernelGGL(lz_apply_dot_c_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, rowptr,
+                     col, val, w, v, ws, nblk);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, alpha_k, 0);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws + nblk, nblk, alpha_im_k, 0);
+  RCG_HIP(hipGetLastError());
+}
+
+
 
 void launch_lz_start(int64_t n, double* v, double* w, double* ws, size_t ws_elems,
                      double* scratch, hipStream_t st) {
@@ -1160,10 +1326,10 @@ void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* al
 // ------------------------------------------------------- instantiations
 #define RCG_INST(T)                                                                              \
   template void launch_csr_apply<T>(int64_t, const int*, const int*, const T*, const T*, T*, int, \
-                                    int, T, T, hipStream_t, const CsrSched*);                     \
+                                    int, T, T, hipStream_t, const CsrSched*, bool);               \
   template void launch_cheb_step<T>(int64_t, const int*, const int*, const T*, const T*,          \
                                     const T*, T*, int, int, T, T, T, bool, hipStream_t, int,      \
-                                    const CsrSched*);                                             \
+                                    const CsrSched*, bool);                                       \
   template size_t gram_ws_elems<T>(int64_t, int, int);                                           \
   template void launch_gram<T>(int64_t, const T*, int, int, const T*, int, int, T*, T*, size_t,   \
                                hipStream_t);                                                      \

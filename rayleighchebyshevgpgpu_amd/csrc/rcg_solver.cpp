@@ -14,12 +14,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/rcg_c_api.h"
@@ -28,6 +30,10 @@
 #include "rcg_internal.hpp"
 
 using rcg::Error;
+using cplx = std::complex<double>;
+
+inline double conj_s(double x) { return x; }
+inline cplx conj_s(cplx x) { return std::conj(x); }
 
 // ------------------------------------------------------------------ memory
 struct DevBuf {
@@ -172,9 +178,16 @@ double wall() {
 // W (A V, ld) and Y (found vectors, ldy).
 class Engine {
  public:
+  // Complex operators (dtype C128): every block column is a complex column
+  // stored as two interleaved real columns; ld and the attached buffers are
+  // in real elements, and widths passed to the methods count complex columns.
   Engine(rcg_context_s* ctx, rcg_operator_s* op, int64_t n) : ctx_(ctx), op_(op), n_(n) {
     st_ = ctx->stream;
+    cx_ = op && op->dtype == RCG_C128;
+    cw_ = cx_ ? 2 : 1;
   }
+  bool cx() const { return cx_; }
+  int cw() const { return cw_; }
 
   // Size the context's workspaces for widths up to bmax and kmax.
   void reserve(int64_t bmax, int64_t kmax) {
@@ -323,11 +336,22 @@ class Engine {
     }
     return t;
   }
+  // b counts real columns here.
   rcg_operator_s::Tuned tuned_for(int64_t b) {
-    return tuned_for(b, 0, [&](int panel, const rcg::CsrSched* sc) {
+    return tuned_for(b, cx_ ? 2 : 0, [&](int panel, const rcg::CsrSched* sc) {
       rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, V(), T(), T(), (int)ld_,
-                                    (int)b, 0.0, 1.0, 0.0, false, st_, panel, sc);
+                                    (int)b, 0.0, 1.0, 0.0, false, st_, panel, sc, cx_);
     });
+  }
+  // W[:, :b] = A V[:, :b] (b complex columns when complex).
+  void apply(const double* X, double* out, int64_t b) {
+    rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, X, out, (int)ld_,
+                                  (int)(cw_ * b), 1.0, 0.0, st_, sched_for(cw_ * b), cx_);
+  }
+  void copy_cols(const double* src, int64_t lds, int64_t s0, double* dst, int64_t ldd, int64_t d0,
+                 int64_t cnt) {
+    rcg::launch_copy_cols<double>(n_, src, (int)lds, (int)(cw_ * s0), dst, (int)ldd,
+                                  (int)(cw_ * d0), (int)(cw_ * cnt), st_);
   }
   const rcg::CsrSched* sched_for(int64_t b) {
     if (!op_ || n_ * b < (int64_t)1 << 22) return nullptr;
@@ -338,24 +362,27 @@ class Engine {
   // X1 = g2(g1 A X0 + X0), Xj = g2(g1 A X_{j-1} + X_{j-1}) - X_{j-2}, out = Xm/(m+1).
   void filter(int64_t b, long m, long rep, double lmax, double shift) {
     const auto c = rcg::cpoly::coefficients(m, lmax, shift);
-    const auto tn = tuned_for(b);
+    const int64_t rb = cw_ * b;  // real columns
+    const auto tn = tuned_for(rb);
     const rcg::CsrSched* sc = &op_->scheds[tn.sched]->sc;
     last_panel_ = tn.panel;
     last_sched_ = op_->scheds[tn.sched]->name;
-    const double csr = double(op_->nnz) * 12.0 + 4.0 * double(n_ + 1);
-    const double blk = 8.0 * double(n_) * double(b);
-    step_begin(rep * m * ((b + tn.panel - 1) / tn.panel),
+    const double csr = double(op_->nnz) * (4.0 + 8.0 * cw_) + 4.0 * double(n_ + 1);
+    const double blk = 8.0 * double(n_) * double(rb);
+    step_begin(rep * m * ((rb + tn.panel - 1) / tn.panel),
                double(rep) * (csr + 2.0 * blk + double(m - 1) * (csr + 3.0 * blk)));
     for (long r = 0; r < rep; ++r) {
       double* X0 = V();
       double* X1 = T();
       const double s1 = (m == 1) ? 0.5 : 1.0;
       rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, X0, nullptr, X1,
-                                    (int)ld_, (int)b, c.g1, c.g2, s1, true, st_, tn.panel, sc);
+                                    (int)ld_, (int)rb, c.g1, c.g2, s1, true, st_, tn.panel, sc,
+                                    cx_);
       for (long j = 2; j <= m; ++j) {
         const double s = (j == m) ? 1.0 / double(m + 1) : 1.0;
         rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, X1, X0, X0,
-                                      (int)ld_, (int)b, c.g1, c.g2, s, false, st_, tn.panel, sc);
+                                      (int)ld_, (int)rb, c.g1, c.g2, s, false, st_, tn.panel, sc,
+                                      cx_);
         std::swap(X0, X1);
       }
       if (X1 != V()) swapVT();
@@ -367,16 +394,24 @@ class Engine {
   // launch_cheb_step_src): V <- P_m(A')^rep V over b columns with FP32 steps
   // and FP32 CSR values.  theta[c] / ritz[c]: the Ritz value of column c and
   // whether column c is a Ritz vector (others are filtered directly in FP32).
-  // Needs m >= 2 and the FP32 buffers attached with attach_mixed.
+  // Needs m >= 2 and the FP32 buffers attached with attach_mixed.  Complex
+  // columns run as their two real columns with the same theta.
   void filter_mixed(int64_t b, long m, long rep, double lmax, double shift,
-                    const std::vector<double>& theta, const std::vector<char>& ritz) {
+                    const std::vector<double>& theta_c, const std::vector<char>& ritz_c) {
     if (m < 2) return filter(b, m, rep, lmax, shift);
     if (!E_[0]) throw Error(RCG_ERR_INVALID, "mixed-precision buffers are not attached");
+    const int64_t rb = cw_ * b;
+    std::vector<double> theta(rb);
+    std::vector<char> ritz(rb);
+    for (int64_t c = 0; c < rb; ++c) {
+      theta[c] = theta_c[c / cw_];
+      ritz[c] = ritz_c[c / cw_];
+    }
     const auto c = rcg::cpoly::coefficients(m, lmax, shift);
     const double g12 = c.g1 * c.g2;
-    // coefficient table: cp (theta, flag) [2b doubles], um [ld doubles],
+    // coefficient table: cp (theta, flag) [2 rb doubles], um [ld doubles],
     // kap rows j = 0..m [ld floats each]
-    const size_t nd = 2 * (size_t)b + (size_t)ld_;
+    const size_t nd = 2 * (size_t)rb + (size_t)ld_;
     const size_t bytes = sizeof(double) * nd + sizeof(float) * (size_t)(m + 1) * (size_t)ld_;
     sync();  // the pinned table of the previous call is no longer in flight
     coef_.ensure(bytes);
@@ -385,10 +420,10 @@ class Engine {
     float* hk = reinterpret_cast<float*>(hd + nd);
     std::fill(hd, hd + nd, 0.0);
     std::fill(hk, hk + (size_t)(m + 1) * (size_t)ld_, 0.0f);
-    for (int64_t col = 0; col < b; ++col) {
+    for (int64_t col = 0; col < rb; ++col) {
       if (!ritz[col]) continue;
       hd[col] = theta[col];
-      hd[b + col] = 1.0;
+      hd[rb + col] = 1.0;
       const double l = c.g2 * (1.0 + c.g1 * theta[col]);
       double u0 = 1.0, u1 = l;  // u_{j-2}, u_{j-1}
       for (long j = 2; j <= m; ++j) {
@@ -397,39 +432,39 @@ class Engine {
         u0 = u1;
         u1 = u;
       }
-      hd[2 * b + col] = u1;  // u_m
+      hd[2 * rb + col] = u1;  // u_m
     }
     RCG_HIP(hipMemcpyAsync(coef_.p, hcoef_.p, bytes, hipMemcpyHostToDevice, st_));
     const double* dcp = coef_.as<double>();
-    const double* dum = dcp + 2 * b;
+    const double* dum = dcp + 2 * rb;
     const float* dk = reinterpret_cast<const float*>(dcp + nd);
     const float* val32 = values_f32();
     const float g1f = float(c.g1), g2f = float(c.g2);
-    const auto tn = tuned_for(b, 1, [&](int panel, const rcg::CsrSched* sc) {
+    const auto tn = tuned_for(rb, cx_ ? 3 : 1, [&](int panel, const rcg::CsrSched* sc) {
       rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, E_[0], E_[1], E_[1], R32_,
-                                dk + 2 * ld_, (int)ld_, (int)b, g1f, g2f, 1.0f, st_, panel, sc);
+                                dk + 2 * ld_, (int)ld_, (int)rb, g1f, g2f, 1.0f, st_, panel, sc,
+                                cx_);
     });
     const rcg::CsrSched* sc = &op_->scheds[tn.sched]->sc;
     last_panel_ = tn.panel;
     last_sched_ = op_->scheds[tn.sched]->name;
-    const double csr = double(op_->nnz) * 8.0 + 4.0 * double(n_ + 1);
-    const double blk = 4.0 * double(n_) * double(b);
+    const double csr = double(op_->nnz) * (4.0 + 4.0 * cw_) + 4.0 * double(n_ + 1);
+    const double blk = 4.0 * double(n_) * double(rb);
     for (long r = 0; r < rep; ++r) {
-      rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, V(), W_, (int)ld_, (int)b,
-                                    1.0, 0.0, st_, sched_for(b));
-      rcg::launch_mixed_start(n_, V(), W_, (int)ld_, (int)b, dcp, g12, c.g1, c.g2, E_[0], E_[1],
+      apply(V(), W_, b);
+      rcg::launch_mixed_start(n_, V(), W_, (int)ld_, (int)rb, dcp, g12, c.g1, c.g2, E_[0], E_[1],
                               R32_, st_);
       float* X0 = E_[0];
       float* X1 = E_[1];
-      step_begin((m - 1) * ((b + tn.panel - 1) / tn.panel), double(m - 1) * (csr + 4.0 * blk));
+      step_begin((m - 1) * ((rb + tn.panel - 1) / tn.panel), double(m - 1) * (csr + 4.0 * blk));
       for (long j = 2; j <= m; ++j) {
         rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, X1, X0, X0, R32_,
-                                  dk + (size_t)j * ld_, (int)ld_, (int)b, g1f, g2f, 1.0f, st_,
-                                  tn.panel, sc);
+                                  dk + (size_t)j * ld_, (int)ld_, (int)rb, g1f, g2f, 1.0f, st_,
+                                  tn.panel, sc, cx_);
         std::swap(X0, X1);
       }
       step_end();
-      rcg::launch_mixed_combine(n_, V(), (int)ld_, (int)b, dum, 1.0 / double(m + 1), X1, st_);
+      rcg::launch_mixed_combine(n_, V(), (int)ld_, (int)rb, dum, 1.0 / double(m + 1), X1, st_);
     }
   }
   void attach_mixed(float* E0, float* E1, float* R) {
@@ -439,8 +474,9 @@ class Engine {
   }
   const float* values_f32() {
     if (!op_->val32.p) {
-      op_->val32.ensure(sizeof(float) * (size_t)std::max<int64_t>(op_->nnz, 1));
-      rcg::launch_to_f32(op_->nnz, op_->val, op_->val32.as<float>(), st_);
+      const int64_t nv = cw_ * op_->nnz;
+      op_->val32.ensure(sizeof(float) * (size_t)std::max<int64_t>(nv, 1));
+      rcg::launch_to_f32(nv, op_->val, op_->val32.as<float>(), st_);
     }
     return op_->val32.as<float>();
   }
@@ -482,44 +518,75 @@ class Engine {
   int last_panel() const { return last_panel_; }
   const std::string& last_sched() const { return last_sched_; }
 
-  // host[bx x by] = X^H Y (column-major).
+  // host[bx x by] = X^H Y (column-major); widths in columns of S.  Complex:
+  // the real Gram of the interleaved blocks (2bx x 2by) is combined on the host.
+  template <class S>
   void gram_to_host(const double* X, int64_t ldx, int64_t bx, const double* Yp, int64_t ldy,
-                    int64_t by, double* host) {
-    rcg::launch_gram<double>(n_, X, (int)ldx, (int)bx, Yp, (int)ldy, (int)by, dmatA(), ws(),
+                    int64_t by, S* host) {
+    constexpr int w = std::is_same<S, cplx>::value ? 2 : 1;
+    const int64_t rx = w * bx, ry = w * by;
+    rcg::launch_gram<double>(n_, X, (int)ldx, (int)rx, Yp, (int)ldy, (int)ry, dmatA(), ws(),
                              ws_elems_, st_);
-    RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * bx * by, hipMemcpyDeviceToHost, st_));
+    RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * rx * ry, hipMemcpyDeviceToHost, st_));
     sync();
-    std::memcpy(host, hmatA(), sizeof(double) * bx * by);
+    const double* g = hmatA();
+    if constexpr (w == 1) {
+      std::memcpy(host, g, sizeof(double) * bx * by);
+    } else {
+      for (int64_t c = 0; c < by; ++c)
+        for (int64_t a = 0; a < bx; ++a) {
+          const double g00 = g[2 * a + 2 * c * rx], g10 = g[2 * a + 1 + 2 * c * rx];
+          const double g01 = g[2 * a + (2 * c + 1) * rx], g11 = g[2 * a + 1 + (2 * c + 1) * rx];
+          host[a + c * bx] = cplx(g00 + g11, g01 - g10);
+        }
+    }
   }
 
   // V <- V C, C host b x b column-major; the result lands in T, then swaps.
-  void right_multiply(int64_t b, const double* C) {
-    std::memcpy(hmatB(), C, sizeof(double) * b * b);
-    RCG_HIP(hipMemcpyAsync(dmatA(), hmatB(), sizeof(double) * b * b, hipMemcpyHostToDevice, st_));
-    rcg::launch_tall_gemm<double>(n_, V(), (int)ld_, (int)b, dmatA(), (int)b, (int)b, 1.0, 0.0,
+  // Complex C enters the real product through its 2b x 2b real embedding.
+  template <class S>
+  void right_multiply(int64_t b, const S* C) {
+    if constexpr (std::is_same<S, double>::value) {
+      std::memcpy(hmatB(), C, sizeof(double) * b * b);
+    } else {
+      const int64_t r = 2 * b;
+      double* e = hmatB();
+      for (int64_t c = 0; c < b; ++c)
+        for (int64_t k = 0; k < b; ++k) {
+          const cplx v = C[k + c * b];
+          e[2 * k + 2 * c * r] = v.real();
+          e[2 * k + 1 + 2 * c * r] = -v.imag();
+          e[2 * k + (2 * c + 1) * r] = v.imag();
+          e[2 * k + 1 + (2 * c + 1) * r] = v.real();
+        }
+    }
+    const int64_t rb = cw_ * b;
+    RCG_HIP(hipMemcpyAsync(dmatA(), hmatB(), sizeof(double) * rb * rb, hipMemcpyHostToDevice, st_));
+    rcg::launch_tall_gemm<double>(n_, V(), (int)ld_, (int)rb, dmatA(), (int)rb, (int)rb, 1.0, 0.0,
                                   nullptr, T(), (int)ld_, st_);
     swapVT();
   }
 
-  // One CholQR pass on V[:, :b]: G = V^T V scaled to unit diagonal,
-  // G = R^T R, V <- V D R^{-1}.  A failed factorisation retries with the
+  // One CholQR pass on V[:, :b]: G = V^H V scaled to unit diagonal,
+  // G = R^H R, V <- V D R^{-1}.  A failed factorisation retries with the
   // shift 11(nb + b(b+1)) u ||G|| (shifted CholeskyQR); returns false then.
+  template <class S>
   bool cholqr_pass(int64_t b, bool allow_shift) {
-    std::vector<double> G(b * b);
-    gram_to_host(V(), ld_, b, V(), ld_, b, G.data());
+    std::vector<S> G(b * b);
+    gram_to_host<S>(V(), ld_, b, V(), ld_, b, G.data());
     std::vector<double> d(b);
     for (int64_t j = 0; j < b; ++j) {
-      const double g = G[j + j * b];
+      const double g = std::real(G[j + j * b]);
       d[j] = (g > 0.0 && std::isfinite(g)) ? 1.0 / std::sqrt(g) : 1.0;
     }
     for (int64_t j = 0; j < b; ++j)
       for (int64_t i = 0; i < b; ++i) G[i + j * b] *= d[i] * d[j];
-    std::vector<double> R, Ri;
+    std::vector<S> R, Ri;
     bool clean = rcg::dense::cholesky_upper(G.data(), b, R);
     if (!clean) {
       if (!allow_shift) throw Error(RCG_ERR_NUMERIC, "CholQR breakdown after a shifted pass");
       double gn = 0.0;
-      for (double x : G) gn += x * x;
+      for (const S& x : G) gn += std::norm(x);
       const double s = 11.0 * (double(n_) * b + double(b) * (b + 1)) * 1.1102230246251565e-16 * std::sqrt(gn);
       for (int64_t j = 0; j < b; ++j) G[j + j * b] += s;
       if (!rcg::dense::cholesky_upper(G.data(), b, R))
@@ -528,7 +595,7 @@ class Engine {
     rcg::dense::upper_inverse(R, b, Ri);
     for (int64_t j = 0; j < b; ++j)
       for (int64_t i = 0; i < b; ++i) Ri[i + j * b] *= d[i];
-    right_multiply(b, Ri.data());
+    right_multiply<S>(b, Ri.data());
     return clean;
   }
 
@@ -536,45 +603,57 @@ class Engine {
   // pass when the first needed a shift.
   void orthonormalize(int64_t b) {
     if (b <= 0) return;
-    const bool clean = cholqr_pass(b, true);
-    cholqr_pass(b, true);
-    if (!clean) cholqr_pass(b, false);
+    if (cx_) return orthonormalize_t<cplx>(b);
+    orthonormalize_t<double>(b);
+  }
+  template <class S>
+  void orthonormalize_t(int64_t b) {
+    const bool clean = cholqr_pass<S>(b, true);
+    cholqr_pass<S>(b, true);
+    if (!clean) cholqr_pass<S>(b, false);
   }
 
   // OrthogonalizeAtoB (RayleighChebyshevLM.h:1844-1917) against Y[:, :found]:
-  // V -= Y (Y^T V), twice.  The normalisation of the reference is implied by
+  // V -= Y (Y^H V), twice.  The normalisation of the reference is implied by
   // the CholQR that follows in the solve.
   void deflate(int64_t b, int64_t found) {
     if (found <= 0 || b <= 0) return;
+    const int64_t rf = cw_ * found, rb = cw_ * b;
     for (int pass = 0; pass < 2; ++pass) {
-      rcg::launch_gram<double>(n_, Y_, (int)ldy_, (int)found, V(), (int)ld_, (int)b, dmatA(),
-                               ws(), ws_elems_, st_);
-      rcg::launch_tall_gemm<double>(n_, Y_, (int)ldy_, (int)found, dmatA(), (int)found, (int)b,
-                                    -1.0, 1.0, V(), V(), (int)ld_, st_);
+      rcg::launch_gram<double>(n_, Y_, (int)ldy_, (int)rf, V(), (int)ld_, (int)rb, dmatA(), ws(),
+                               ws_elems_, st_);
+      if (cx_) rcg::launch_herm_embed(dmatA(), (int)found, (int)b, st_);
+      rcg::launch_tall_gemm<double>(n_, Y_, (int)ldy_, (int)rf, dmatA(), (int)rf, (int)rb, -1.0,
+                                    1.0, V(), V(), (int)ld_, st_);
     }
   }
 
   void normalize(int64_t b) {
-    rcg::launch_col_sumsq<double>(n_, V(), (int)ld_, (int)b, dsums(), ws(), ws_elems_, st_);
-    rcg::launch_normalize_cols<double>(n_, V(), (int)ld_, (int)b, dsums(), st_);
+    const int64_t rb = cw_ * b;
+    rcg::launch_col_sumsq<double>(n_, V(), (int)ld_, (int)rb, dsums(), ws(), ws_elems_, st_);
+    if (cx_) rcg::launch_pair_sums(dsums(), (int)b, st_);
+    rcg::launch_normalize_cols<double>(n_, V(), (int)ld_, (int)rb, dsums(), st_);
   }
 
-  // formVtAV (RayleighChebyshevLM.h:1928-1969): W = A V, H(j,i) = V_j . W_i
-  // for j >= i, mirrored.
-  void project(int64_t b, std::vector<double>& H) {
-    rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, V(), W_, (int)ld_, (int)b,
-                                  1.0, 0.0, st_, sched_for(b));
-    H.assign(b * b, 0.0);
-    gram_to_host(V(), ld_, b, W_, ld_, b, H.data());
-    for (int64_t i = 0; i < b; ++i)
-      for (int64_t j = i + 1; j < b; ++j) H[i + j * b] = H[j + i * b];
+  // formVtAV (RayleighChebyshevLM.h:1928-1969): W = A V, H(j,i) = V_j^H W_i
+  // for j >= i, mirrored (conjugated).
+  template <class S>
+  void project(int64_t b, std::vector<S>& H) {
+    apply(V(), W_, b);
+    H.assign(b * b, S(0.0));
+    gram_to_host<S>(V(), ld_, b, W_, ld_, b, H.data());
+    for (int64_t i = 0; i < b; ++i) {
+      H[i + i * b] = std::real(H[i + i * b]);
+      for (int64_t j = i + 1; j < b; ++j) H[i + j * b] = conj_s(H[j + i * b]);
+    }
   }
 
   // createEigenVectorsAndResiduals (RayleighChebyshevLM.h:1984-2087):
   // V <- normalize(V Y); W = A V; r_c = ||theta_c V_c - W_c||, c < count.
-  void rotate_residuals(int64_t b, const std::vector<double>& Yh, const std::vector<double>& th,
+  template <class S>
+  void rotate_residuals(int64_t b, const std::vector<S>& Yh, const std::vector<double>& th,
                         int64_t count, std::vector<double>& res) {
-    right_multiply(b, Yh.data());
+    right_multiply<S>(b, Yh.data());
     normalize(b);
     residuals(count, th.data(), res);
   }
@@ -582,20 +661,29 @@ class Engine {
   void residuals(int64_t count, const double* th, std::vector<double>& res) {
     res.assign(count, 0.0);
     if (count <= 0) return;
-    rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, V(), W_, (int)ld_,
-                                  (int)count, 1.0, 0.0, st_, sched_for(count));
-    std::memcpy(hvecUp(), th, sizeof(double) * count);
-    RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * count, hipMemcpyHostToDevice, st_));
-    rcg::launch_residual_sumsq<double>(n_, V(), W_, (int)ld_, (int)count, dtheta(), dsums(), ws(),
+    apply(V(), W_, count);
+    const int64_t rc = cw_ * count;
+    for (int64_t c = 0; c < rc; ++c) hvecUp()[c] = th[c / cw_];
+    RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * rc, hipMemcpyHostToDevice, st_));
+    rcg::launch_residual_sumsq<double>(n_, V(), W_, (int)ld_, (int)rc, dtheta(), dsums(), ws(),
                                        ws_elems_, st_);
-    RCG_HIP(hipMemcpyAsync(hvecDown(), dsums(), sizeof(double) * count, hipMemcpyDeviceToHost, st_));
+    RCG_HIP(hipMemcpyAsync(hvecDown(), dsums(), sizeof(double) * rc, hipMemcpyDeviceToHost, st_));
     sync();
-    for (int64_t c = 0; c < count; ++c) res[c] = std::sqrt(std::abs(hvecDown()[c]));
+    for (int64_t c = 0; c < count; ++c) {
+      double t = 0.0;
+      for (int e = 0; e < cw_; ++e) t += hvecDown()[cw_ * c + e];
+      res[c] = std::sqrt(std::abs(t));
+    }
   }
 
   // ArandomizeOpClassLMCuda::randomize semantics: column j of the range
-  // takes draws counter + j*n + i.
+  // takes draws counter + j*n + i (complex: counter + 2(j*n + i) + {0, 1}).
   void randomize(int64_t c0, int64_t cnt, uint64_t seed, uint64_t* counter) {
+    if (cx_) {
+      rcg::launch_randomize_cplx(n_, V(), (int)ld_, (int)c0, (int)cnt, seed, *counter, st_);
+      *counter += 2 * (uint64_t)cnt * (uint64_t)n_;
+      return;
+    }
     rcg::launch_randomize_cols<double>(n_, V(), (int)ld_, (int)c0, (int)cnt, seed, *counter, st_);
     *counter += (uint64_t)cnt * (uint64_t)n_;
   }
@@ -611,17 +699,21 @@ class Engine {
     };
     if (errorTolerance > 0.01) errorTolerance = 0.01;
     const double tol = errorTolerance < smallTol ? smallTol : errorTolerance;
+    // complex vectors are 2n interleaved reals: the start draws, norms,
+    // scalings and the real-alpha axpy act on them as real vectors
+    const int64_t nr = cw_ * n_;
     DevBuf vecs, ab;
-    vecs.ensure(sizeof(double) * (size_t)n_ * 2);
-    ab.ensure(sizeof(double) * (size_t)(2 * itmax + 8));
+    vecs.ensure(sizeof(double) * (size_t)nr * 2);
+    ab.ensure(sizeof(double) * (size_t)(3 * itmax + 8));
     double* v = vecs.as<double>();
-    double* w = v + n_;
+    double* w = v + nr;
     double* alpha = ab.as<double>();
     double* beta = alpha + itmax;
-    double* scratch = beta + itmax;
-    rcg::launch_randomize_cols<double>(n_, v, 1, 0, 1, seed, *counter, st_);
-    *counter += (uint64_t)n_;
-    rcg::launch_lz_start(n_, v, w, ws(), ws_elems_, scratch, st_);
+    double* alpha_im = beta + itmax;
+    double* scratch = alpha_im + itmax;
+    rcg::launch_randomize_cols<double>(nr, v, 1, 0, 1, seed, *counter, st_);
+    *counter += (uint64_t)nr;
+    rcg::launch_lz_start(nr, v, w, ws(), ws_elems_, scratch, st_);
     std::vector<double> ha(itmax), hb(itmax);
     double minDiff = 1.0e10, maxDiff = 1.0e10, lmin = 0.0, lmax = 0.0;
     double rfMax = 9.0, rfMin = 9.0, rateMax = 1.0, rateMin = 1.0;
@@ -630,18 +722,28 @@ class Engine {
     long k = 0;
     bool done = false;
     while (k < itmax && !done) {
-      if (k != 0) rcg::launch_lz_swap(n_, v, w, beta + (k - 1), st_);
-      rcg::launch_lz_apply_dot(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k, ws(),
-                               ws_elems_, st_);
-      rcg::launch_lz_axpy_norm(n_, w, v, alpha + k, beta + k, ws(), ws_elems_, st_);
+      if (k != 0) rcg::launch_lz_swap(nr, v, w, beta + (k - 1), st_);
+      if (cx_)
+        rcg::launch_lz_apply_dot_c(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k,
+                                   alpha_im + k, ws(), ws_elems_, st_);
+      else
+        rcg::launch_lz_apply_dot(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k, ws(),
+                                 ws_elems_, st_);
+      rcg::launch_lz_axpy_norm(nr, w, v, alpha + k, beta + k, ws(), ws_elems_, st_);
       k += 1;
       if (k % 5 == 0) {
         RCG_HIP(hipMemcpyAsync(hlz(), alpha + (k - 5), sizeof(double) * 5, hipMemcpyDeviceToHost, st_));
         RCG_HIP(hipMemcpyAsync(hlz() + 8, beta + (k - 5), sizeof(double) * 5, hipMemcpyDeviceToHost, st_));
+        if (cx_)
+          RCG_HIP(hipMemcpyAsync(hlz() + 16, alpha_im + (k - 5), sizeof(double) * 5,
+                                 hipMemcpyDeviceToHost, st_));
         sync();
         for (int q = 0; q < 5; ++q) {
           ha[k - 5 + q] = hlz()[q];
           hb[k - 5 + q] = hlz()[8 + q];
+          // LanczosMaxMinFinder.h:350-356: a non-real <w, Aw> means A is not Hermitian
+          if (cx_ && std::abs(hlz()[16 + q]) > (1.0 + std::abs(hlz()[q])) * 1.0e-12)
+            throw Error(RCG_ERR_NOT_HERMITIAN, "Lanczos max/min finder: operator is not Hermitian");
         }
         std::vector<double> D(ha.begin(), ha.begin() + k), E(hb.begin(), hb.begin() + (k - 1));
         lmax = rcg::dense::tridiag_eigenvalue(D, E, k - 1, 1.0e-14);
@@ -705,6 +807,8 @@ class Engine {
   double* Y_ = nullptr;
   int64_t ld_ = 0, ldy_ = 0, bb_ = 4;
   size_t ws_elems_ = 0;
+  bool cx_ = false;
+  int cw_ = 1;
   int last_panel_ = 0;
   std::string last_sched_;
   float* E_[2] = {nullptr, nullptr};  // FP32 correction blocks (mixed filter)
@@ -738,6 +842,12 @@ class Solver {
   }
 
   long run(rcg_solve_result* R, std::vector<double>& eigValues, std::vector<double>& eigRes) {
+    if (eng_.cx()) return run_t<cplx>(R, eigValues, eigRes);
+    return run_t<double>(R, eigValues, eigRes);
+  }
+
+  template <class S>
+  long run_t(rcg_solve_result* R, std::vector<double>& eigValues, std::vector<double>& eigRes) {
     const double t0 = wall();
     const int64_t n = op_->n;
     const long N = (long)n;
@@ -756,9 +866,11 @@ class Solver {
       }
       maxDim = N;
     }
-    const int64_t ld = block_ld(subspaceSize);
+    // leading dimensions in real elements (2 per complex entry)
+    const int cw = eng_.cw();
+    const int64_t ld = cw * block_ld(subspaceSize);
     const int64_t ycols = std::max<long>(std::min<long>(maxDim, N), subspaceSize);
-    const int64_t ldy = block_ld(ycols);
+    const int64_t ldy = cw * block_ld(ycols);
     const size_t blk = sizeof(double) * (size_t)n * (size_t)ld;
     Vb_.ensure(blk);
     Tb_.ensure(blk);
@@ -822,12 +934,13 @@ class Solver {
     std::vector<char> fRitz(subspaceSize, 0);
     std::vector<double> theta(subspaceSize, 0.0), oldEigs(subspaceSize, 0.0),
         eigDiffs(subspaceSize, 1.0), oldEigDiffs(subspaceSize, 1.0), subspaceResiduals,
-        residualHistory, H, Yh;
+        residualHistory;
+    std::vector<S> H, Yh;
 
     // start vectors (LM:948-985)
     if (start_ && start_cols_ > 0) {
       const long have = std::min<long>((long)start_cols_, subspaceSize);
-      rcg::launch_copy_cols<double>(n, start_, (int)start_ld_, 0, eng_.V(), (int)ld, 0, (int)have, st);
+      eng_.copy_cols(start_, cw * start_ld_, 0, eng_.V(), ld, 0, have);
       if (have < subspaceSize) eng_.randomize(have, subspaceSize - have, p_.seed, &counter_);
     } else {
       eng_.randomize(0, subspaceSize, p_.seed, &counter_);
@@ -838,7 +951,7 @@ class Solver {
       eng_.project(subspaceSize, H);
       rcg::dense::jacobi_eigensystem(H.data(), subspaceSize, theta, Yh);
       eng_.rotate_residuals(subspaceSize, Yh, theta, subspaceSize, eigRes);
-      rcg::launch_copy_cols<double>(n, eng_.V(), (int)ld, 0, eng_.Y(), (int)ldy, 0, (int)subspaceSize, st);
+      eng_.copy_cols(eng_.V(), ld, 0, eng_.Y(), ldy, 0, subspaceSize);
       eigValues = theta;
       found_ = subspaceSize;
       R->found = subspaceSize;
@@ -999,8 +1112,7 @@ class Solver {
         exitFlag = true;
       }
       if (foundCount > 0) {
-        rcg::launch_copy_cols<double>(n, eng_.V(), (int)ld, 0, eng_.Y(), (int)ldy, (int)foundSize,
-                                      (int)foundCount, st);
+        eng_.copy_cols(eng_.V(), ld, 0, eng_.Y(), ldy, foundSize, foundCount);
         for (long i = 0; i < foundCount; ++i) {
           eigValues.push_back(theta[i]);
           eigRes.push_back(subspaceResiduals[i]);
@@ -1008,8 +1120,7 @@ class Solver {
         foundSize += foundCount;
       }
       if (!exitFlag && foundCount > 0) {  // shuffle (LM:1549-1555)
-        rcg::launch_copy_cols<double>(n, eng_.V(), (int)ld, (int)foundCount, eng_.T(), (int)ld, 0,
-                                      (int)(subspaceSize - foundCount), st);
+        eng_.copy_cols(eng_.V(), ld, foundCount, eng_.T(), ld, 0, subspaceSize - foundCount);
         eng_.swapVT();
         for (long i = 0; i < subspaceSize; ++i) {
           const bool kept = i + foundCount < subspaceSize;
@@ -1052,7 +1163,7 @@ class Solver {
       if (eigValues[i] > eigValues[i + 1]) nonMonotone = true;
     if (nonMonotone || ((p_.refine || mixed) && foundSize > 0)) {
       const double tr = wall();
-      repair(foundSize, eigValues, eigRes);
+      repair<S>(foundSize, eigValues, eigRes);
       count_apply_ += 2 * foundSize;
       count_eig_ += 1;
       t_rr_ += wall() - tr;
@@ -1092,6 +1203,7 @@ class Solver {
   // Rayleigh-Ritz over the found vectors Y[:, :f] (LM:1672-1707): the
   // projection, its eigensystem, the rotated and normalized vectors and
   // their residuals, with the Ritz values stored as the eigenvalues.
+  template <class S>
   void repair(long f, std::vector<double>& eigValues, std::vector<double>& eigRes) {
     const int64_t n = op_->n;
     const int64_t ldy = eng_.ldy();
@@ -1102,12 +1214,12 @@ class Solver {
     Engine e2(op_->ctx, op_, n);
     e2.reserve(ldy, ldy);
     e2.attach(eng_.Y(), T2.as<double>(), AY.as<double>(), ldy, eng_.Y(), ldy);
-    std::vector<double> H, th, Z, r;
+    std::vector<S> H, Z;
+    std::vector<double> th, r;
     e2.project(f, H);
     rcg::dense::jacobi_eigensystem(H.data(), f, th, Z);
     e2.rotate_residuals(f, Z, th, f, r);
-    if (!e2.v_is_first())
-      rcg::launch_copy_cols<double>(n, e2.V(), (int)ldy, 0, eng_.Y(), (int)ldy, 0, (int)f, e2.stream());
+    if (!e2.v_is_first()) e2.copy_cols(e2.V(), ldy, 0, eng_.Y(), ldy, 0, f);
     e2.sync();
     eigValues.assign(th.begin(), th.end());
     eigRes = r;
@@ -1116,9 +1228,8 @@ class Solver {
   void export_vectors(rcg_block_s* out) {
     if (!out || found_ <= 0) return;
     const long cnt = std::min<long>(found_, (long)out->cols);
-    rcg::launch_copy_cols<double>(op_->n, eng_.Y(), (int)eng_.ldy(), 0,
-                                  static_cast<double*>(out->d()), (int)out->ld, 0, (int)cnt,
-                                  eng_.stream());
+    eng_.copy_cols(eng_.Y(), eng_.ldy(), 0, static_cast<double*>(out->d()), eng_.cw() * out->ld, 0,
+                   cnt);
     eng_.sync();
   }
 
@@ -1137,14 +1248,23 @@ class Solver {
   double guard_ = 0.0, t_filter_ = 0.0, t_ortho_ = 0.0, t_rr_ = 0.0;
 };
 
-void check_pair(rcg_operator op, rcg_block a, rcg_block b, int64_t cols) {
+// allow_cx: a complex operator with complex (C128) blocks is accepted too.
+void check_pair(rcg_operator op, rcg_block a, rcg_block b, int64_t cols, bool allow_cx = false) {
   need(op && a && b, "NULL argument");
   need(a->n == op->n && b->n == op->n, "block row count differs from the operator");
   need(a->ld == b->ld, "blocks must share the leading dimension");
   need(cols >= 0 && cols <= a->cols && cols <= b->cols, "column count exceeds the block");
-  need(a->dtype == RCG_F64 && b->dtype == RCG_F64, "FP64 blocks expected");
+  if (allow_cx && op->dtype == RCG_C128) {
+    need(a->dtype == RCG_C128 && b->dtype == RCG_C128, "a complex operator takes C128 blocks");
+  } else {
+    need(a->dtype == RCG_F64 && b->dtype == RCG_F64, "FP64 blocks expected");
+    need(op->dtype == RCG_F64, "this entry point takes real FP64 operators");
+  }
   need(a->ctx == op->ctx && b->ctx == op->ctx, "handles belong to different contexts");
 }
+
+// leading dimension of a block in real elements
+int64_t real_ld(const rcg_block_s* b) { return (b->dtype == RCG_C128 ? 2 : 1) * b->ld; }
 
 // Engine over caller blocks for the piecewise entry points.
 struct Piece {
@@ -1225,7 +1345,7 @@ int rcg_block_create(rcg_context ctx, int64_t n, int64_t cols, int dtype, rcg_bl
   return guarded([&] {
     need(ctx && out, "NULL argument");
     need(n > 0 && cols > 0, "block dimensions must be positive");
-    need(dtype == RCG_F64 || dtype == RCG_F32, "blocks are FP64 or FP32");
+    need(dtype == RCG_F64 || dtype == RCG_F32 || dtype == RCG_C128, "blocks are FP64, FP32 or C128");
     DeviceGuard g(ctx->device);
     auto b = std::make_unique<rcg_block_s>();
     b->ctx = ctx;
@@ -1307,13 +1427,19 @@ int rcg_block_randomize(rcg_block b, int64_t col0, int64_t ncols, uint64_t seed,
     need(b && counter, "NULL argument");
     need(col0 >= 0 && ncols >= 0 && col0 + ncols <= b->cols, "column range exceeds the block");
     DeviceGuard g(b->ctx->device);
-    if (b->dtype == RCG_F64)
-      rcg::launch_randomize_cols<double>(b->n, static_cast<double*>(b->d()), (int)b->ld, (int)col0,
-                                         (int)ncols, seed, *counter, b->ctx->stream);
-    else
-      rcg::launch_randomize_cols<float>(b->n, static_cast<float*>(b->d()), (int)b->ld, (int)col0,
-                                        (int)ncols, seed, *counter, b->ctx->stream);
-    *counter += (uint64_t)ncols * (uint64_t)b->n;
+    if (b->dtype == RCG_C128) {
+      rcg::launch_randomize_cplx(b->n, static_cast<double*>(b->d()), (int)(2 * b->ld), (int)col0,
+                                 (int)ncols, seed, *counter, b->ctx->stream);
+      *counter += 2 * (uint64_t)ncols * (uint64_t)b->n;
+    } else {
+      if (b->dtype == RCG_F64)
+        rcg::launch_randomize_cols<double>(b->n, static_cast<double*>(b->d()), (int)b->ld,
+                                           (int)col0, (int)ncols, seed, *counter, b->ctx->stream);
+      else
+        rcg::launch_randomize_cols<float>(b->n, static_cast<float*>(b->d()), (int)b->ld, (int)col0,
+                                          (int)ncols, seed, *counter, b->ctx->stream);
+      *counter += (uint64_t)ncols * (uint64_t)b->n;
+    }
     RCG_HIP(hipStreamSynchronize(b->ctx->stream));
   });
 }
@@ -1337,7 +1463,7 @@ int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32
     need(ctx && rowptr && col && val && out, "NULL argument");
     need(n > 0 && nnz >= 0, "bad operator size");
     need(nnz <= INT32_MAX && n <= INT32_MAX, "sizes exceed the int32 index range");
-    need(val_dtype == RCG_F64, "CSR values must be FP64");
+    need(val_dtype == RCG_F64 || val_dtype == RCG_C128, "CSR values must be FP64 or C128");
     need(mem == RCG_MEM_HOST || mem == RCG_MEM_DEVICE, "bad memory flag");
     DeviceGuard g(ctx->device);
     auto o = std::make_unique<rcg_operator_s>();
@@ -1356,10 +1482,11 @@ int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32
       for (int64_t p = 0; p < nnz; ++p) need(col[p] >= 0 && col[p] < n, "column index out of range");
       o->own_rowptr.ensure(sizeof(int) * (size_t)(n + 1));
       o->own_col.ensure(sizeof(int) * (size_t)std::max<int64_t>(nnz, 1));
-      o->own_val.ensure(sizeof(double) * (size_t)std::max<int64_t>(nnz, 1));
+      const size_t vs = rcg::dtype_size(val_dtype);
+      o->own_val.ensure(vs * (size_t)std::max<int64_t>(nnz, 1));
       RCG_HIP(hipMemcpy(o->own_rowptr.p, rowptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice));
       RCG_HIP(hipMemcpy(o->own_col.p, col, sizeof(int) * nnz, hipMemcpyHostToDevice));
-      RCG_HIP(hipMemcpy(o->own_val.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+      RCG_HIP(hipMemcpy(o->own_val.p, val, vs * nnz, hipMemcpyHostToDevice));
       o->rowptr = o->own_rowptr.as<int>();
       o->col = o->own_col.as<int>();
       o->val = o->own_val.as<double>();
@@ -1394,12 +1521,14 @@ int rcg_operator_destroy(rcg_operator op) {
 int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alpha, double beta,
                        int64_t cols) {
   return guarded([&] {
-    check_pair(op, in, out, cols);
+    check_pair(op, in, out, cols, true);
     need(in != out, "in and out must differ");
     DeviceGuard g(op->ctx->device);
+    const bool cx = op->dtype == RCG_C128;
     rcg::launch_csr_apply<double>(op->n, op->rowptr, op->col, op->val,
                                   static_cast<const double*>(in->d()), static_cast<double*>(out->d()),
-                                  (int)in->ld, (int)cols, alpha, beta, op->ctx->stream);
+                                  (int)real_ld(in), (int)((cx ? 2 : 1) * cols), alpha, beta,
+                                  op->ctx->stream, nullptr, cx);
     RCG_HIP(hipStreamSynchronize(op->ctx->stream));
   });
 }
@@ -1516,12 +1645,13 @@ int rcg_small_eigensystem(int64_t n, const double* H, double* evals, double* eve
 int rcg_residuals(rcg_operator op, rcg_block V, rcg_block W, int64_t count, const double* theta,
                   double* res_host) {
   return guarded([&] {
-    check_pair(op, V, W, count);
+    check_pair(op, V, W, count, true);
     need(V != W && theta && res_host, "bad arguments");
     DeviceGuard g(op->ctx->device);
     Engine e(op->ctx, op, op->n);
-    e.reserve(V->ld, 4);
-    e.attach(static_cast<double*>(V->d()), nullptr, static_cast<double*>(W->d()), V->ld, nullptr, 0);
+    e.reserve(real_ld(V), 4);
+    e.attach(static_cast<double*>(V->d()), nullptr, static_cast<double*>(W->d()), real_ld(V),
+             nullptr, 0);
     std::vector<double> r;
     e.residuals(count, theta, r);
     std::memcpy(res_host, r.data(), sizeof(double) * count);
@@ -1554,7 +1684,9 @@ int rcg_solve_lowest(rcg_operator op, const rcg_solve_params* params, rcg_solve_
     need(params->k >= 1, "eigCount must be >= 1");
     need(params->filter_dtype == RCG_F64 || params->filter_dtype == RCG_F32,
          "filter_dtype must be RCG_F64 or RCG_F32");
-    if (evecs) need(evecs->n == op->n && evecs->dtype == RCG_F64 && evecs->ctx == op->ctx, "evecs block mismatch");
+    if (evecs)
+      need(evecs->n == op->n && evecs->dtype == op->dtype && evecs->ctx == op->ctx,
+           "evecs block mismatch (rows, dtype of the operator, context)");
     DeviceGuard g(op->ctx->device);
     std::memset(result, 0, sizeof(*result));
     Solver s(op, *params);

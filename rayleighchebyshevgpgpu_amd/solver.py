@@ -65,7 +65,11 @@ class Block:
         self.ctx = ctx or default_context()
         self.n, self.cols = int(n), int(cols)
         self.dtype = np.dtype(dtype)
-        code = capi.F64 if self.dtype == np.float64 else capi.F32
+        codes = {np.dtype(np.float64): capi.F64, np.dtype(np.float32): capi.F32,
+                 np.dtype(np.complex128): capi.C128}
+        if self.dtype not in codes:
+            raise TypeError(f"blocks are float64, float32 or complex128, not {self.dtype}")
+        code = codes[self.dtype]
         h = C.c_void_p()
         check(lib().rcg_block_create(self.ctx.h, self.n, self.cols, code, C.byref(h)))
         self.h = h
@@ -113,15 +117,17 @@ class CsrOperator:
 
     def __init__(self, csr, ctx: Context | None = None, grid=None):
         self.ctx = ctx or default_context()
-        if csr.is_complex:
-            raise NotImplementedError("complex CSR operators are not available in this build")
         self.n, self.nnz = csr.n, csr.nnz
-        self._keep = (csr.rowptr, csr.col, csr.val)
+        self.is_complex = bool(csr.is_complex)
+        self.dtype = np.dtype(np.complex128 if self.is_complex else np.float64)
+        val = np.ascontiguousarray(csr.val, dtype=self.dtype)
+        rowptr = np.ascontiguousarray(csr.rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(csr.col, dtype=np.int32)
         h = C.c_void_p()
         check(lib().rcg_operator_csr_create(
-            self.ctx.h, csr.n, csr.nnz, csr.rowptr.ctypes.data_as(C.c_void_p),
-            csr.col.ctypes.data_as(C.c_void_p), csr.val.ctypes.data_as(C.c_void_p), capi.F64,
-            capi.MEM_HOST, C.byref(h)))
+            self.ctx.h, csr.n, csr.nnz, rowptr.ctypes.data_as(C.c_void_p),
+            col.ctypes.data_as(C.c_void_p), val.ctypes.data_as(C.c_void_p),
+            capi.C128 if self.is_complex else capi.F64, capi.MEM_HOST, C.byref(h)))
         self.h = h
         self._keep = None
         grid = grid if grid is not None else getattr(csr, "grid", None)
