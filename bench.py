@@ -47,6 +47,18 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "converged eigenpairs/s @ resid≤1e-10, mixed vs FP64; p50/p99 solve latency"
 K, INC, BUF, TOL = 64, 32, 32, 1e-10
 GRID = 128
+# workloads: the default is the 128^3 Anderson solve; "hofstadter1024" is the
+# complex-Hermitian 1024^2 lattice (k=128, inc=64, buf=32, complex64 mixed)
+WORKLOADS = {
+    "anderson128": dict(k=64, inc=32, buf=32, tol=1e-10, complex=False,
+                        label="3-D 7-point periodic Laplacian 128^3 + Anderson U[-2,2] (seed 1)",
+                        profile="128cube"),
+    "hofstadter1024": dict(k=128, inc=64, buf=32, tol=1e-10, complex=True,
+                           label=("2-D periodic Hofstadter lattice 1024^2, Peierls phase "
+                                  "exp(2 pi i x/64), onsite U[-1,1] (seed 2), complex128 CSR"),
+                           profile="hofstadter1024"),
+}
+WL = WORKLOADS["anderson128"]
 
 
 def log(msg):
@@ -55,6 +67,8 @@ def log(msg):
 
 def build_operator(grid=GRID):
     from rayleighchebyshevgpgpu_amd import operators as ops
+    if WL["complex"]:
+        return ops.hofstadter_2d(1024, phi=1.0 / 64, W=2.0, seed=2)
     n = grid ** 3
     pot = ops.anderson_potential(n, 4.0, 1)
     return ops.laplacian_3d(grid, bc="periodic", potential=pot)
@@ -62,7 +76,7 @@ def build_operator(grid=GRID):
 
 def traffic_from_profiles(precision="fp64"):
     """HBM bytes per Chebyshev-step launch from the committed rocprofv3 --pmc summary."""
-    name = "cheb_step_128cube_f32src" if precision == "mixed" else "cheb_step_128cube_f64"
+    name = f"cheb_step_{WL['profile']}_" + ("f32src" if precision == "mixed" else "f64")
     path = os.path.join(ROOT, "profiles", name + "_traffic.json")
     if not os.path.exists(path):
         return None
@@ -92,7 +106,7 @@ def cpu_baseline(A, gpu_counts, max_inner=1):
         "cores": threads,
         "kind": "port",
         "sample": (f"CPU oracle (restated RayleighChebyshevLM, OpenMP, {threads} threads) on the "
-                   f"same 128^3 operator: Lanczos bounds + {max_inner} inner iteration(s) per outer "
+                   f"same operator ({WL['label']}): Lanczos bounds + {max_inner} inner iteration(s) per outer "
                    f"pass, {wall:.1f} s wall; per-degree filter {per_deg:.3f} s, per-MGS "
                    f"{per_ortho:.3f} s, per-deflation {per_defl:.3f} s, per-Rayleigh-Ritz "
                    f"{per_rr:.3f} s, scaled by the GPU solve's {gpu_counts['degrees']} filter "
@@ -148,6 +162,7 @@ def main():
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--workload", default="anderson128", choices=sorted(WORKLOADS))
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solves")
     ap.add_argument("--kernel-only", type=int, default=0, metavar="M",
                     help="profiling mode: run M Chebyshev steps of the filter on the 64-column "
@@ -160,9 +175,15 @@ def main():
               "on a single MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    global K, INC, BUF, TOL, WL
+    WL = WORKLOADS[args.workload]
+    K, INC, BUF, TOL = WL["k"], WL["inc"], WL["buf"], WL["tol"]
     from rayleighchebyshevgpgpu_amd import capi, solver
 
     if args.kernel_only:
+        if WL["complex"]:
+            print("--kernel-only profiles the real workload only", file=sys.stderr)
+            sys.exit(2)
         return kernel_only(args)
 
     t0 = time.time()
@@ -170,7 +191,8 @@ def main():
     log(f"operator n={A.n} nnz={A.nnz} built in {time.time() - t0:.1f}s")
     ctx = solver.Context(0)
     op = solver.CsrOperator(A, ctx)
-    V = solver.Block(A.n, K, ctx=ctx)
+    bdt = np.complex128 if WL["complex"] else np.float64
+    V = solver.Block(A.n, max(K, INC + BUF), dtype=bdt, ctx=ctx)
 
     def run(precision, steps, warmup):
         """warmup + timed solves at one filter precision; returns the measurements."""
@@ -215,8 +237,14 @@ def main():
         t_dev = sum(x[2].t_filter_device for x in results)
         alg_bytes = sum(x[2].filter_bytes for x in results)
         achieved = alg_bytes / t_dev / 1e9 if t_dev > 0 else 0.0
-        kern = ("csr_block_kernel<float,16,1,kChebSrc> (FP32 correction step)" if precision == "mixed"
-                else "csr_block_kernel<double,32,1,kCheb> (fused FP64 Chebyshev step)")
+        if WL["complex"]:
+            kern = ("csr_block_kernel<float,...,kChebSrc,complex> (complex64 correction step)"
+                    if precision == "mixed" else
+                    "csr_block_kernel<double,...,kCheb,complex> (fused complex128 Chebyshev step)")
+        else:
+            kern = ("csr_block_kernel<float,16,1,kChebSrc> (FP32 correction step)"
+                    if precision == "mixed" else
+                    "csr_block_kernel<double,32,1,kCheb> (fused FP64 Chebyshev step)")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profiles(precision), "kernel": kern,
@@ -243,7 +271,7 @@ def main():
         }
 
     import ctypes as C
-    Wb = solver.Block(A.n, K, ctx=ctx)
+    Wb = solver.Block(A.n, max(K, INC + BUF), dtype=bdt, ctx=ctx)
     fp64 = run("fp64", args.steps, args.warmup)
     mixed = None if args.no_mixed else run("mixed", args.steps, args.warmup)
     if mixed is not None:
@@ -266,12 +294,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "c128" if WL["complex"] else "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"lowest-eigenpair solve, 3-D 7-point periodic Laplacian {args.grid}^3 + "
-                         "Anderson U[-2,2] (seed 1), k=64 inc=32 buf=32 tol=1e-10 residual-only, "
-                         "FP64 filter"),
+            "workload": (f"lowest-eigenpair solve, {WL['label']}, k={K} inc={INC} buf={BUF} "
+                         f"tol={TOL:g} residual-only, FP64 filter"),
             "n": A.n, "nnz": A.nnz, "k": K, "inc": INC, "buf": BUF, "tol": TOL,
             "precision": "fp64",
         },
@@ -285,8 +312,8 @@ def main():
             "ms_per_step": round(mixed["total"] / args.steps * 1e3, 2),
             "p50_ms": mixed["p50_ms"], "p99_ms": mixed["p99_ms"],
             "max_rel_eig_diff_vs_fp64": mixed["max_rel_eig_diff_vs_fp64"],
-            "dtype": "f32 filter steps, f64 start/recombination, orthonormalisation, "
-                     "Rayleigh-Ritz and refinement",
+            "dtype": (("c64" if WL["complex"] else "f32") + " filter steps, f64 start/"
+                      "recombination, orthonormalisation, Rayleigh-Ritz and refinement"),
             "solve": mixed["solve"], "roofline": mixed["roofline"]},
         "cpu_baseline": None,
     }

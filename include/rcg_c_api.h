@@ -108,6 +108,12 @@ int rcg_block_info(rcg_block b, int64_t* n, int64_t* cols, int64_t* ld, int* dty
 int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32_t* rowptr,
                             const int32_t* col, const void* val, int val_dtype, int mem,
                             rcg_operator* out);
+/* The user operator as a dense real symmetric n x n matrix (column-major
+ * with leading dimension lda; symmetric, so also row-major).  Copied to the
+ * device (RCG_MEM_HOST) or referenced (RCG_MEM_DEVICE).  Its filter steps are
+ * MFMA block products (FP64, or FP32 for the mixed-precision filter). */
+int rcg_operator_dense_create(rcg_context ctx, int64_t n, const double* A, int64_t lda, int mem,
+                              rcg_operator* out);
 /* No counterpart in the reference: a locality hint for operators on a
  * structured nx x ny x nz grid whose rows are numbered x + nx*(y + ny*z).
  * The CSR kernels may then visit x-lines in (y, z) tiles; results do not

@@ -77,6 +77,7 @@ def lib() -> C.CDLL:
         "rcg_block_info": ([vp, P(i64), P(i64), P(i64), P(i32), P(vp)], i32),
         "rcg_operator_csr_create": ([vp, i64, i64, vp, vp, vp, i32, i32, P(vp)], i32),
         "rcg_operator_destroy": ([vp], i32),
+        "rcg_operator_dense_create": ([vp, i64, vp, i64, i32, P(vp)], i32),
         "rcg_operator_set_grid": ([vp, i64, i64, i64], i32),
         "rcg_operator_apply": ([vp, vp, vp, d, d, i64], i32),
         "rcg_cheb_filter": ([vp, vp, vp, i64, i64, i64, d, d], i32),

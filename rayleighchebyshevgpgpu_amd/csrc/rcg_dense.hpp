@@ -11,6 +11,7 @@
 #include <cmath>
 #include <complex>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace rcg {
@@ -215,6 +216,190 @@ inline void jacobi_eigensystem(const cplx* H, long n, std::vector<double>& w,
     w[j] = a(idx[j], idx[j]).real();
     for (long i = 0; i < n; ++i) Z[i + j * n] = V[i + idx[j] * n];
   }
+}
+
+// ------------------------------------------- Householder + implicit QL
+// Eigensystem of a symmetric (real) or Hermitian (complex) n x n matrix in
+// O(n^3) with a small constant, for the larger Rayleigh-Ritz problems:
+//  1. Householder reduction A = Q T Q^H with T real symmetric tridiagonal
+//     (for complex A the off-diagonal phases are moved into Q);
+//  2. implicit QL iteration with Wilkinson-type shifts on T (the tql2 method
+//     of Bowdler, Martin, Reinsch and Wilkinson, 1968), each plane rotation
+//     applied to two contiguous columns of Q.
+// Ascending eigenvalues w, eigenvectors in the columns of Z.
+inline double conj_v(double x) { return x; }
+inline cplx conj_v(cplx x) { return std::conj(x); }
+inline double real_v(double x) { return x; }
+inline double real_v(cplx x) { return x.real(); }
+
+template <class S>
+inline void tridiagonalize(std::vector<S>& A, long n, std::vector<double>& d,
+                           std::vector<double>& e, std::vector<S>& Q) {
+  auto a = [&](long i, long j) -> S& { return A[i + j * n]; };
+  d.assign(n, 0.0);
+  e.assign(n, 0.0);
+  std::vector<std::vector<S>> vs(n > 2 ? n - 2 : 0);
+  std::vector<double> betas(n > 2 ? n - 2 : 0, 0.0);
+  std::vector<S> sub(n, S(0.0));  // complex subdiagonal before the phase fix
+  std::vector<S> p(n), w(n);
+  for (long k = 0; k + 2 < n; ++k) {
+    const long m = n - k - 1;
+    double sx = 0.0;
+    for (long i = 1; i < m; ++i) sx += std::norm(a(k + 1 + i, k));
+    const S x0 = a(k + 1, k);
+    if (sx == 0.0) {  // column already tridiagonal
+      sub[k] = x0;
+      continue;
+    }
+    const double nx = std::sqrt(sx + std::norm(x0));
+    const double ax0 = std::abs(x0);
+    const S ph = ax0 > 0.0 ? x0 / ax0 : S(1.0);
+    const S alpha = -ph * nx;
+    std::vector<S>& v = vs[k];
+    v.assign(m, S(0.0));
+    v[0] = x0 - alpha;
+    for (long i = 1; i < m; ++i) v[i] = a(k + 1 + i, k);
+    double vv = 0.0;
+    for (long i = 0; i < m; ++i) vv += std::norm(v[i]);
+    const double beta = 2.0 / vv;
+    betas[k] = beta;
+    sub[k] = alpha;
+    // trailing block A22 = A[k+1.., k+1..]: p = beta A22 v, w = p - K v,
+    // K = beta (v^H p) / 2, A22 -= v w^H + w v^H
+    for (long i = 0; i < m; ++i) {
+      S t = S(0.0);
+      for (long j = 0; j < m; ++j) t += a(k + 1 + i, k + 1 + j) * v[j];
+      p[i] = beta * t;
+    }
+    S vp = S(0.0);
+    for (long i = 0; i < m; ++i) vp += conj_v(v[i]) * p[i];
+    const double K = 0.5 * beta * real_v(vp);
+    for (long i = 0; i < m; ++i) w[i] = p[i] - K * v[i];
+    for (long j = 0; j < m; ++j) {
+      const S cvj = conj_v(v[j]), cwj = conj_v(w[j]);
+      for (long i = 0; i < m; ++i) a(k + 1 + i, k + 1 + j) -= v[i] * cwj + w[i] * cvj;
+    }
+  }
+  for (long k = 0; k < n; ++k) d[k] = real_v(a(k, k));
+  if (n >= 2) sub[n - 2] = a(n - 1, n - 2);
+  // Q = H_0 H_1 ... H_{n-3}, accumulated from the right end
+  Q.assign(n * n, S(0.0));
+  for (long i = 0; i < n; ++i) Q[i + i * n] = S(1.0);
+  for (long k = n - 3; k >= 0; --k) {
+    if (vs[k].empty()) continue;
+    const std::vector<S>& v = vs[k];
+    const long m = n - k - 1;
+    for (long j = k + 1; j < n; ++j) {
+      S t = S(0.0);
+      for (long i = 0; i < m; ++i) t += conj_v(v[i]) * Q[(k + 1 + i) + j * n];
+      t *= betas[k];
+      for (long i = 0; i < m; ++i) Q[(k + 1 + i) + j * n] -= v[i] * t;
+    }
+  }
+  // real off-diagonals: T' = D^H T D with D = diag(phases), Q <- Q D
+  S phase = S(1.0);
+  for (long k = 0; k + 1 < n; ++k) {
+    const double mag = std::abs(sub[k]);
+    e[k] = mag;
+    // column k+1 of Q takes the accumulated phase making sub[k] real and >= 0
+    const S t = mag > 0.0 ? sub[k] / mag : S(1.0);
+    phase = phase * t;
+    if constexpr (std::is_same<S, cplx>::value) {
+      for (long i = 0; i < n; ++i) Q[i + (k + 1) * n] *= phase;
+    } else {
+      if (phase < 0.0)
+        for (long i = 0; i < n; ++i) Q[i + (k + 1) * n] = -Q[i + (k + 1) * n];
+    }
+  }
+}
+
+template <class S>
+inline void tridiag_ql(std::vector<double>& d, std::vector<double>& e, long n, std::vector<S>& Z) {
+  const double eps = 2.220446049250313e-16;
+  for (long l = 0; l < n; ++l) {
+    int iter = 0;
+    long m;
+    do {
+      for (m = l; m + 1 < n; ++m) {
+        const double dd = std::abs(d[m]) + std::abs(d[m + 1]);
+        if (std::abs(e[m]) <= eps * dd) break;
+      }
+      if (m != l) {
+        if (++iter > 200) break;  // no convergence: keep what we have
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+        double s = 1.0, c = 1.0, pp = 0.0;
+        long i;
+        bool early = false;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[i];
+          const double b = c * e[i];
+          r = std::hypot(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= pp;
+            e[m] = 0.0;
+            early = true;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - pp;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          pp = s * r;
+          d[i + 1] = g + pp;
+          g = c * r - b;
+          S* zi = &Z[i * n];
+          S* zj = &Z[(i + 1) * n];
+          for (long k = 0; k < n; ++k) {
+            const S fz = zj[k];
+            zj[k] = s * zi[k] + c * fz;
+            zi[k] = c * zi[k] - s * fz;
+          }
+        }
+        if (early) continue;
+        d[l] -= pp;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+}
+
+template <class S>
+inline void tridiag_eigensystem(const S* H, long n, std::vector<double>& w, std::vector<S>& Z) {
+  std::vector<S> A(H, H + n * n);
+  for (long j = 0; j < n; ++j) {
+    A[j + j * n] = real_v(A[j + j * n]);
+    for (long i = j + 1; i < n; ++i) {
+      const S m = 0.5 * (A[i + j * n] + conj_v(A[j + i * n]));
+      A[i + j * n] = m;
+      A[j + i * n] = conj_v(m);
+    }
+  }
+  std::vector<double> d, e;
+  std::vector<S> Q;
+  tridiagonalize(A, n, d, e, Q);
+  tridiag_ql(d, e, n, Q);
+  std::vector<long> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](long x, long y) { return d[x] < d[y]; });
+  w.resize(n);
+  Z.assign(n * n, S(0.0));
+  for (long j = 0; j < n; ++j) {
+    w[j] = d[idx[j]];
+    std::copy(Q.begin() + idx[j] * n, Q.begin() + (idx[j] + 1) * n, Z.begin() + j * n);
+  }
+}
+
+// The Rayleigh-Ritz eigensolver: cyclic Jacobi for small matrices (the
+// reference's JacobiDiagonalizer role), Householder + QL from 48 on, where
+// Jacobi's O(sweeps n^3) with rotations of whole rows dominates the solve.
+template <class S>
+inline void small_eigensystem(const S* H, long n, std::vector<double>& w, std::vector<S>& Z) {
+  if (n >= 48) tridiag_eigensystem(H, n, w, Z);
+  else jacobi_eigensystem(H, n, w, Z);
 }
 
 // idx-th smallest eigenvalue of the symmetric tridiagonal (d, e) by Sturm

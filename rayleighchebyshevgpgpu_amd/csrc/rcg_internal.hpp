@@ -96,6 +96,17 @@ void launch_mixed_combine(int64_t n, double* V, int ld, int nc, const double* um
                           const float* E, hipStream_t st);
 void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st);
 
+// Dense symmetric operator A (n x n, lda; row-major = column-major): one
+// step on a row-interleaved block, mode as the CSR kernels:
+//   0 apply: out = p0 A X; 1: out = p0 A X + p1 out;
+//   2 first: out = p2 p1 (p0 A X + X); 3: out = p2 (p1 (p0 A X + X) - Z);
+//   4 source: mode 3 + kap[c] R.  MFMA FP64 (T = double) or FP32.
+enum StepMode : int { kStepApply = 0, kStepApplyBeta = 1, kStepFirst = 2, kStepCheb = 3, kStepSrc = 4 };
+template <typename T>
+void launch_dense_step(int64_t n, const T* A, int64_t lda, const T* X, const T* Z, T* out, int ld,
+                       int ncols, T p0, T p1, T p2, int mode, const T* R, const T* kap,
+                       hipStream_t st);
+
 // G (bx x by, column-major, ldg = bx) = X^H Y over n rows; ws holds partials.
 template <typename T>
 void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, int by, T* G,
@@ -147,6 +158,9 @@ void launch_lz_apply_dot(int64_t n, const int* rowptr, const int* col, const dou
 void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* alpha_k,
                          double* beta_k, double* ws, size_t ws_elems, hipStream_t st);
 size_t lz_ws_elems(int64_t n);
+// *out = w . v (two-phase, fixed order)
+void launch_lz_dot(int64_t n, const double* w, const double* v, double* out, double* ws,
+                   size_t ws_elems, hipStream_t st);
 // complex (interleaved) v += A w; alpha_k = Re sum conj(w) v, alpha_im_k = Im.
 void launch_lz_apply_dot_c(int64_t n, const int* rowptr, const int* col, const double* val,
                            const double* w, double* v, double* alpha_k, double* alpha_im_k,

@@ -715,6 +715,133 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
   }
 }
 
+// ------------------------------------------------ dense operator (MFMA)
+// v_mfma_f32_16x16x4f32: lane l supplies A[l%16][l/16] and B[l/16][l%16] as
+// for FP64; register j holds D[4(l/16) + j][l%16].
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4v mfma16x16x4(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<double> {
+  typedef d4v type;
+  // row within the 16x16 tile of register j of lane l
+  __device__ static int row(int j, int l) { return 4 * j + (l >> 4); }
+};
+template <>
+struct MfmaAcc<float> {
+  typedef f4v type;
+  __device__ static int row(int j, int l) { return 4 * (l >> 4) + j; }
+};
+
+// One filter step (or apply) of a dense symmetric operator A (n x n,
+// row-major = column-major) on a row-interleaved block:
+//   kApply: out = p0 A X;  kApplyBeta: out = p0 A X + p1 out;
+//   kChebFirst: out = p2 p1 (p0 A X + X);  kCheb: out = p2 (p1 (p0 A X + X) - Z);
+//   kChebSrc: kCheb + kap[c] R.
+// 64 x 64 output tile per workgroup, 4 waves in 2 x 2, each 32 x 32 (2 x 2
+// MFMA tiles); K in chunks of DBK staged in LDS, the next chunk's loads
+// issued before the current chunk's MFMAs.  Workgroups of one row tile sit
+// on one XCD so the A rows they share come from the same L2.
+template <typename T>
+struct DenseCfg {
+  static constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+};
+template <typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void dense_step_kernel(
+    int64_t n, const T* __restrict__ A, int64_t lda, const T* __restrict__ X, const T* Z, T* out,
+    int ld, int ncols, T p0, T p1, T p2, const T* __restrict__ R, const T* __restrict__ kap,
+    int ntn) {
+  constexpr int BK = DenseCfg<T>::BK;
+  constexpr int BM = 64, BN = 64;
+  constexpr int AE = BM * BK / kThreads;  // A elements per thread per chunk
+  constexpr int XE = BK * BN / kThreads;  // X elements per thread per chunk
+  __shared__ T As[BM][BK + 1];
+  __shared__ T Xs[BK][BN + 1];
+  using Acc = typename MfmaAcc<T>::type;
+  const int64_t L = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t ntm = (n + BM - 1) / BM;
+  if (L >= ntm * ntn) return;
+  const int64_t i0 = (L / ntn) * BM;
+  const int c0 = (int)(L % ntn) * BN;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  Acc acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+  T ra[AE], rx[XE];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < AE; ++q) {
+      const int e = threadIdx.x + q * kThreads;
+      const int r = e / BK, kk = e % BK;
+      const int64_t i = i0 + r, k = k0 + kk;
+      ra[q] = (i < n && k < n) ? A[i * lda + k] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < XE; ++q) {
+      const int e = threadIdx.x + q * kThreads;
+      const int kk = e / BN, cc = e % BN;
+      const int64_t k = k0 + kk;
+      const int c = c0 + cc;
+      rx[q] = (k < n && c < ncols) ? X[k * ld + c] : T(0);
+    }
+  };
+  fetch(0);
+  for (int64_t k0 = 0; k0 < n; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < AE; ++q) {
+      const int e = threadIdx.x + q * kThreads;
+      As[e / BK][e % BK] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < XE; ++q) {
+      const int e = threadIdx.x + q * kThreads;
+      Xs[e / BN][e % BN] = rx[q];
+    }
+    __syncthreads();
+    if (k0 + BK < n) fetch(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      const int kk = ks + (lane >> 4);
+      T af[2], bf[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) af[a] = As[32 * wm + 16 * a + (lane & 15)][kk];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bf[b] = Xs[kk][32 * wn + 16 * b + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = mfma16x16x4(af[a], bf[b], acc[a][b]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t i = i0 + 32 * wm + 16 * a + MfmaAcc<T>::row(j, lane);
+        const int c = c0 + 32 * wn + 16 * b + (lane & 15);
+        if (i >= n || c >= ncols) continue;
+        const int64_t off = i * ld + c;
+        const T v = acc[a][b][j];
+        T o;
+        if constexpr (MODE == kApply) o = p0 * v;
+        else if constexpr (MODE == kApplyBeta) o = fma(p0, v, p1 * out[off]);
+        else if constexpr (MODE == kChebFirst) o = p2 * (p1 * fma(p0, v, X[off]));
+        else if constexpr (MODE == kCheb) o = p2 * fma(p1, fma(p0, v, X[off]), -Z[off]);
+        else o = p2 * fma(kap[c], R[off], fma(p1, fma(p0, v, X[off]), -Z[off]));
+        out[off] = o;
+      }
+}
+
 // ------------------------------------------------------ column reductions
 // Threads are laid out as (row slot, column): CW columns per pass.
 constexpr int kColRows = 512;  // rows per partial block
@@ -1010,6 +1137,18 @@ __global__ __launch_bounds__(kThreads) void lz_apply_dot_c_kernel(
   }
 }
 
+// partial sum w . v (the dense operator's Lanczos alpha)
+__global__ __launch_bounds__(kThreads) void lz_dot_kernel(int64_t n, const double* w,
+                                                          const double* v, double* P) {
+  __shared__ double red[kThreads];
+  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
+  const int64_t i1 = min(n, i0 + kLzRows);
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(w[i], v[i], s);
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) P[blockIdx.x] = s;
+}
+
 // v += (-alpha) w; partial |v|^2
 __global__ __launch_bounds__(kThreads) void lz_axpy_norm_kernel(int64_t n, const double* w,
                                                                 double* v, const double* alpha,
@@ -1106,6 +1245,29 @@ void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads),
                      0, st, n, x, y);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_dense_step(int64_t n, const T* A, int64_t lda, const T* X, const T* Z, T* out, int ld,
+                       int ncols, T p0, T p1, T p2, int mode, const T* R, const T* kap,
+                       hipStream_t st) {
+  if (n <= 0 || ncols <= 0) return;
+  const int ntn = (ncols + 63) / 64;
+  const int64_t tiles = ((n + 63) / 64) * ntn;
+  const dim3 grid((unsigned)round_up(tiles, kXcds));
+#define RCG_DENSE(M)                                                                           \
+  hipLaunchKernelGGL((dense_step_kernel<T, M>), grid, dim3(kThreads), 0, st, n, A, lda, X, Z, out, \
+                     ld, ncols, p0, p1, p2, R, kap, ntn)
+  switch (mode) {
+    case kApply: RCG_DENSE(kApply); break;
+    case kApplyBeta: RCG_DENSE(kApplyBeta); break;
+    case kChebFirst: RCG_DENSE(kChebFirst); break;
+    case kCheb: RCG_DENSE(kCheb); break;
+    case kChebSrc: RCG_DENSE(kChebSrc); break;
+    default: throw Error(1, "unknown dense step mode");
+  }
+#undef RCG_DENSE
   RCG_HIP(hipGetLastError());
 }
 
@@ -1250,6 +1412,15 @@ void launch_fill_cols(int64_t n, T* X, int ld, int c0, int cnt, T value, hipStre
 
 size_t lz_ws_elems(int64_t n) { return 2 * (size_t)((n + kLzRows - 1) / kLzRows) + 4; }
 
+void launch_lz_dot(int64_t n, const double* w, const double* v, double* out, double* ws,
+                   size_t ws_elems, hipStream_t st) {
+  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
+  hipLaunchKernelGGL(lz_dot_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, w, v, ws);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, out, 0);
+  RCG_HIP(hipGetLastError());
+}
+
 void launch_randomize_cplx(int64_t n, double* X, int ld, int c0, int cnt, uint64_t seed,
                            uint64_t counter, hipStream_t st) {
   if (n <= 0 || cnt <= 0) return;
@@ -1343,7 +1514,9 @@ void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* al
   template void launch_copy_cols<T>(int64_t, const T*, int, int, T*, int, int, int, hipStream_t); \
   template void launch_randomize_cols<T>(int64_t, T*, int, int, int, uint64_t, uint64_t,          \
                                          hipStream_t);                                            \
-  template void launch_fill_cols<T>(int64_t, T*, int, int, int, T, hipStream_t);
+  template void launch_fill_cols<T>(int64_t, T*, int, int, int, T, hipStream_t);                 \
+  template void launch_dense_step<T>(int64_t, const T*, int64_t, const T*, const T*, T*, int, int, \
+                                     T, T, T, int, const T*, const T*, hipStream_t);
 
 RCG_INST(double)
 RCG_INST(float)

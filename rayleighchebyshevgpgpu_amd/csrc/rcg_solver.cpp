@@ -127,6 +127,12 @@ struct rcg_operator_s {
   };
   std::map<int64_t, Tuned> tuned;  // autotuned filter shape per (kind, ld, columns)
   DevBuf val32;                    // FP32 copy of val for the mixed-precision filter
+  // dense symmetric operator (kind 1): n x n, row-major = column-major
+  int kind = 0;                    // 0 CSR, 1 dense
+  const double* dense = nullptr;
+  int64_t lda = 0;
+  DevBuf own_dense;
+  bool is_dense() const { return kind == 1; }
 };
 
 namespace {
@@ -345,6 +351,11 @@ class Engine {
   }
   // W[:, :b] = A V[:, :b] (b complex columns when complex).
   void apply(const double* X, double* out, int64_t b) {
+    if (op_->is_dense()) {
+      rcg::launch_dense_step<double>(n_, op_->dense, op_->lda, X, nullptr, out, (int)ld_, (int)b,
+                                     1.0, 0.0, 0.0, rcg::kStepApply, nullptr, nullptr, st_);
+      return;
+    }
     rcg::launch_csr_apply<double>(n_, op_->rowptr, op_->col, op_->val, X, out, (int)ld_,
                                   (int)(cw_ * b), 1.0, 0.0, st_, sched_for(cw_ * b), cx_);
   }
@@ -361,6 +372,7 @@ class Engine {
   // V <- P_m(A')^rep V over b columns (LanczosCpolyOperatorLM.h:192-304):
   // X1 = g2(g1 A X0 + X0), Xj = g2(g1 A X_{j-1} + X_{j-1}) - X_{j-2}, out = Xm/(m+1).
   void filter(int64_t b, long m, long rep, double lmax, double shift) {
+    if (op_->is_dense()) return filter_dense(b, m, rep, lmax, shift);
     const auto c = rcg::cpoly::coefficients(m, lmax, shift);
     const int64_t rb = cw_ * b;  // real columns
     const auto tn = tuned_for(rb);
@@ -383,6 +395,30 @@ class Engine {
         rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, X1, X0, X0,
                                       (int)ld_, (int)rb, c.g1, c.g2, s, false, st_, tn.panel, sc,
                                       cx_);
+        std::swap(X0, X1);
+      }
+      if (X1 != V()) swapVT();
+    }
+    step_end();
+  }
+
+  // The same recurrence with the dense MFMA step.
+  void filter_dense(int64_t b, long m, long rep, double lmax, double shift) {
+    const auto c = rcg::cpoly::coefficients(m, lmax, shift);
+    last_panel_ = (int)b;
+    last_sched_ = "dense";
+    const double nn = double(n_) * double(n_);
+    step_begin(rep * m, double(rep) * double(m) * (8.0 * nn + 24.0 * double(n_) * double(b)));
+    for (long r = 0; r < rep; ++r) {
+      double* X0 = V();
+      double* X1 = T();
+      const double s1 = (m == 1) ? 0.5 : 1.0;
+      rcg::launch_dense_step<double>(n_, op_->dense, op_->lda, X0, nullptr, X1, (int)ld_, (int)b,
+                                     c.g1, c.g2, s1, rcg::kStepFirst, nullptr, nullptr, st_);
+      for (long j = 2; j <= m; ++j) {
+        const double s = (j == m) ? 1.0 / double(m + 1) : 1.0;
+        rcg::launch_dense_step<double>(n_, op_->dense, op_->lda, X1, X0, X0, (int)ld_, (int)b,
+                                       c.g1, c.g2, s, rcg::kStepCheb, nullptr, nullptr, st_);
         std::swap(X0, X1);
       }
       if (X1 != V()) swapVT();
@@ -440,15 +476,19 @@ class Engine {
     const float* dk = reinterpret_cast<const float*>(dcp + nd);
     const float* val32 = values_f32();
     const float g1f = float(c.g1), g2f = float(c.g2);
-    const auto tn = tuned_for(rb, cx_ ? 3 : 1, [&](int panel, const rcg::CsrSched* sc) {
-      rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, E_[0], E_[1], E_[1], R32_,
-                                dk + 2 * ld_, (int)ld_, (int)rb, g1f, g2f, 1.0f, st_, panel, sc,
-                                cx_);
-    });
-    const rcg::CsrSched* sc = &op_->scheds[tn.sched]->sc;
+    const bool dn = op_->is_dense();
+    rcg_operator_s::Tuned tn{(int)rb, 0};
+    if (!dn)
+      tn = tuned_for(rb, cx_ ? 3 : 1, [&](int panel, const rcg::CsrSched* sc) {
+        rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, E_[0], E_[1], E_[1], R32_,
+                                  dk + 2 * ld_, (int)ld_, (int)rb, g1f, g2f, 1.0f, st_, panel, sc,
+                                  cx_);
+      });
+    const rcg::CsrSched* sc = dn ? nullptr : &op_->scheds[tn.sched]->sc;
     last_panel_ = tn.panel;
-    last_sched_ = op_->scheds[tn.sched]->name;
-    const double csr = double(op_->nnz) * (4.0 + 4.0 * cw_) + 4.0 * double(n_ + 1);
+    last_sched_ = dn ? "dense" : op_->scheds[tn.sched]->name;
+    const double csr = dn ? 4.0 * double(n_) * double(n_)
+                          : double(op_->nnz) * (4.0 + 4.0 * cw_) + 4.0 * double(n_ + 1);
     const double blk = 4.0 * double(n_) * double(rb);
     for (long r = 0; r < rep; ++r) {
       apply(V(), W_, b);
@@ -458,9 +498,13 @@ class Engine {
       float* X1 = E_[1];
       step_begin((m - 1) * ((rb + tn.panel - 1) / tn.panel), double(m - 1) * (csr + 4.0 * blk));
       for (long j = 2; j <= m; ++j) {
-        rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, X1, X0, X0, R32_,
-                                  dk + (size_t)j * ld_, (int)ld_, (int)rb, g1f, g2f, 1.0f, st_,
-                                  tn.panel, sc, cx_);
+        if (dn)
+          rcg::launch_dense_step<float>(n_, val32, op_->lda, X1, X0, X0, (int)ld_, (int)rb, g1f,
+                                        g2f, 1.0f, rcg::kStepSrc, R32_, dk + (size_t)j * ld_, st_);
+        else
+          rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, X1, X0, X0, R32_,
+                                    dk + (size_t)j * ld_, (int)ld_, (int)rb, g1f, g2f, 1.0f, st_,
+                                    tn.panel, sc, cx_);
         std::swap(X0, X1);
       }
       step_end();
@@ -474,7 +518,7 @@ class Engine {
   }
   const float* values_f32() {
     if (!op_->val32.p) {
-      const int64_t nv = cw_ * op_->nnz;
+      const int64_t nv = op_->is_dense() ? op_->lda * n_ : cw_ * op_->nnz;
       op_->val32.ensure(sizeof(float) * (size_t)std::max<int64_t>(nv, 1));
       rcg::launch_to_f32(nv, op_->val, op_->val32.as<float>(), st_);
     }
@@ -723,7 +767,11 @@ class Engine {
     bool done = false;
     while (k < itmax && !done) {
       if (k != 0) rcg::launch_lz_swap(nr, v, w, beta + (k - 1), st_);
-      if (cx_)
+      if (op_->is_dense()) {
+        rcg::launch_dense_step<double>(n_, op_->dense, op_->lda, w, nullptr, v, 1, 1, 1.0, 1.0,
+                                       0.0, rcg::kStepApplyBeta, nullptr, nullptr, st_);
+        rcg::launch_lz_dot(n_, w, v, alpha + k, ws(), ws_elems_, st_);
+      } else if (cx_)
         rcg::launch_lz_apply_dot_c(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k,
                                    alpha_im + k, ws(), ws_elems_, st_);
       else
@@ -949,7 +997,7 @@ class Solver {
     if (N == subspaceSize) {  // quick return (LM:1005-1036)
       eng_.orthonormalize(subspaceSize);
       eng_.project(subspaceSize, H);
-      rcg::dense::jacobi_eigensystem(H.data(), subspaceSize, theta, Yh);
+      rcg::dense::small_eigensystem(H.data(), subspaceSize, theta, Yh);
       eng_.rotate_residuals(subspaceSize, Yh, theta, subspaceSize, eigRes);
       eng_.copy_cols(eng_.V(), ld, 0, eng_.Y(), ldy, 0, subspaceSize);
       eigValues = theta;
@@ -1031,7 +1079,7 @@ class Solver {
         ta = wall();
         eng_.project(subspaceSize, H);
         count_apply_ += subspaceSize;
-        rcg::dense::jacobi_eigensystem(H.data(), subspaceSize, theta, Yh);
+        rcg::dense::small_eigensystem(H.data(), subspaceSize, theta, Yh);
         const long residualCheckCount = (foundSize + subspaceSize < N) ? inc : subspaceSize;
         eng_.rotate_residuals(subspaceSize, Yh, theta, residualCheckCount, subspaceResiduals);
         fTheta.assign(theta.begin(), theta.end());
@@ -1217,7 +1265,7 @@ class Solver {
     std::vector<S> H, Z;
     std::vector<double> th, r;
     e2.project(f, H);
-    rcg::dense::jacobi_eigensystem(H.data(), f, th, Z);
+    rcg::dense::small_eigensystem(H.data(), f, th, Z);
     e2.rotate_residuals(f, Z, th, f, r);
     if (!e2.v_is_first()) e2.copy_cols(e2.V(), ldy, 0, eng_.Y(), ldy, 0, f);
     e2.sync();
@@ -1495,9 +1543,37 @@ int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32
   });
 }
 
+int rcg_operator_dense_create(rcg_context ctx, int64_t n, const double* A, int64_t lda, int mem,
+                              rcg_operator* out) {
+  return guarded([&] {
+    need(ctx && A && out, "NULL argument");
+    need(n > 0 && lda >= n, "bad dense operator size");
+    need(mem == RCG_MEM_HOST || mem == RCG_MEM_DEVICE, "bad memory flag");
+    DeviceGuard g(ctx->device);
+    auto o = std::make_unique<rcg_operator_s>();
+    o->ctx = ctx;
+    o->n = n;
+    o->nnz = n * n;
+    o->dtype = RCG_F64;
+    o->kind = 1;
+    if (mem == RCG_MEM_DEVICE) {
+      o->dense = A;
+      o->lda = lda;
+    } else {
+      o->own_dense.ensure(sizeof(double) * (size_t)n * (size_t)n);
+      RCG_HIP(hipMemcpy2D(o->own_dense.p, sizeof(double) * n, A, sizeof(double) * lda,
+                          sizeof(double) * n, n, hipMemcpyHostToDevice));
+      o->dense = o->own_dense.as<double>();
+      o->lda = n;
+    }
+    *out = o.release();
+  });
+}
+
 int rcg_operator_set_grid(rcg_operator op, int64_t nx, int64_t ny, int64_t nz) {
   return guarded([&] {
     need(op != nullptr, "operator is NULL");
+    need(!op->is_dense(), "a grid hint applies to CSR operators");
     need(nx >= 1 && ny >= 1 && nz >= 1 && nx * ny * nz == op->n, "grid does not match the operator size");
     DeviceGuard g(op->ctx->device);
     RCG_HIP(hipStreamSynchronize(op->ctx->stream));
@@ -1525,10 +1601,17 @@ int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alph
     need(in != out, "in and out must differ");
     DeviceGuard g(op->ctx->device);
     const bool cx = op->dtype == RCG_C128;
-    rcg::launch_csr_apply<double>(op->n, op->rowptr, op->col, op->val,
-                                  static_cast<const double*>(in->d()), static_cast<double*>(out->d()),
-                                  (int)real_ld(in), (int)((cx ? 2 : 1) * cols), alpha, beta,
-                                  op->ctx->stream, nullptr, cx);
+    if (op->is_dense())
+      rcg::launch_dense_step<double>(op->n, op->dense, op->lda, static_cast<const double*>(in->d()),
+                                     nullptr, static_cast<double*>(out->d()), (int)in->ld,
+                                     (int)cols, alpha, beta, 0.0,
+                                     beta == 0.0 ? rcg::kStepApply : rcg::kStepApplyBeta, nullptr,
+                                     nullptr, op->ctx->stream);
+    else
+      rcg::launch_csr_apply<double>(op->n, op->rowptr, op->col, op->val,
+                                    static_cast<const double*>(in->d()), static_cast<double*>(out->d()),
+                                    (int)real_ld(in), (int)((cx ? 2 : 1) * cols), alpha, beta,
+                                    op->ctx->stream, nullptr, cx);
     RCG_HIP(hipStreamSynchronize(op->ctx->stream));
   });
 }
@@ -1636,7 +1719,7 @@ int rcg_small_eigensystem(int64_t n, const double* H, double* evals, double* eve
   return guarded([&] {
     need(n >= 1 && H && evals && evecs, "bad arguments");
     std::vector<double> w, Z;
-    rcg::dense::jacobi_eigensystem(H, n, w, Z);
+    rcg::dense::small_eigensystem(H, n, w, Z);
     std::memcpy(evals, w.data(), sizeof(double) * n);
     std::memcpy(evecs, Z.data(), sizeof(double) * n * n);
   });

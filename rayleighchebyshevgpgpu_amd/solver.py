@@ -150,6 +150,44 @@ class CsrOperator:
             pass
 
 
+class DenseOperator:
+    """Dense real symmetric operator on the device (the user Otype as a dense
+    matrix); its filter steps are MFMA block products."""
+
+    def __init__(self, A, ctx: Context | None = None, device_ptr: int | None = None, n=None):
+        self.ctx = ctx or default_context()
+        self.is_complex = False
+        self.dtype = np.dtype(np.float64)
+        h = C.c_void_p()
+        if device_ptr is not None:  # A already in HBM (column-major, lda = n)
+            self.n = int(n)
+            check(lib().rcg_operator_dense_create(self.ctx.h, self.n, C.c_void_p(device_ptr), self.n,
+                                                  capi.MEM_DEVICE, C.byref(h)))
+        else:
+            A = np.asfortranarray(A, dtype=np.float64)
+            if A.ndim != 2 or A.shape[0] != A.shape[1]:
+                raise ValueError("a dense operator is a square matrix")
+            self.n = A.shape[0]
+            check(lib().rcg_operator_dense_create(self.ctx.h, self.n, A.ctypes.data_as(C.c_void_p),
+                                                  self.n, capi.MEM_HOST, C.byref(h)))
+        self.nnz = self.n * self.n
+        self.h = h
+
+    def apply(self, X: Block, Y: Block, alpha: float = 1.0, beta: float = 0.0, cols=None):
+        check(lib().rcg_operator_apply(self.h, X.h, Y.h, alpha, beta, X.cols if cols is None else cols))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rcg_operator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class RandomizeOp:
     """Start-vector stream (ArandomizeOpClassLMCuda role, seeded)."""
 

@@ -37,16 +37,21 @@ def test_small_eigensystem_host_path():
     """rcg_small_eigensystem runs on the host (no device needed)."""
     import numpy as np
     rng = np.random.default_rng(0)
-    for n in (1, 2, 5, 33):
+    # Jacobi below 48, Householder + QL from 48 (with a clustered spectrum too)
+    cases = [(1, False), (2, False), (5, False), (33, False), (48, False), (100, False), (64, True)]
+    for n, clustered in cases:
         G = rng.standard_normal((n, n))
+        if clustered:
+            Q, _ = np.linalg.qr(G)
+            G = (Q * np.repeat([1.0, 2.0, 2.0 + 1e-9, 5.0], n // 4)) @ Q.T
         H = np.asfortranarray((G + G.T) / 2)
         w = np.empty(n)
         Z = np.empty((n, n), order="F")
         capi.check(capi.lib().rcg_small_eigensystem(n, H.ctypes.data_as(C.c_void_p),
                                                     w.ctypes.data_as(C.c_void_p),
                                                     Z.ctypes.data_as(C.c_void_p)))
-        assert np.allclose(w, np.linalg.eigvalsh(H), atol=1e-13)
-        assert np.max(np.abs(H @ Z - Z * w)) < 1e-12
+        assert np.allclose(w, np.linalg.eigvalsh(H), atol=1e-13 * max(1.0, n / 10))
+        assert np.max(np.abs(H @ Z - Z * w)) < 1e-12 * max(1.0, n / 10)
         assert np.max(np.abs(Z.T @ Z - np.eye(n))) < 1e-13
 
 
