@@ -29,6 +29,7 @@ extern "C" {
 typedef struct rcg_context_s* rcg_context;
 typedef struct rcg_operator_s* rcg_operator;
 typedef struct rcg_block_s* rcg_block;
+typedef struct rcg_server_s* rcg_server;
 
 enum rcg_status {
   RCG_OK = 0,
@@ -234,6 +235,36 @@ typedef struct rcg_solve_result {
 int rcg_solve_lowest(rcg_operator op, const rcg_solve_params* params, rcg_solve_result* result,
                      double* evals, double* res, int64_t cap, rcg_block evecs, double* trace,
                      int64_t trace_cap);
+
+/* ------------------------------------------------------------ serving */
+/* No counterpart in the reference (it solves one operator per call): a
+ * continuous-batching solve server on one device.  Requests queue in arrival
+ * order and are admitted when one of `max_active` worker slots is free and
+ * the request's device footprint fits `hbm_budget` bytes next to the running
+ * solves.  Running solves share their Chebyshev filter launches (a batcher
+ * gathers the sweeps of all running solves for up to `window_us`
+ * microseconds); everything else runs on each worker's own stream.  A served
+ * solve gives the same result as the same request solved alone.  The caller
+ * keeps each operator alive until its ticket is done. */
+int rcg_server_create(rcg_context ctx, int64_t hbm_budget, int32_t max_active, int32_t window_us,
+                      rcg_server* out);
+/* Queue a solve (params as rcg_solve_lowest; random start vectors). */
+int rcg_server_submit(rcg_server s, rcg_operator op, const rcg_solve_params* params,
+                      int64_t* ticket);
+/* Wait for a ticket: result, eigenvalues and residuals (capacity cap), and
+ * times[3] = submit, admit, completion (seconds, steady clock).  Returns the
+ * solve's own status. */
+int rcg_server_wait(rcg_server s, int64_t ticket, rcg_solve_result* result, double* evals,
+                    double* res, int64_t cap, double* times);
+/* The ticket's eigenvectors, resident on the device (owned by the server
+ * until rcg_server_release). */
+int rcg_server_vectors(rcg_server s, int64_t ticket, rcg_block* evecs);
+int rcg_server_release(rcg_server s, int64_t ticket);
+/* stats[6]: completed, filter batches, requests in them, HBM in use, HBM
+ * peak, queued. */
+int rcg_server_stats(rcg_server s, int64_t* stats);
+/* Finishes the running solves, drops queued ones. */
+int rcg_server_destroy(rcg_server s);
 
 /* Sizes of the two structs (ABI check for bindings). */
 int64_t rcg_sizeof_solve_params(void);

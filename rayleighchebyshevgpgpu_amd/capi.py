@@ -90,6 +90,13 @@ def lib() -> C.CDLL:
         "rcg_residuals": ([vp, vp, vp, i64, vp, vp], i32),
         "rcg_lanczos_bounds": ([vp, d, i32, i64, u64, P(u64), P(d), P(d), P(i64)], i32),
         "rcg_solve_lowest": ([vp, P(SolveParams), P(SolveResult), vp, vp, i64, vp, vp, i64], i32),
+        "rcg_server_create": ([vp, i64, i32, i32, P(vp)], i32),
+        "rcg_server_submit": ([vp, vp, P(SolveParams), P(i64)], i32),
+        "rcg_server_wait": ([vp, i64, P(SolveResult), vp, vp, i64, vp], i32),
+        "rcg_server_vectors": ([vp, i64, P(vp)], i32),
+        "rcg_server_release": ([vp, i64], i32),
+        "rcg_server_stats": ([vp, vp], i32),
+        "rcg_server_destroy": ([vp], i32),
         "rcg_sizeof_solve_params": ([], i64),
         "rcg_sizeof_solve_result": ([], i64),
     }

@@ -55,6 +55,35 @@ struct CsrSched {
   int remap = 1;               // 1: each XCD takes a contiguous range of logical blocks
 };
 
+// ---- batched filter steps over many CSR operators (the serving engine)
+// One request: an operator and its ping-pong blocks.  dtype of val/buf/R/kap
+// is double (plain FP64 filter) or float (FP32 correction steps).
+struct BatchDesc {
+  const int* rowptr;
+  const int* col;
+  const void* val;
+  void* buf[2];
+  const void* R;    // correction source block (FP32 steps only)
+  const void* kap;  // correction coefficient rows, row j at kap + j*ld
+  int64_t n;
+  int ld;           // real elements per row
+  int ncv;          // 16-byte column vectors per row
+  int m;            // filter degree
+  int rows_per_blk;
+  double g1, g2;
+};
+struct BatchTask {
+  int req;
+  int blk;
+};
+// Launch class (lanes per row, vectors per lane) of a request with ncv
+// 16-byte vectors per row; requests of one class share a launch.
+int csr_class(int ncv);
+// Step j of every task's request (all tasks of one csr_class); src: FP32
+// correction steps (j >= 2).  Descriptors and tasks live in device memory.
+void launch_cheb_batch(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j,
+                       bool f32, bool src, bool cx, int cls, hipStream_t st);
+
 // out = alpha*A*X + beta*out over `ncols` columns (ld shared by X and out).
 // Complex operators (cx): val holds interleaved (re, im) pairs, the blocks
 // interleave each complex column as two real columns, and ncols counts real

@@ -124,25 +124,19 @@ constexpr int kStageNz = 1024;
 // its gathers, so that computing a row costs a single trip to memory.
 // CX: complex operator and block (interleaved re, im): each 16-byte vector
 // holds W/2 complex entries and the gather multiplies complex values.
+// Rows [i0, i1) of one step (the body shared by the single-operator and the
+// batched kernels); s_rp / s_col / s_val are the workgroup's LDS stages.
 template <typename T, int G, int NV, int MODE, int RPI, bool CX>
-__global__ __launch_bounds__(kThreads) void csr_block_kernel(
+__device__ __forceinline__ void csr_rows(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
-    T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk, const int* __restrict__ order,
-    int remap, const T* __restrict__ R, const T* __restrict__ kap) {
+    T p0, T p1, T p2, int64_t i0, int64_t i1, const T* __restrict__ R,
+    const T* __restrict__ kap, int* s_rp, int* s_col, T* s_val) {
   using VT = V16<T>;
   constexpr int W = VT::W;
   constexpr int RS = kThreads / G;
-  __shared__ int s_rp[kMaxRpb + 1];
-  __shared__ int s_col[kStageNz];
-  __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
-  const int64_t lq = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-  if (lq >= nblk) return;  // uniform across the workgroup
-  const int64_t lb = order ? (int64_t)order[lq] : lq;
   const int g = threadIdx.x % G;
   const int r = threadIdx.x / G;
-  const int64_t i0 = lb * rows_per_blk;
-  const int64_t i1 = min(n, i0 + rows_per_blk);
   const int nr = (int)(i1 - i0);
   for (int t = threadIdx.x; t <= nr; t += kThreads) s_rp[t] = rowptr[i0 + t];
   __syncthreads();
@@ -300,6 +294,70 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
         VT::st_stream(out + rowoff + cv * W, o);
       }
     }
+  }
+}
+
+template <typename T, int G, int NV, int MODE, int RPI, bool CX>
+__global__ __launch_bounds__(kThreads) void csr_block_kernel(
+    int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
+    T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk, const int* __restrict__ order,
+    int remap, const T* __restrict__ R, const T* __restrict__ kap) {
+  __shared__ int s_rp[kMaxRpb + 1];
+  __shared__ int s_col[kStageNz];
+  __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
+  const int64_t lq = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  if (lq >= nblk) return;  // uniform across the workgroup
+  const int64_t lb = order ? (int64_t)order[lq] : lq;
+  const int64_t i0 = lb * rows_per_blk;
+  const int64_t i1 = min(n, i0 + rows_per_blk);
+  csr_rows<T, G, NV, MODE, RPI, CX>(n, rowptr, col, val, X, Z, out, ld, ncv, p0, p1, p2, i0, i1, R,
+                                    kap, s_rp, s_col, s_val);
+}
+
+// ---------------------------------------------------- batched filter step
+// Step j of the Chebyshev filters of many operators at once: task t is row
+// block tasks[t].blk of request tasks[t].req.  Requests hold their own CSR
+// arrays, blocks and coefficients; a request's blocks buf[0], buf[1] are the
+// recurrence's ping-pong pair (X_{j-1} in buf[(j-1)%2], X_{j-2} and X_j in
+// buf[j%2]; the first step reads buf[0] and writes buf[1]).  SRC: the FP32
+// correction steps (j >= 2 only) with the source block R and the kap table
+// (row j at kap + j*ld).  Workgroups stride over the tasks.
+template <typename T, int G, int NV, bool SRC, bool CX>
+__global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __restrict__ descs,
+                                                             const BatchTask* __restrict__ tasks,
+                                                             int ntasks, int j) {
+  __shared__ int s_rp[kMaxRpb + 1];
+  __shared__ int s_col[kStageNz];
+  __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
+  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    const BatchTask tk = tasks[t];
+    const BatchDesc& d = descs[tk.req];
+    const int64_t i0 = (int64_t)tk.blk * d.rows_per_blk;
+    const int64_t i1 = min(d.n, i0 + d.rows_per_blk);
+    const T* val = static_cast<const T*>(d.val);
+    T* b0 = static_cast<T*>(d.buf[0]);
+    T* b1 = static_cast<T*>(d.buf[1]);
+    const T g1 = T(d.g1), g2 = T(d.g2);
+    if constexpr (SRC) {
+      T* xin = (j % 2 == 0) ? b1 : b0;
+      T* xz = (j % 2 == 0) ? b0 : b1;
+      csr_rows<T, G, NV, kChebSrc, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, g1,
+                                          g2, T(1), i0, i1, static_cast<const T*>(d.R),
+                                          static_cast<const T*>(d.kap) + (int64_t)j * d.ld, s_rp,
+                                          s_col, s_val);
+    } else if (j == 1) {
+      csr_rows<T, G, NV, kChebFirst, 1, CX>(d.n, d.rowptr, d.col, val, b0, nullptr, b1, d.ld,
+                                            d.ncv, g1, g2, T(d.m == 1 ? 0.5 : 1.0), i0, i1,
+                                            nullptr, nullptr, s_rp, s_col, s_val);
+    } else {
+      T* xin = (j % 2 == 0) ? b1 : b0;
+      T* xz = (j % 2 == 0) ? b0 : b1;
+      const T sc = (j == d.m) ? T(1.0 / double(d.m + 1)) : T(1);
+      csr_rows<T, G, NV, kCheb, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, g1, g2,
+                                       sc, i0, i1, nullptr, nullptr, s_rp, s_col, s_val);
+    }
+    __syncthreads();  // the LDS stages are reused by the next task
   }
 }
 
@@ -1269,6 +1327,52 @@ void launch_dense_step(int64_t n, const T* A, int64_t lda, const T* X, const T* 
   }
 #undef RCG_DENSE
   RCG_HIP(hipGetLastError());
+}
+
+int csr_class(int ncv) {
+  const int lim[] = {1, 2, 4, 8, 16, 32, 64, 128, 256};
+  for (int c = 0; c < 9; ++c)
+    if (ncv <= lim[c]) return c;
+  throw Error(6, "CSR block kernels support at most 256 16-byte column vectors per row");
+}
+
+template <typename T, bool SRC, bool CX>
+void cheb_batch_t(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, int cls,
+                  hipStream_t st) {
+  const unsigned grid = (unsigned)std::min(ntasks, 8192);
+#define RCG_BATCH_CASE(C, GG, NN)                                                                 \
+  case C:                                                                                         \
+    hipLaunchKernelGGL((csr_batch_kernel<T, GG, NN, SRC, CX>), dim3(grid), dim3(kThreads), 0, st, \
+                       descs, tasks, ntasks, j);                                                  \
+    break;
+  switch (cls) {
+    RCG_BATCH_CASE(0, 1, 1)
+    RCG_BATCH_CASE(1, 2, 1)
+    RCG_BATCH_CASE(2, 4, 1)
+    RCG_BATCH_CASE(3, 8, 1)
+    RCG_BATCH_CASE(4, 16, 1)
+    RCG_BATCH_CASE(5, 32, 1)
+    RCG_BATCH_CASE(6, 64, 1)
+    RCG_BATCH_CASE(7, 64, 2)
+    RCG_BATCH_CASE(8, 64, 4)
+    default: throw Error(1, "bad CSR launch class");
+  }
+#undef RCG_BATCH_CASE
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_cheb_batch(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, bool f32,
+                       bool src, bool cx, int cls, hipStream_t st) {
+  if (ntasks <= 0) return;
+  if (f32) {
+    if (!src) throw Error(1, "FP32 batched steps are the correction steps");
+    if (cx) cheb_batch_t<float, true, true>(descs, tasks, ntasks, j, cls, st);
+    else cheb_batch_t<float, true, false>(descs, tasks, ntasks, j, cls, st);
+  } else {
+    if (src) throw Error(1, "FP64 batched steps are the plain recurrence");
+    if (cx) cheb_batch_t<double, false, true>(descs, tasks, ntasks, j, cls, st);
+    else cheb_batch_t<double, false, false>(descs, tasks, ntasks, j, cls, st);
+  }
 }
 
 template <typename T>

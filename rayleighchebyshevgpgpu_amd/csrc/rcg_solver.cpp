@@ -15,12 +15,16 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -128,6 +132,7 @@ struct rcg_operator_s {
   std::map<int64_t, Tuned> tuned;  // autotuned filter shape per (kind, ld, columns)
   DevBuf val32;                    // FP32 copy of val for the mixed-precision filter
   // dense symmetric operator (kind 1): n x n, row-major = column-major
+  std::mutex lazy_mu;              // guards the lazily built members below and above
   int kind = 0;                    // 0 CSR, 1 dense
   const double* dense = nullptr;
   int64_t lda = 0;
@@ -178,6 +183,162 @@ double wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// ---------------------------------------------------------- filter batcher
+// Collects the filter sweeps of concurrent solves (each on its own stream and
+// host thread) and runs them together: for every step j one launch per
+// launch class covers the row blocks of all requests whose degree reaches j.
+// A request's producer stream records `ready`; the batcher's stream waits on
+// it, runs the steps and records `done`, on which the producer stream waits
+// in turn -- no host synchronisation on either side.
+struct FilterReq {
+  rcg::BatchDesc desc{};
+  int cls = 0;
+  bool f32 = false, src = false, cx = false;
+  int jstart = 1;
+  hipEvent_t ready = nullptr, done = nullptr;
+  bool enqueued = false;
+};
+
+class FilterBatcher {
+ public:
+  FilterBatcher(int device, int window_us) : device_(device), window_us_(window_us) {
+    RCG_HIP(hipSetDevice(device_));
+    RCG_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    RCG_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming));
+    thr_ = std::thread([this] { loop(); });
+  }
+  ~FilterBatcher() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    if (thr_.joinable()) thr_.join();
+    (void)hipStreamSynchronize(st_);
+    (void)hipEventDestroy(uploaded_);
+    (void)hipStreamDestroy(st_);
+  }
+  // Solves that may submit; the batcher launches as soon as all of them wait.
+  void attach() {
+    std::lock_guard<std::mutex> g(mu_);
+    ++members_;
+  }
+  void detach() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      --members_;
+    }
+    cv_.notify_all();
+  }
+  // Blocks until the request's steps are enqueued (not until they finish).
+  void submit(FilterReq* r) {
+    std::unique_lock<std::mutex> g(mu_);
+    if (!error_.empty()) throw Error(RCG_ERR_HIP, error_);
+    pending_.push_back(r);
+    cv_.notify_all();
+    cv_done_.wait(g, [&] { return r->enqueued || !error_.empty(); });
+    if (!r->enqueued) throw Error(RCG_ERR_HIP, error_);
+  }
+  int64_t batches() const { return batches_; }
+  int64_t requests() const { return requests_; }
+
+ private:
+  void loop() {
+    (void)hipSetDevice(device_);
+    std::unique_lock<std::mutex> g(mu_);
+    while (true) {
+      cv_.wait(g, [&] { return stop_ || !pending_.empty(); });
+      if (stop_ && pending_.empty()) return;
+      // gather: until every attached solve waits here, or the window closes
+      const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(window_us_);
+      cv_.wait_until(g, until, [&] { return stop_ || (int)pending_.size() >= members_; });
+      std::vector<FilterReq*> batch;
+      batch.swap(pending_);
+      g.unlock();
+      std::string err;
+      try {
+        run(batch);
+      } catch (const std::exception& e) {
+        err = e.what();
+      }
+      g.lock();
+      if (!err.empty()) error_ = err;
+      for (FilterReq* r : batch) r->enqueued = err.empty();
+      ++batches_;
+      requests_ += (int64_t)batch.size();
+      cv_done_.notify_all();
+    }
+  }
+
+  void run(std::vector<FilterReq*>& batch) {
+    for (FilterReq* r : batch) RCG_HIP(hipStreamWaitEvent(st_, r->ready, 0));
+    // groups of one launch kind, requests by degree descending
+    std::stable_sort(batch.begin(), batch.end(), [](const FilterReq* a, const FilterReq* b) {
+      const int ka = (a->f32 ? 1000 : 0) + (a->cx ? 100 : 0) + a->cls;
+      const int kb = (b->f32 ? 1000 : 0) + (b->cx ? 100 : 0) + b->cls;
+      if (ka != kb) return ka < kb;
+      return a->desc.m > b->desc.m;
+    });
+    std::vector<rcg::BatchTask> tasks;
+    std::vector<int> first_task(batch.size() + 1, 0);
+    for (size_t q = 0; q < batch.size(); ++q) {
+      first_task[q] = (int)tasks.size();
+      const rcg::BatchDesc& d = batch[q]->desc;
+      const int nb = (int)((d.n + d.rows_per_blk - 1) / d.rows_per_blk);
+      for (int b = 0; b < nb; ++b) tasks.push_back({(int)q, b});
+    }
+    first_task[batch.size()] = (int)tasks.size();
+    const size_t db = sizeof(rcg::BatchDesc) * batch.size();
+    const size_t tb = sizeof(rcg::BatchTask) * tasks.size();
+    RCG_HIP(hipEventSynchronize(uploaded_));  // the staging copy of the previous batch is done
+    // growing the device tables frees them: the previous batch's steps must be done
+    if (db > ddesc_.bytes || tb > dtask_.bytes) RCG_HIP(hipStreamSynchronize(st_));
+    hdesc_.ensure(db);
+    htask_.ensure(tb);
+    ddesc_.ensure(db);
+    dtask_.ensure(tb);
+    for (size_t q = 0; q < batch.size(); ++q) hdesc_.as<rcg::BatchDesc>()[q] = batch[q]->desc;
+    std::memcpy(htask_.p, tasks.data(), tb);
+    RCG_HIP(hipMemcpyAsync(ddesc_.p, hdesc_.p, db, hipMemcpyHostToDevice, st_));
+    RCG_HIP(hipMemcpyAsync(dtask_.p, htask_.p, tb, hipMemcpyHostToDevice, st_));
+    RCG_HIP(hipEventRecord(uploaded_, st_));
+    const rcg::BatchTask* dt = dtask_.as<rcg::BatchTask>();
+    for (size_t g0 = 0; g0 < batch.size();) {
+      size_t g1 = g0;
+      const FilterReq* h = batch[g0];
+      while (g1 < batch.size() && batch[g1]->f32 == h->f32 && batch[g1]->cx == h->cx &&
+             batch[g1]->cls == h->cls)
+        ++g1;
+      // requests g0..g1-1 by degree descending: step j covers a prefix
+      const int mmax = batch[g0]->desc.m;
+      size_t end = g1;
+      for (int j = h->jstart; j <= mmax; ++j) {
+        while (end > g0 && batch[end - 1]->desc.m < j) --end;
+        const int nt = first_task[end] - first_task[g0];
+        rcg::launch_cheb_batch(ddesc_.as<rcg::BatchDesc>(), dt + first_task[g0], nt, j, h->f32,
+                               h->src, h->cx, h->cls, st_);
+      }
+      g0 = g1;
+    }
+    for (FilterReq* r : batch) RCG_HIP(hipEventRecord(r->done, st_));
+  }
+
+  int device_;
+  int window_us_;
+  hipStream_t st_ = nullptr;
+  hipEvent_t uploaded_ = nullptr;
+  std::thread thr_;
+  std::mutex mu_;
+  std::condition_variable cv_, cv_done_;
+  std::vector<FilterReq*> pending_;
+  int members_ = 0;
+  bool stop_ = false;
+  std::string error_;
+  int64_t batches_ = 0, requests_ = 0;
+  DevBuf ddesc_, dtask_;
+  HostBuf hdesc_, htask_;
+};
+
 // ------------------------------------------------------------------ engine
 // Device operations of the solve over attached row-interleaved buffers:
 // V (the subspace, ld), T (second recurrence / rotation buffer, ld),
@@ -194,6 +355,7 @@ class Engine {
   }
   bool cx() const { return cx_; }
   int cw() const { return cw_; }
+  rcg_context_s* context() const { return ctx_; }
 
   // Size the context's workspaces for widths up to bmax and kmax.
   void reserve(int64_t bmax, int64_t kmax) {
@@ -291,6 +453,7 @@ class Engine {
   // override the choice.
   template <class Launch>
   rcg_operator_s::Tuned tuned_for(int64_t b, int kind, Launch&& launch) {
+    std::lock_guard<std::mutex> lk(op_->lazy_mu);
     build_schedules();
     const int64_t key = (int64_t)kind * 1000000000000LL + ld_ * 100000 + b;
     auto it = op_->tuned.find(key);
@@ -342,8 +505,16 @@ class Engine {
     }
     return t;
   }
-  // b counts real columns here.
+  // b counts real columns here.  Timing writes Chebyshev steps into T, so an
+  // engine without a T block (the piecewise residual / apply entry points)
+  // never tunes: it uses the cached choice or the contiguous default.
   rcg_operator_s::Tuned tuned_for(int64_t b) {
+    if (!T() || !V()) {
+      std::lock_guard<std::mutex> lk(op_->lazy_mu);
+      build_schedules();
+      auto it = op_->tuned.find((int64_t)(cx_ ? 2 : 0) * 1000000000000LL + ld_ * 100000 + b);
+      return it != op_->tuned.end() ? it->second : rcg_operator_s::Tuned{(int)b, 0};
+    }
     return tuned_for(b, cx_ ? 2 : 0, [&](int panel, const rcg::CsrSched* sc) {
       rcg::launch_cheb_step<double>(n_, op_->rowptr, op_->col, op_->val, V(), T(), T(), (int)ld_,
                                     (int)b, 0.0, 1.0, 0.0, false, st_, panel, sc, cx_);
@@ -365,7 +536,7 @@ class Engine {
                                   (int)(cw_ * d0), (int)(cw_ * cnt), st_);
   }
   const rcg::CsrSched* sched_for(int64_t b) {
-    if (!op_ || n_ * b < (int64_t)1 << 22) return nullptr;
+    if (!op_ || batcher_ || n_ * b < (int64_t)1 << 22) return nullptr;
     return &op_->scheds[tuned_for(b).sched]->sc;
   }
 
@@ -373,6 +544,7 @@ class Engine {
   // X1 = g2(g1 A X0 + X0), Xj = g2(g1 A X_{j-1} + X_{j-1}) - X_{j-2}, out = Xm/(m+1).
   void filter(int64_t b, long m, long rep, double lmax, double shift) {
     if (op_->is_dense()) return filter_dense(b, m, rep, lmax, shift);
+    if (batcher_) return filter_batched(b, m, rep, lmax, shift);
     const auto c = rcg::cpoly::coefficients(m, lmax, shift);
     const int64_t rb = cw_ * b;  // real columns
     const auto tn = tuned_for(rb);
@@ -401,6 +573,55 @@ class Engine {
     }
     step_end();
   }
+
+  // Serving: the steps run in the shared batcher, ordered against this
+  // engine's stream by events.
+  void batch_submit(const rcg::BatchDesc& d, bool f32, bool cx, int jstart) {
+    if (!ev_ready_) {
+      RCG_HIP(hipEventCreateWithFlags(&ev_ready_, hipEventDisableTiming));
+      RCG_HIP(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming));
+    }
+    FilterReq r;
+    r.desc = d;
+    r.f32 = f32;
+    r.src = f32;
+    r.cx = cx;
+    r.jstart = jstart;
+    r.cls = rcg::csr_class(d.ncv);
+    r.ready = ev_ready_;
+    r.done = ev_done_;
+    RCG_HIP(hipEventRecord(ev_ready_, st_));
+    batcher_->submit(&r);
+    RCG_HIP(hipStreamWaitEvent(st_, ev_done_, 0));
+  }
+  rcg::BatchDesc batch_desc(long m, double g1, double g2, int64_t rb, int w) {
+    rcg::BatchDesc d{};
+    d.rowptr = op_->rowptr;
+    d.col = op_->col;
+    d.n = n_;
+    d.ld = (int)ld_;
+    d.ncv = (int)((rb + w - 1) / w);
+    d.m = (int)m;
+    d.rows_per_blk = 128;
+    d.g1 = g1;
+    d.g2 = g2;
+    return d;
+  }
+  void filter_batched(int64_t b, long m, long rep, double lmax, double shift) {
+    const auto c = rcg::cpoly::coefficients(m, lmax, shift);
+    const int64_t rb = cw_ * b;
+    last_panel_ = (int)rb;
+    last_sched_ = "batched";
+    for (long r = 0; r < rep; ++r) {
+      rcg::BatchDesc d = batch_desc(m, c.g1, c.g2, rb, 2);
+      d.val = op_->val;
+      d.buf[0] = V();
+      d.buf[1] = T();
+      batch_submit(d, false, cx_, 1);
+      if (m % 2 == 1) swapVT();
+    }
+  }
+  void set_batcher(FilterBatcher* b) { batcher_ = b; }
 
   // The same recurrence with the dense MFMA step.
   void filter_dense(int64_t b, long m, long rep, double lmax, double shift) {
@@ -478,15 +699,15 @@ class Engine {
     const float g1f = float(c.g1), g2f = float(c.g2);
     const bool dn = op_->is_dense();
     rcg_operator_s::Tuned tn{(int)rb, 0};
-    if (!dn)
+    if (!dn && !batcher_)
       tn = tuned_for(rb, cx_ ? 3 : 1, [&](int panel, const rcg::CsrSched* sc) {
         rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, E_[0], E_[1], E_[1], R32_,
                                   dk + 2 * ld_, (int)ld_, (int)rb, g1f, g2f, 1.0f, st_, panel, sc,
                                   cx_);
       });
-    const rcg::CsrSched* sc = dn ? nullptr : &op_->scheds[tn.sched]->sc;
+    const rcg::CsrSched* sc = (dn || batcher_) ? nullptr : &op_->scheds[tn.sched]->sc;
     last_panel_ = tn.panel;
-    last_sched_ = dn ? "dense" : op_->scheds[tn.sched]->name;
+    last_sched_ = dn ? "dense" : (batcher_ ? "batched" : op_->scheds[tn.sched]->name);
     const double csr = dn ? 4.0 * double(n_) * double(n_)
                           : double(op_->nnz) * (4.0 + 4.0 * cw_) + 4.0 * double(n_ + 1);
     const double blk = 4.0 * double(n_) * double(rb);
@@ -494,6 +715,18 @@ class Engine {
       apply(V(), W_, b);
       rcg::launch_mixed_start(n_, V(), W_, (int)ld_, (int)rb, dcp, g12, c.g1, c.g2, E_[0], E_[1],
                               R32_, st_);
+      if (batcher_ && !dn) {
+        rcg::BatchDesc d = batch_desc(m, c.g1, c.g2, rb, 4);
+        d.val = val32;
+        d.buf[0] = E_[0];
+        d.buf[1] = E_[1];
+        d.R = R32_;
+        d.kap = dk;
+        batch_submit(d, true, cx_, 2);
+        rcg::launch_mixed_combine(n_, V(), (int)ld_, (int)rb, dum, 1.0 / double(m + 1),
+                                  E_[m % 2], st_);
+        continue;
+      }
       float* X0 = E_[0];
       float* X1 = E_[1];
       step_begin((m - 1) * ((rb + tn.panel - 1) / tn.panel), double(m - 1) * (csr + 4.0 * blk));
@@ -517,10 +750,14 @@ class Engine {
     R32_ = R;
   }
   const float* values_f32() {
+    std::lock_guard<std::mutex> lk(op_->lazy_mu);
     if (!op_->val32.p) {
+      // dense operators hold their FP64 matrix in `dense` (lda x n), CSR ones in `val`
+      const double* src = op_->is_dense() ? op_->dense : op_->val;
       const int64_t nv = op_->is_dense() ? op_->lda * n_ : cw_ * op_->nnz;
+      if (!src) throw Error(RCG_ERR_INVALID, "operator has no values to convert");
       op_->val32.ensure(sizeof(float) * (size_t)std::max<int64_t>(nv, 1));
-      rcg::launch_to_f32(nv, op_->val, op_->val32.as<float>(), st_);
+      rcg::launch_to_f32(nv, src, op_->val32.as<float>(), st_);
     }
     return op_->val32.as<float>();
   }
@@ -558,6 +795,8 @@ class Engine {
   ~Engine() {
     for (auto& e : fev_)
       if (e) (void)hipEventDestroy(e);
+    if (ev_ready_) (void)hipEventDestroy(ev_ready_);
+    if (ev_done_) (void)hipEventDestroy(ev_done_);
   }
   int last_panel() const { return last_panel_; }
   const std::string& last_sched() const { return last_sched_; }
@@ -864,6 +1103,8 @@ class Engine {
   DevBuf coef_;                       // mixed filter coefficient table
   HostBuf hcoef_;
   hipEvent_t fev_[2] = {nullptr, nullptr};
+  FilterBatcher* batcher_ = nullptr;
+  hipEvent_t ev_ready_ = nullptr, ev_done_ = nullptr;
   bool step_pending_ = false;
   long pend_launches_ = 0, step_launches_ = 0;
   double pend_bytes_ = 0.0, step_bytes_ = 0.0, step_seconds_ = 0.0;
@@ -876,7 +1117,11 @@ struct TraceRow {
 
 class Solver {
  public:
-  Solver(rcg_operator_s* op, const rcg_solve_params& p) : op_(op), p_(p), eng_(op->ctx, op, op->n) {}
+  Solver(rcg_operator_s* op, const rcg_solve_params& p, rcg_context_s* ctx = nullptr,
+         FilterBatcher* batcher = nullptr)
+      : op_(op), p_(p), eng_(ctx ? ctx : op->ctx, op, op->n) {
+    eng_.set_batcher(batcher);
+  }
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
 
@@ -1259,7 +1504,7 @@ class Solver {
     DevBuf AY, T2;
     AY.ensure(bytes);
     T2.ensure(bytes);
-    Engine e2(op_->ctx, op_, n);
+    Engine e2(eng_.context(), op_, n);
     e2.reserve(ldy, ldy);
     e2.attach(eng_.Y(), T2.as<double>(), AY.as<double>(), ldy, eng_.Y(), ldy);
     std::vector<S> H, Z;
@@ -1297,6 +1542,124 @@ class Solver {
 };
 
 // allow_cx: a complex operator with complex (C128) blocks is accepted too.
+// ------------------------------------------------------------------ server
+// Continuous batching of lowest-eigenpair solves.  Requests queue in arrival
+// order; a worker admits the queue head when a worker slot is free and the
+// head's device footprint fits the HBM budget next to the running solves.
+// Each worker has its own stream and scratch (a context of its own) and runs
+// the solve's state machine; the filter sweeps of all running solves go
+// through one FilterBatcher, so solves that arrive while others run share
+// their step launches.  Eigenvectors stay on the device in a server-owned
+// block until the ticket is released.
+struct Ticket {
+  int64_t id = 0;
+  rcg_operator_s* op = nullptr;
+  rcg_solve_params p{};
+  int64_t footprint = 0;
+  int state = 0;  // 0 queued, 1 running, 2 done
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  rcg_solve_result res{};
+  std::vector<double> evals, resid;
+  rcg_block_s* evecs = nullptr;
+  int status = RCG_OK;
+  std::string error;
+};
+
+int64_t solve_footprint(const rcg_operator_s* op, const rcg_solve_params& p) {
+  const int64_t n = op->n;
+  const int cw = op->dtype == RCG_C128 ? 2 : 1;
+  int64_t sub = std::min<int64_t>(p.inc + p.buf, n);
+  const int64_t ld = cw * block_ld(sub);
+  const int64_t ldy = cw * block_ld(std::max<int64_t>(std::min<int64_t>(p.k, n), sub));
+  int64_t bytes = 8 * n * (3 * ld + 2 * ldy);  // V, T, W, Y and the caller's copy of Y
+  if (p.filter_dtype == RCG_F32) bytes += 4 * 3 * n * ld;
+  if (p.refine || p.filter_dtype == RCG_F32) bytes += 8 * 2 * n * ldy;  // refinement blocks
+  return bytes;
+}
+
+}  // namespace
+
+struct rcg_server_s {
+  rcg_context_s* ctx = nullptr;
+  int64_t budget = 0, used = 0, peak = 0;
+  int max_active = 1;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Ticket*> queue;
+  std::map<int64_t, std::unique_ptr<Ticket>> tickets;
+  int64_t next_id = 1;
+  bool stop = false;
+  std::unique_ptr<FilterBatcher> batcher;
+  std::vector<std::unique_ptr<rcg_context_s>> wctx;
+  std::vector<std::thread> workers;
+  int64_t completed = 0;
+};
+
+namespace {
+
+double now_s() { return wall(); }
+
+void server_worker(rcg_server_s* S, int w) {
+  (void)hipSetDevice(S->ctx->device);
+  rcg_context_s* ctx = S->wctx[w].get();
+  while (true) {
+    Ticket* t = nullptr;
+    {
+      std::unique_lock<std::mutex> g(S->mu);
+      S->cv.wait(g, [&] {
+        return S->stop || (!S->queue.empty() && (S->used + S->queue.front()->footprint <= S->budget ||
+                                                 S->used == 0));
+      });
+      if (S->stop) return;
+      t = S->queue.front();
+      S->queue.pop_front();
+      S->used += t->footprint;
+      S->peak = std::max(S->peak, S->used);
+      t->state = 1;
+      t->t_admit = now_s();
+      S->batcher->attach();
+    }
+    try {
+      Solver s(t->op, t->p, ctx, S->batcher.get());
+      std::vector<double> ev, rr;
+      s.run(&t->res, ev, rr);
+      t->res.trace_len = (int64_t)s.trace_.size();
+      t->res.rng_counter = s.counter_;
+      t->evals = ev;
+      t->resid = rr;
+      if (t->res.found > 0) {
+        auto b = std::make_unique<rcg_block_s>();
+        b->ctx = ctx;  // filled on the worker stream, then handed to the server context
+        b->n = t->op->n;
+        b->cols = t->res.found;
+        b->ld = block_ld(b->cols);
+        b->dtype = t->op->dtype;
+        b->mem.ensure(rcg::dtype_size(b->dtype) * (size_t)b->n * (size_t)b->ld);
+        s.export_vectors(b.get());
+        RCG_HIP(hipStreamSynchronize(ctx->stream));
+        b->ctx = S->ctx;
+        t->evecs = b.release();
+      }
+      RCG_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (const Error& e) {
+      t->status = e.code;
+      t->error = e.what();
+    } catch (const std::exception& e) {
+      t->status = RCG_ERR_INVALID;
+      t->error = e.what();
+    }
+    S->batcher->detach();
+    {
+      std::lock_guard<std::mutex> g(S->mu);
+      S->used -= t->footprint;
+      t->state = 2;
+      t->t_done = now_s();
+      ++S->completed;
+    }
+    S->cv.notify_all();
+  }
+}
+
 void check_pair(rcg_operator op, rcg_block a, rcg_block b, int64_t cols, bool allow_cx = false) {
   need(op && a && b, "NULL argument");
   need(a->n == op->n && b->n == op->n, "block row count differs from the operator");
@@ -1797,5 +2160,147 @@ int rcg_solve_lowest(rcg_operator op, const rcg_solve_params* params, rcg_solve_
 
 int64_t rcg_sizeof_solve_params(void) { return (int64_t)sizeof(rcg_solve_params); }
 int64_t rcg_sizeof_solve_result(void) { return (int64_t)sizeof(rcg_solve_result); }
+
+int rcg_server_create(rcg_context ctx, int64_t hbm_budget, int32_t max_active, int32_t window_us,
+                      rcg_server* out) {
+  return guarded([&] {
+    need(ctx && out, "NULL argument");
+    need(hbm_budget > 0 && max_active >= 1 && max_active <= 1024 && window_us >= 0,
+         "bad server limits");
+    DeviceGuard g(ctx->device);
+    auto S = std::make_unique<rcg_server_s>();
+    S->ctx = ctx;
+    S->budget = hbm_budget;
+    S->max_active = max_active;
+    S->batcher = std::make_unique<FilterBatcher>(ctx->device, window_us);
+    for (int w = 0; w < max_active; ++w) {
+      auto c = std::make_unique<rcg_context_s>();
+      c->device = ctx->device;
+      RCG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      c->own_stream = true;
+      S->wctx.push_back(std::move(c));
+    }
+    rcg_server_s* raw = S.get();
+    for (int w = 0; w < max_active; ++w) S->workers.emplace_back(server_worker, raw, w);
+    *out = S.release();
+  });
+}
+
+int rcg_server_submit(rcg_server s, rcg_operator op, const rcg_solve_params* params,
+                      int64_t* ticket) {
+  return guarded([&] {
+    need(s && op && params && ticket, "NULL argument");
+    need(params->k >= 1, "eigCount must be >= 1");
+    need(params->inc >= 0 && params->buf >= 0 && params->inc + params->buf >= 1,
+         "subspace increment + buffer must be >= 1");
+    need(params->filter_dtype == RCG_F64 || params->filter_dtype == RCG_F32,
+         "filter_dtype must be RCG_F64 or RCG_F32");
+    need(params->nonrandom_start == 0, "served solves start from random vectors");
+    need(op->ctx->device == s->ctx->device, "operator on another device");
+    auto t = std::make_unique<Ticket>();
+    t->op = op;
+    t->p = *params;
+    t->footprint = solve_footprint(op, *params);
+    t->t_submit = now_s();
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      t->id = s->next_id++;
+      *ticket = t->id;
+      s->queue.push_back(t.get());
+      s->tickets[t->id] = std::move(t);
+    }
+    s->cv.notify_all();
+  });
+}
+
+int rcg_server_wait(rcg_server s, int64_t ticket, rcg_solve_result* result, double* evals,
+                    double* res, int64_t cap, double* times) {
+  int status = RCG_OK;
+  std::string err;
+  int rc = guarded([&] {
+    need(s && result, "NULL argument");
+    std::unique_lock<std::mutex> g(s->mu);
+    auto it = s->tickets.find(ticket);
+    need(it != s->tickets.end(), "unknown ticket");
+    Ticket* t = it->second.get();
+    s->cv.wait(g, [&] { return t->state == 2; });
+    *result = t->res;
+    for (int64_t i = 0; i < (int64_t)t->evals.size() && i < cap; ++i) {
+      if (evals) evals[i] = t->evals[i];
+      if (res) res[i] = t->resid[i];
+    }
+    if (times) {
+      times[0] = t->t_submit;
+      times[1] = t->t_admit;
+      times[2] = t->t_done;
+    }
+    status = t->status;
+    err = t->error;
+  });
+  if (rc != RCG_OK) return rc;
+  if (status != RCG_OK) g_last_error = err;
+  return status;
+}
+
+int rcg_server_vectors(rcg_server s, int64_t ticket, rcg_block* evecs) {
+  return guarded([&] {
+    need(s && evecs, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->tickets.find(ticket);
+    need(it != s->tickets.end(), "unknown ticket");
+    need(it->second->state == 2, "the solve has not finished");
+    *evecs = it->second->evecs;
+  });
+}
+
+int rcg_server_release(rcg_server s, int64_t ticket) {
+  return guarded([&] {
+    need(s != nullptr, "NULL argument");
+    std::unique_ptr<Ticket> t;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      auto it = s->tickets.find(ticket);
+      need(it != s->tickets.end(), "unknown ticket");
+      need(it->second->state == 2, "the solve has not finished");
+      t = std::move(it->second);
+      s->tickets.erase(it);
+    }
+    if (t->evecs) delete t->evecs;
+  });
+}
+
+int rcg_server_stats(rcg_server s, int64_t* stats) {
+  return guarded([&] {
+    need(s && stats, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    stats[0] = s->completed;
+    stats[1] = s->batcher->batches();
+    stats[2] = s->batcher->requests();
+    stats[3] = s->used;
+    stats[4] = s->peak;
+    stats[5] = (int64_t)s->queue.size();
+  });
+}
+
+int rcg_server_destroy(rcg_server s) {
+  return guarded([&] {
+    if (!s) return;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->stop = true;
+    }
+    s->cv.notify_all();
+    for (auto& th : s->workers)
+      if (th.joinable()) th.join();
+    s->batcher.reset();
+    for (auto& kv : s->tickets)
+      if (kv.second->evecs) delete kv.second->evecs;
+    for (auto& c : s->wctx) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipStreamDestroy(c->stream);
+    }
+    delete s;
+  });
+}
 
 }  // extern "C"

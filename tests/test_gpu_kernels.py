@@ -133,6 +133,33 @@ def test_deflate(rcg):
     assert np.max(np.abs(Z - P)) < 1e-13
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_residuals_on_untuned_wide_block(rcg, cplx):
+    """Residuals / apply through the piecewise entry points on a fresh operator at a
+    width large enough for the filter autotuner (n * columns >= 2^22): the entry
+    points hold no scratch block, so they must run untuned."""
+    print(f"case: residuals on a fresh 40^3 operator, 112 columns, complex={cplx}", flush=True)
+    if cplx:
+        A = ops.hofstadter_2d(256, phi=1 / 64, W=1.0, seed=2)
+    else:
+        A = anderson3d(40)
+    cols = 112
+    S = A.to_scipy()
+    op = rcg.CsrOperator(A)
+    rng = np.random.default_rng(5)
+    V = rng.standard_normal((A.n, cols))
+    if cplx:
+        V = V + 1j * rng.standard_normal((A.n, cols))
+    bv = rcg.Block.from_host(V)
+    bw = rcg.Block(A.n, cols, dtype=bv.dtype)
+    th = rng.standard_normal(cols)
+    res = np.zeros(cols)
+    capi.check(capi.lib().rcg_residuals(op.h, bv.h, bw.h, cols, th.ctypes.data_as(C.c_void_p),
+                                        res.ctypes.data_as(C.c_void_p)))
+    rref = np.linalg.norm(th * V - S @ V, axis=0)
+    assert np.max(np.abs(res - rref) / rref) < 1e-12
+
+
 def test_project_rotate_residuals(rcg):
     print("case: projection, rotation and residuals on 12^3 Anderson, b=20", flush=True)
     A = anderson3d(12)

@@ -1,0 +1,130 @@
+"""The continuous-batching solve server (rcg_server_*).
+
+Pins: each request's result is the same alone (one worker, nothing to batch
+with) and in the mix (many workers, shared filter batches) -- the batched
+step computes every row exactly as the single-operator step does; served
+eigenvalues match the analytic Dirichlet spectra and the FP64 oracle to a
+relative 1e-10; every pair's residual meets the tolerance; the HBM budget is
+never exceeded by the admitted solves; a failing request reports its own
+error without stopping the others.
+"""
+import numpy as np
+import pytest
+
+from rayleighchebyshevgpgpu_amd import operators as ops
+from rayleighchebyshevgpgpu_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rcg():
+    from rayleighchebyshevgpgpu_amd import solver
+    return solver
+
+
+def small_requests(count, seed=11):
+    reqs = serving.serving_requests(count, rate=1e6, seed=seed, n_min=300, n_max=6000)
+    return reqs
+
+
+def submit_all(srv, rcg, reqs, precision):
+    ops_, tickets = [], []
+    for r in reqs:
+        op = rcg.CsrOperator(r.operator())
+        ops_.append(op)
+        tickets.append(srv.submit(op, r.k, r.tol, r.k, max(4, r.k // 2), precision=precision))
+    return ops_, tickets
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_alone_equals_mix(rcg, precision):
+    reqs = small_requests(10)
+    print(f"case: 10 requests alone vs in the mix, {precision}", flush=True)
+    alone = []
+    srv1 = serving.Server(max_active=1, window_us=0)
+    for r in reqs:
+        op = rcg.CsrOperator(r.operator())
+        t = srv1.submit(op, r.k, r.tol, r.k, max(4, r.k // 2), precision=precision)
+        out = srv1.wait(t)
+        assert out["status"] == 0, out["error"]
+        alone.append(out)
+        srv1.release(t)
+    srv1.close()
+    srv = serving.Server(max_active=8, window_us=300)
+    ops_, tickets = submit_all(srv, rcg, reqs, precision)
+    mix = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    print(f"server stats: {st}")
+    assert st["batched_requests"] > st["filter_batches"]  # some batches held several solves
+    for r, a, m in zip(reqs, alone, mix):
+        assert m["status"] == 0, m["error"]
+        assert a["found"] == m["found"] == r.k
+        d = np.max(np.abs(a["eigenvalues"] - m["eigenvalues"]) / np.maximum(np.abs(a["eigenvalues"]), 0.1))
+        assert d <= 1e-12, (r, d)
+    srv.close()
+
+
+def test_served_results_vs_analytic_and_oracle(rcg, orc):
+    print("case: served W=0 Dirichlet Laplacians vs closed forms and the oracle", flush=True)
+    shapes = [(2, 40, 16), (3, 14, 8), (2, 64, 32), (3, 10, 16)]
+    srv = serving.Server(max_active=4, window_us=300)
+    held, tickets = [], []
+    for dim, side, k in shapes:
+        A = ops.laplacian_2d(side) if dim == 2 else ops.laplacian_3d(side, bc="dirichlet")
+        op = rcg.CsrOperator(A)
+        held.append((A, op, dim, side, k))
+        tickets.append(srv.submit(op, k, 1e-10, k, k // 2, precision="mixed"))
+    for (A, op, dim, side, k), t in zip(held, tickets):
+        out = srv.wait(t)
+        assert out["status"] == 0, out["error"]
+        ev = out["eigenvalues"]
+        exact = (ops.laplacian_2d_spectrum(side) if dim == 2
+                 else ops.laplacian_3d_spectrum(side, bc="dirichlet"))[:k]
+        assert np.max(np.abs(ev - exact) / np.maximum(exact, 0.1)) < 1e-10
+        o = orc.solve(A, k, 1e-10, k, k // 2, seed=0, stop="RESIDUAL_ONLY")
+        assert np.max(np.abs(ev - o["eigenvalues"]) / np.maximum(np.abs(ev), 0.1)) < 1e-10
+        X = srv.vectors(t, A.n, k)
+        r = np.linalg.norm(A.to_scipy() @ X - X * ev, axis=0) / (4.0 * dim)
+        assert np.max(r) <= 1e-10
+        srv.release(t)
+    srv.close()
+
+
+def test_hbm_budget_respected(rcg):
+    reqs = small_requests(12, seed=12)
+    print("case: HBM budget of about three requests, 8 worker slots", flush=True)
+    # a budget of 6 MB admits a few of these at a time
+    srv = serving.Server(max_active=8, hbm_budget=6 << 20, window_us=200)
+    ops_, tickets = submit_all(srv, rcg, reqs, "mixed")
+    for t in tickets:
+        out = srv.wait(t)
+        assert out["status"] == 0, out["error"]
+        assert out["t_submit"] <= out["t_admit"] <= out["t_done"]
+    st = srv.stats()
+    print(f"server stats: {st}")
+    fps = [serving.solve_footprint(r.n, r.k, r.k, max(4, r.k // 2)) for r in reqs]
+    # a request larger than the whole budget is admitted only when nothing else runs
+    assert st["hbm_peak"] <= max(6 << 20, max(fps))
+    assert st["hbm_peak"] >= max(fps)
+    assert st["hbm_in_use"] == 0 and st["queued"] == 0
+    srv.close()
+
+
+def test_failing_request_does_not_stop_others(rcg):
+    print("case: a non-Hermitian request fails alone; its neighbours complete", flush=True)
+    n = 300
+    rows = np.arange(n)
+    bad = ops._from_coo(n, np.concatenate([rows, rows[:-1]]), np.concatenate([rows, rows[1:]]),
+                        np.concatenate([np.ones(n), 1j * np.ones(n - 1)]).astype(complex))
+    good = ops.laplacian_2d(30)
+    srv = serving.Server(max_active=3, window_us=100)
+    ob, og1, og2 = rcg.CsrOperator(bad), rcg.CsrOperator(good), rcg.CsrOperator(good)
+    t1 = srv.submit(og1, 8, 1e-10, 8, 4)
+    t2 = srv.submit(ob, 4, 1e-8, 4, 4)
+    t3 = srv.submit(og2, 8, 1e-10, 8, 4)
+    r1, r2, r3 = srv.wait(t1), srv.wait(t2), srv.wait(t3)
+    assert r1["status"] == 0 and r3["status"] == 0
+    assert r2["status"] == 4 and "Hermitian" in r2["error"]
+    assert np.array_equal(r1["eigenvalues"], r3["eigenvalues"])
+    srv.close()
