@@ -10,9 +10,10 @@ Workload (one "step" = one whole solve, operator resident in HBM):
 
 value = converged eigenpairs (||Av - lambda v|| / ||A|| <= 1e-10, checked on
 the device after the timed region) per second of timed wall time, FP64
-filter.  The same solve with the mixed-precision filter (FP32 Chebyshev
-steps on each Ritz column's correction, FP64 for everything else and a final
-FP64 Rayleigh-Ritz refinement over the found pairs) is timed the same way
+filter, with the final FP64 Rayleigh-Ritz refinement over the found pairs.
+The same solve with the mixed-precision filter (FP32 Chebyshev steps on each
+Ritz column's correction, FP64 for everything else and the same refinement)
+is timed the same way
 and reported under "mixed", with its speed-up and its largest eigenvalue
 difference from the FP64 solve.
 
@@ -57,12 +58,46 @@ WORKLOADS = {
                            label=("2-D periodic Hofstadter lattice 1024^2, Peierls phase "
                                   "exp(2 pi i x/64), onsite U[-1,1] (seed 2), complex128 CSR"),
                            profile="hofstadter1024"),
+    "dense16384": dict(k=256, inc=128, buf=128, tol=1e-10, complex=False, dense=True,
+                       label=("dense symmetric A = Q diag(linspace(0,1,n)^2) Q^T, Q the Haar QR "
+                              "factor of a Gaussian matrix (seed 3), n=16384"),
+                       profile="dense16384"),
 }
 WL = WORKLOADS["anderson128"]
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def dense_qlq_device(n, seed=3):
+    """A = Q diag(linspace(0,1,n)^2) Q^T on the device (FP64), Q = qr(G).Q diag(sign diag R)
+    with G ~ N(0,1) by Box-Muller from stream `seed` (the same draws as
+    operators.dense_qlq).  The QR and the products run in torch on the GPU: input
+    set-up, outside any timed region."""
+    import torch
+    from rayleighchebyshevgpgpu_amd import operators as ops
+    dev = torch.device("cuda:0")
+    G = torch.empty((n, n), dtype=torch.float64, device=dev)
+    cols = max(1, (1 << 26) // n)  # columns per chunk of draws (column-major order)
+    for c0 in range(0, n, cols):
+        c1 = min(n, c0 + cols)
+        cnt = (c1 - c0) * n
+        u = ops.unit_draws(seed, c0 * n, cnt + cnt % 2)
+        u1, u2 = 1.0 - u[0::2], u[1::2]
+        r = np.sqrt(-2.0 * np.log(u1))
+        g = np.empty(u.shape[0])
+        g[0::2] = r * np.cos(2 * np.pi * u2)
+        g[1::2] = r * np.sin(2 * np.pi * u2)
+        G[:, c0:c1] = torch.from_numpy(g[:cnt].reshape(c1 - c0, n).T.copy()).to(dev)
+    Q, R = torch.linalg.qr(G)
+    Q = Q * torch.sign(torch.diagonal(R))[None, :]
+    del G, R
+    lam = torch.linspace(0.0, 1.0, n, dtype=torch.float64, device=dev) ** 2
+    A = (Q * lam[None, :]) @ Q.T
+    A = 0.5 * (A + A.T)
+    torch.cuda.synchronize()
+    return A.contiguous(), lam.cpu().numpy()
 
 
 def build_operator(grid=GRID):
@@ -266,6 +301,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--workload", default="anderson128", choices=sorted(WORKLOADS) + ["serving"])
+    ap.add_argument("--dense-n", type=int, default=16384, help="dense16384: matrix size")
     ap.add_argument("--requests", type=int, default=4096, help="serving: stream length")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
     ap.add_argument("--slots", type=int, default=64, help="serving: concurrent solves")
@@ -301,10 +337,17 @@ def main():
         return kernel_only(args)
 
     t0 = time.time()
-    A = build_operator(args.grid)
-    log(f"operator n={A.n} nnz={A.nnz} built in {time.time() - t0:.1f}s")
     ctx = solver.Context(0)
-    op = solver.CsrOperator(A, ctx)
+    if WL.get("dense"):
+        Ad, lam_exact = dense_qlq_device(args.dense_n)
+        op = solver.DenseOperator(None, ctx, device_ptr=Ad.data_ptr(), n=args.dense_n)
+        A = op
+        log(f"dense operator n={args.dense_n} built on the device in {time.time() - t0:.1f}s")
+    else:
+        A = build_operator(args.grid)
+        lam_exact = None
+        log(f"operator n={A.n} nnz={A.nnz} built in {time.time() - t0:.1f}s")
+        op = solver.CsrOperator(A, ctx)
     bdt = np.complex128 if WL["complex"] else np.float64
     V = solver.Block(A.n, max(K, INC + BUF), dtype=bdt, ctx=ctx)
 
@@ -314,6 +357,7 @@ def main():
             rc = solver.RayleighChebyshevLM()
             rc.setStopCondition("RESIDUAL_ONLY")
             rc.filterPrecision = np.float32 if precision == "mixed" else np.float64
+            rc.refine = True  # the final FP64 Rayleigh-Ritz sweep over the found pairs
             ev = []
             t = time.perf_counter()
             f = rc.getMinEigenSystem(K, TOL, INC, BUF, None, op, solver.RandomizeOp(0), ev, V)
@@ -365,6 +409,24 @@ def main():
                     "avg_launch_us": round(t_dev / max(launches, 1) * 1e6, 2),
                     "alg_bytes_per_launch": int(alg_bytes / max(launches, 1)),
                     "launches": int(launches)}
+        if WL.get("dense"):
+            # dense steps are MFMA-bound: 2 n^2 b flops per launch, b recovered from
+            # the step's byte count (FP64: 8n^2 + 24nb; FP32 correction: 4n^2 + 16nb)
+            nn = float(A.n)
+            bpl = alg_bytes / max(launches, 1)
+            b_eff = (bpl - 4 * nn * nn) / (16 * nn) if precision == "mixed" else \
+                (bpl - 8 * nn * nn) / (24 * nn)
+            flops = 2.0 * nn * nn * b_eff * launches
+            tf = flops / t_dev / 1e12 if t_dev > 0 else 0.0
+            peak = 157.3 if precision == "mixed" else 78.6
+            roofline = {"bound": "mfma", "achieved": round(tf, 2), "peak": peak,
+                        "unit": "TFLOP/s", "frac": round(tf / peak, 4), "traffic": None,
+                        "kernel": ("dense_step_kernel<float,kChebSrc> (FP32 MFMA 16x16x4)"
+                                   if precision == "mixed" else
+                                   "dense_step_kernel<double,kCheb> (FP64 MFMA 16x16x4)"),
+                        "avg_launch_us": round(t_dev / max(launches, 1) * 1e6, 2),
+                        "alg_flops_per_launch": int(2.0 * nn * nn * b_eff),
+                        "launches": int(launches)}
         last = results[-1][2]
         lat_ms = np.array(lat) * 1e3
         return {
@@ -395,6 +457,13 @@ def main():
         mixed["max_rel_eig_diff_vs_fp64"] = float(np.max(
             np.abs(mixed["ev"][:n_cmp] - ev64[:n_cmp]) / floor[:n_cmp])) if n_cmp else None
 
+    if lam_exact is not None:  # the dense operator's spectrum is known exactly
+        for r in (fp64, mixed):
+            if r is None or not len(r["ev"]):
+                continue
+            ref = lam_exact[: len(r["ev"])]
+            r["solve"]["max_rel_err_vs_known_lambda"] = float(np.max(
+                np.abs(r["ev"] - ref) / np.maximum(np.abs(ref), 1e-11 / TOL)))
     total, f, ev, last = fp64["total"], fp64["f"], fp64["ev"], fp64["last"]
     value, roofline = fp64["value"], fp64["roofline"]
     line = {
@@ -435,7 +504,11 @@ def main():
         log("cpu baseline sample ...")
         counts = {"degrees": int(last.filter_degree_sum), "ortho": int(last.count_ortho),
                   "rr": int(last.count_eig)}
-        line["cpu_baseline"] = cpu_baseline(A, counts, args.cpu_iters)
+        if WL.get("dense"):
+            A_host = np.asfortranarray(Ad.cpu().numpy())
+            line["cpu_baseline"] = cpu_baseline(A_host, counts, args.cpu_iters)
+        else:
+            line["cpu_baseline"] = cpu_baseline(A, counts, args.cpu_iters)
     print(json.dumps(line), flush=True)
 
 

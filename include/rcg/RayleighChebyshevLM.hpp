@@ -27,6 +27,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../rcg_c_api.h"
@@ -126,7 +127,10 @@ class AmatrixClass {
   std::vector<T> mData;
 
  private:
-  static int dtype() { return sizeof(T) == 8 ? RCG_F64 : RCG_F32; }
+  static int dtype() {
+    if (std::is_same<T, std::complex<double>>::value) return RCG_C128;
+    return std::is_same<T, float>::value ? RCG_F32 : RCG_F64;
+  }
   void release() {
     if (h_) rcg_block_destroy(h_);
     h_ = nullptr;
@@ -149,10 +153,42 @@ class CsrOperator {
     n_ = n;
   }
   ~CsrOperator() { rcg_operator_destroy(h_); }
+  // Complex Hermitian operator (values as std::complex<double>).
+  CsrOperator(Context& ctx, long n, const std::vector<int32_t>& rowptr,
+              const std::vector<int32_t>& col, const std::vector<std::complex<double>>& val)
+      : ctx_(&ctx) {
+    if ((long)rowptr.size() != n + 1) throw std::runtime_error("rowptr must have n+1 entries");
+    check(rcg_operator_csr_create(ctx.handle(), n, (int64_t)col.size(), rowptr.data(), col.data(),
+                                  val.data(), RCG_C128, RCG_MEM_HOST, &h_));
+    n_ = n;
+  }
   CsrOperator(const CsrOperator&) = delete;
   CsrOperator& operator=(const CsrOperator&) = delete;
   // Locality hint for operators on a structured grid (rows x + nx(y + ny z)).
   void setGrid(long nx, long ny, long nz) { check(rcg_operator_set_grid(h_, nx, ny, nz)); }
+  template <class Atype>
+  void apply(Atype& in, Atype& out, double alpha, double beta) {
+    check(rcg_operator_apply(h_, in.handle(), out.handle(), alpha, beta, in.getColSize()));
+  }
+  rcg_operator handle() const { return h_; }
+  long getDimension() const { return n_; }
+  Context& context() const { return *ctx_; }
+
+ private:
+  Context* ctx_;
+  rcg_operator h_ = nullptr;
+  long n_ = 0;
+};
+
+// A dense real symmetric operator (column-major n x n, leading dimension lda).
+class DenseOperator {
+ public:
+  DenseOperator(Context& ctx, long n, const double* A, long lda) : ctx_(&ctx), n_(n) {
+    check(rcg_operator_dense_create(ctx.handle(), n, A, lda, RCG_MEM_HOST, &h_));
+  }
+  ~DenseOperator() { rcg_operator_destroy(h_); }
+  DenseOperator(const DenseOperator&) = delete;
+  DenseOperator& operator=(const DenseOperator&) = delete;
   template <class Atype>
   void apply(Atype& in, Atype& out, double alpha, double beta) {
     check(rcg_operator_apply(h_, in.handle(), out.handle(), alpha, beta, in.getColSize()));

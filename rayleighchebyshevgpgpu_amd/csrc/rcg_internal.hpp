@@ -160,6 +160,12 @@ template <typename T>
 void launch_residual_sumsq(int64_t n, const T* V, const T* W, int ld, int nc, const double* theta,
                            double* sums, double* ws, size_t ws_elems, hipStream_t st);
 size_t colred_ws_elems(int64_t n, int nc);
+// sums[c] = sum_i |theta[c] V[i,c] - W[i,c]|^2 and sums[nc + c] = sum_i |V[i,c]|^2,
+// c < nc (workspace for 2 nc columns).
+template <typename T>
+void launch_resid_norm_sumsq(int64_t n, const T* V, const T* W, int ld, int nc,
+                             const double* theta, double* sums, double* ws, size_t ws_elems,
+                             hipStream_t st);
 
 // X[i,c] *= 1/sqrt(sums[c]), c < nc.
 template <typename T>

@@ -939,6 +939,49 @@ __global__ __launch_bounds__(kThreads) void col_sumsq_kernel(int64_t n, const T*
   }
 }
 
+// Residual and norm sums together: P[blk*2nc + c] = sum_i |theta_c V - W|^2,
+// P[blk*2nc + nc + c] = sum_i |V|^2 over the block's rows.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void resid_norm_kernel(int64_t n, const T* __restrict__ V,
+                                                              const T* __restrict__ Wm, int ld,
+                                                              int nc, const double* theta,
+                                                              double* __restrict__ P) {
+  __shared__ double red[2 * kThreads];
+  const int64_t blk = blockIdx.x;
+  const int cw = min(nc, kThreads);
+  const int slots = kThreads / cw;
+  const int c_in = threadIdx.x % cw;
+  const int slot = threadIdx.x / cw;
+  const int64_t i0 = blk * kColRows;
+  const int64_t i1 = min(n, i0 + kColRows);
+  for (int cbase = 0; cbase < nc; cbase += cw) {
+    const int c = cbase + c_in;
+    double s = 0.0, q = 0.0;
+    if (slot < slots && c < nc) {
+      const double th = theta[c];
+      for (int64_t i = i0 + slot; i < i1; i += slots) {
+        const double v = double(V[i * ld + c]);
+        const double r = th * v - double(Wm[i * ld + c]);
+        s += r * r;
+        q += v * v;
+      }
+    }
+    red[threadIdx.x] = s;
+    red[kThreads + threadIdx.x] = q;
+    __syncthreads();
+    if (slot == 0 && c < nc) {
+      double t = red[threadIdx.x], u = red[kThreads + threadIdx.x];
+      for (int k = 1; k < slots; ++k) {
+        t += red[threadIdx.x + k * cw];
+        u += red[kThreads + threadIdx.x + k * cw];
+      }
+      P[blk * 2 * nc + c] = t;
+      P[blk * 2 * nc + nc + c] = u;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void normalize_cols_kernel(int64_t n, T* X, int ld, int nc,
                                                                   const double* __restrict__ sums) {
@@ -1475,6 +1518,21 @@ void launch_residual_sumsq(int64_t n, const T* V, const T* W, int ld, int nc, co
 }
 
 template <typename T>
+void launch_resid_norm_sumsq(int64_t n, const T* V, const T* W, int ld, int nc,
+                             const double* theta, double* sums, double* ws, size_t ws_elems,
+                             hipStream_t st) {
+  if (nc <= 0) return;
+  const int64_t nblk = (n + kColRows - 1) / kColRows;
+  if ((size_t)(nblk * 2 * nc) > ws_elems) throw Error(3, "column-reduction workspace too small");
+  hipLaunchKernelGGL((resid_norm_kernel<T>), dim3((unsigned)nblk), dim3(kThreads), 0, st, n, V, W,
+                     ld, nc, theta, ws);
+  RCG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(reduce_grid(2 * nc)),
+                     dim3(kThreads), 0, st, ws, nblk, (int64_t)(2 * nc), sums);
+  RCG_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_normalize_cols(int64_t n, T* X, int ld, int nc, const double* sums, hipStream_t st) {
   if (n <= 0 || nc <= 0) return;
   hipLaunchKernelGGL(normalize_cols_kernel<T>, dim3(grid_for(n * nc)), dim3(kThreads), 0, st, n, X,
@@ -1615,6 +1673,8 @@ void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* al
   template void launch_residual_sumsq<T>(int64_t, const T*, const T*, int, int, const double*,    \
                                          double*, double*, size_t, hipStream_t);                  \
   template void launch_normalize_cols<T>(int64_t, T*, int, int, const double*, hipStream_t);      \
+  template void launch_resid_norm_sumsq<T>(int64_t, const T*, const T*, int, int, const double*,  \
+                                           double*, double*, size_t, hipStream_t);                \
   template void launch_copy_cols<T>(int64_t, const T*, int, int, T*, int, int, int, hipStream_t); \
   template void launch_randomize_cols<T>(int64_t, T*, int, int, int, uint64_t, uint64_t,          \
                                          hipStream_t);                                            \

@@ -361,7 +361,7 @@ class Engine {
   void reserve(int64_t bmax, int64_t kmax) {
     bb_ = std::max<int64_t>({bmax, kmax, 4});
     size_t e = rcg::gram_ws_bound(n_, (int)bb_);
-    e = std::max(e, rcg::colred_ws_elems(n_, (int)bb_));
+    e = std::max(e, rcg::colred_ws_elems(n_, 2 * (int)bb_));
     e = std::max(e, rcg::lz_ws_elems(n_));
     ctx_->ws.ensure(e * sizeof(double));
     ws_elems_ = ctx_->ws.bytes / sizeof(double);
@@ -932,13 +932,54 @@ class Engine {
   }
 
   // createEigenVectorsAndResiduals (RayleighChebyshevLM.h:1984-2087):
-  // V <- normalize(V Y); W = A V; r_c = ||theta_c V_c - W_c||, c < count.
+  // V <- V Y; W = A V; r_c = ||theta_c V_c - W_c|| / ||V_c||, c < count.  With V
+  // orthonormal and Y orthogonal the columns of V Y have unit norm to
+  // round-off; the residuals are divided by the computed norms instead of
+  // rescaling the block (the filter is linear and CholQR normalises), and the
+  // columns handed out as eigenvectors are scaled at lock time.
   template <class S>
   void rotate_residuals(int64_t b, const std::vector<S>& Yh, const std::vector<double>& th,
                         int64_t count, std::vector<double>& res) {
     right_multiply<S>(b, Yh.data());
-    normalize(b);
-    residuals(count, th.data(), res);
+    residuals_normalized(count, th.data(), res);
+  }
+
+  // r_c = ||theta_c V_c - (A V)_c|| / ||V_c||, c < count, and ||V_c||^2 kept
+  // on the device (dsums() + cw*count) for scale_found.
+  void residuals_normalized(int64_t count, const double* th, std::vector<double>& res) {
+    res.assign(count, 0.0);
+    if (count <= 0) return;
+    apply(V(), W_, count);
+    const int64_t rc = cw_ * count;
+    for (int64_t c = 0; c < rc; ++c) hvecUp()[c] = th[c / cw_];
+    RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * rc, hipMemcpyHostToDevice, st_));
+    rcg::launch_resid_norm_sumsq<double>(n_, V(), W_, (int)ld_, (int)rc, dtheta(), dsums(), ws(),
+                                         ws_elems_, st_);
+    RCG_HIP(hipMemcpyAsync(hvecDown(), dsums(), sizeof(double) * 2 * rc, hipMemcpyDeviceToHost, st_));
+    sync();
+    last_norms_.assign(count, 1.0);
+    for (int64_t c = 0; c < count; ++c) {
+      double t = 0.0, q = 0.0;
+      for (int e = 0; e < cw_; ++e) {
+        t += hvecDown()[cw_ * c + e];
+        q += hvecDown()[rc + cw_ * c + e];
+      }
+      last_norms_[c] = q > 0.0 ? std::sqrt(q) : 1.0;
+      res[c] = std::sqrt(std::abs(t)) / last_norms_[c];
+    }
+  }
+  // The columns [c0, c0+cnt) of `dst` (ld ldd) came from V's first columns at
+  // the last residual computation: scale them to unit norm.
+  void scale_found(double* dst, int64_t ldd, int64_t c0, int64_t cnt) {
+    if (cnt <= 0) return;
+    const int64_t rc = cw_ * cnt;
+    for (int64_t c = 0; c < rc; ++c) {
+      const double nrm = c / cw_ < (int64_t)last_norms_.size() ? last_norms_[c / cw_] : 1.0;
+      hvecUp()[c] = nrm * nrm;
+    }
+    RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * rc, hipMemcpyHostToDevice, st_));
+    rcg::launch_normalize_cols<double>(n_, dst + cw_ * c0, (int)ldd, (int)rc, dtheta(), st_);
+    sync();
   }
 
   void residuals(int64_t count, const double* th, std::vector<double>& res) {
@@ -1100,6 +1141,7 @@ class Engine {
   std::string last_sched_;
   float* E_[2] = {nullptr, nullptr};  // FP32 correction blocks (mixed filter)
   float* R32_ = nullptr;              // FP32 residual block (mixed filter)
+  std::vector<double> last_norms_;    // column norms of V at the last residuals
   DevBuf coef_;                       // mixed filter coefficient table
   HostBuf hcoef_;
   hipEvent_t fev_[2] = {nullptr, nullptr};
@@ -1245,6 +1287,7 @@ class Solver {
       rcg::dense::small_eigensystem(H.data(), subspaceSize, theta, Yh);
       eng_.rotate_residuals(subspaceSize, Yh, theta, subspaceSize, eigRes);
       eng_.copy_cols(eng_.V(), ld, 0, eng_.Y(), ldy, 0, subspaceSize);
+      eng_.scale_found(eng_.Y(), ldy, 0, subspaceSize);
       eigValues = theta;
       found_ = subspaceSize;
       R->found = subspaceSize;
@@ -1406,6 +1449,7 @@ class Solver {
       }
       if (foundCount > 0) {
         eng_.copy_cols(eng_.V(), ld, 0, eng_.Y(), ldy, foundSize, foundCount);
+        eng_.scale_found(eng_.Y(), ldy, foundSize, foundCount);
         for (long i = 0; i < foundCount; ++i) {
           eigValues.push_back(theta[i]);
           eigRes.push_back(subspaceResiduals[i]);
@@ -1513,6 +1557,7 @@ class Solver {
     rcg::dense::small_eigensystem(H.data(), f, th, Z);
     e2.rotate_residuals(f, Z, th, f, r);
     if (!e2.v_is_first()) e2.copy_cols(e2.V(), ldy, 0, eng_.Y(), ldy, 0, f);
+    e2.scale_found(eng_.Y(), ldy, 0, f);
     e2.sync();
     eigValues.assign(th.begin(), th.end());
     eigRes = r;
