@@ -70,34 +70,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def dense_qlq_device(n, seed=3):
-    """A = Q diag(linspace(0,1,n)^2) Q^T on the device (FP64), Q = qr(G).Q diag(sign diag R)
-    with G ~ N(0,1) by Box-Muller from stream `seed` (the same draws as
-    operators.dense_qlq).  The QR and the products run in torch on the GPU: input
-    set-up, outside any timed region."""
-    import torch
-    from rayleighchebyshevgpgpu_amd import operators as ops
-    dev = torch.device("cuda:0")
-    G = torch.empty((n, n), dtype=torch.float64, device=dev)
-    cols = max(1, (1 << 26) // n)  # columns per chunk of draws (column-major order)
-    for c0 in range(0, n, cols):
-        c1 = min(n, c0 + cols)
-        cnt = (c1 - c0) * n
-        u = ops.unit_draws(seed, c0 * n, cnt + cnt % 2)
-        u1, u2 = 1.0 - u[0::2], u[1::2]
-        r = np.sqrt(-2.0 * np.log(u1))
-        g = np.empty(u.shape[0])
-        g[0::2] = r * np.cos(2 * np.pi * u2)
-        g[1::2] = r * np.sin(2 * np.pi * u2)
-        G[:, c0:c1] = torch.from_numpy(g[:cnt].reshape(c1 - c0, n).T.copy()).to(dev)
-    Q, R = torch.linalg.qr(G)
-    Q = Q * torch.sign(torch.diagonal(R))[None, :]
-    del G, R
-    lam = torch.linspace(0.0, 1.0, n, dtype=torch.float64, device=dev) ** 2
-    A = (Q * lam[None, :]) @ Q.T
-    A = 0.5 * (A + A.T)
-    torch.cuda.synchronize()
-    return A.contiguous(), lam.cpu().numpy()
+def dense_operator_host(n, seed=3):
+    """A = Q diag(linspace(0,1,n)^2) Q^T (the dense workload), built on the GPU by
+    torch in a child process started before this process touches the GPU, read
+    back from a .npy file; returns (A column-major, the exact eigenvalues)."""
+    import subprocess
+    import tempfile
+    path = os.path.join(tempfile.gettempdir(), f"rcg_dense_{n}_{seed}.npy")
+    if not os.path.exists(path):
+        subprocess.run([sys.executable, "-m", "rayleighchebyshevgpgpu_amd.dense_gen", str(n),
+                        str(seed), path], check=True, cwd=ROOT)
+    A = np.load(path, mmap_mode="r")
+    return A, np.linspace(0.0, 1.0, n) ** 2
 
 
 def build_operator(grid=GRID):
@@ -337,13 +321,14 @@ def main():
         return kernel_only(args)
 
     t0 = time.time()
-    ctx = solver.Context(0)
     if WL.get("dense"):
-        Ad, lam_exact = dense_qlq_device(args.dense_n)
-        op = solver.DenseOperator(None, ctx, device_ptr=Ad.data_ptr(), n=args.dense_n)
+        Ad, lam_exact = dense_operator_host(args.dense_n)  # before this process uses the GPU
+        ctx = solver.Context(0)
+        op = solver.DenseOperator(np.asarray(Ad), ctx)
         A = op
-        log(f"dense operator n={args.dense_n} built on the device in {time.time() - t0:.1f}s")
+        log(f"dense operator n={args.dense_n} built in {time.time() - t0:.1f}s")
     else:
+        ctx = solver.Context(0)
         A = build_operator(args.grid)
         lam_exact = None
         log(f"operator n={A.n} nnz={A.nnz} built in {time.time() - t0:.1f}s")
@@ -505,8 +490,7 @@ def main():
         counts = {"degrees": int(last.filter_degree_sum), "ortho": int(last.count_ortho),
                   "rr": int(last.count_eig)}
         if WL.get("dense"):
-            A_host = np.asfortranarray(Ad.cpu().numpy())
-            line["cpu_baseline"] = cpu_baseline(A_host, counts, args.cpu_iters)
+            line["cpu_baseline"] = cpu_baseline(np.asfortranarray(Ad), counts, args.cpu_iters)
         else:
             line["cpu_baseline"] = cpu_baseline(A, counts, args.cpu_iters)
     print(json.dumps(line), flush=True)

@@ -321,6 +321,19 @@ class RayleighChebyshevLM {
     return solve(maxEigensystemDim, subspaceTol, inc, buf, oP, randOp, eigValues, eigVectors,
                  true, lambdaMax, false, 0.0, 0.0);
   }
+  // getMinIntervalEigenSystem with the caller's spectral bounds, no Lanczos
+  // (RayleighChebyshev.h:812-821).
+  long getMinIntervalEigenSystem(double minEigValueEst, double lambdaMax, double maxEigValueBound,
+                                 double subspaceTol, long inc, long buf, long maxEigensystemDim,
+                                 Vtype& vStart, Otype& oP, VRandomizeOpType& randOp,
+                                 std::vector<double>& eigValues, Atype& eigVectors) {
+    (void)vStart;
+    return solve(maxEigensystemDim, subspaceTol, inc, buf, oP, randOp, eigValues, eigVectors,
+                 true, lambdaMax, true, minEigValueEst, maxEigValueBound);
+  }
+
+ protected:
+  bool nonRandomStart() const { return nonRandom_; }
 
  private:
   long solve(long k, double tol, long inc, long buf, Otype& oP, VRandomizeOpType& randOp,
@@ -355,6 +368,87 @@ class RayleighChebyshevLM {
   rcg_solve_result r_{};
   std::vector<double> res_;
   bool nonRandom_ = false;
+};
+
+// The older RayleighChebyshev template (RayleighChebyshev.h:210-821): the
+// same solves, with eigenvectors exchanged as host vectors (Vtype with a vData
+// member, e.g. AvectorClass<double>) in a std::vector.  With the non-random
+// start flag the vectors passed in are the start block.  Each call moves the
+// block between host and device once.
+template <class Vtype, class Otype, class VRandomizeOpType, typename Dtype = double>
+class RayleighChebyshev
+    : public RayleighChebyshevLM<AmatrixClass<Dtype>, Vtype, Otype, VRandomizeOpType, Dtype> {
+  using Base = RayleighChebyshevLM<AmatrixClass<Dtype>, Vtype, Otype, VRandomizeOpType, Dtype>;
+
+ public:
+  // RayleighChebyshev.h:728-755
+  long getMinEigenSystem(long eigCount, double subspaceTol, long inc, long buf, Vtype& vStart,
+                         Otype& oP, VRandomizeOpType& randOp, std::vector<double>& eigValues,
+                         std::vector<Vtype>& eigVectors) {
+    return host_solve(std::max(eigCount, inc + buf), vStart, oP, eigVectors,
+                      [&](AmatrixClass<Dtype>& V) {
+                        return Base::getMinEigenSystem(eigCount, subspaceTol, inc, buf, vStart, oP,
+                                                       randOp, eigValues, V);
+                      });
+  }
+  // RayleighChebyshev.h:709-724
+  long getMinEigenSystem(long eigCount, double minEigValueEst, double maxEigValueBound,
+                         double subspaceTol, long inc, long buf, Vtype& vStart, Otype& oP,
+                         VRandomizeOpType& randOp, std::vector<double>& eigValues,
+                         std::vector<Vtype>& eigVectors) {
+    return host_solve(std::max(eigCount, inc + buf), vStart, oP, eigVectors,
+                      [&](AmatrixClass<Dtype>& V) {
+                        return Base::getMinEigenSystem(eigCount, minEigValueEst, maxEigValueBound,
+                                                       subspaceTol, inc, buf, vStart, oP, randOp,
+                                                       eigValues, V);
+                      });
+  }
+  // RayleighChebyshev.h:761-805
+  long getMinIntervalEigenSystem(double lambdaMax, double subspaceTol, long inc, long buf,
+                                 long maxEigensystemDim, Vtype& vStart, Otype& oP,
+                                 VRandomizeOpType& randOp, std::vector<double>& eigValues,
+                                 std::vector<Vtype>& eigVectors) {
+    return host_solve(std::max(maxEigensystemDim, inc + buf), vStart, oP, eigVectors,
+                      [&](AmatrixClass<Dtype>& V) {
+                        return Base::getMinIntervalEigenSystem(lambdaMax, subspaceTol, inc, buf,
+                                                               maxEigensystemDim, vStart, oP,
+                                                               randOp, eigValues, V);
+                      });
+  }
+  // RayleighChebyshev.h:812-821
+  long getMinIntervalEigenSystem(double minEigValueEst, double lambdaMax, double maxEigValueBound,
+                                 double subspaceTol, long inc, long buf, long maxEigensystemDim,
+                                 Vtype& vStart, Otype& oP, VRandomizeOpType& randOp,
+                                 std::vector<double>& eigValues, std::vector<Vtype>& eigVectors) {
+    return host_solve(std::max(maxEigensystemDim, inc + buf), vStart, oP, eigVectors,
+                      [&](AmatrixClass<Dtype>& V) {
+                        return Base::getMinIntervalEigenSystem(
+                            minEigValueEst, lambdaMax, maxEigValueBound, subspaceTol, inc, buf,
+                            maxEigensystemDim, vStart, oP, randOp, eigValues, V);
+                      });
+  }
+
+ private:
+  template <class F>
+  long host_solve(long cap, Vtype& vStart, Otype& oP, std::vector<Vtype>& eigVectors, F&& call) {
+    const long n = oP.getDimension();
+    AmatrixClass<Dtype> V(oP.context(), n, cap);
+    if (this->nonRandomStart()) {
+      const long c = std::min<long>((long)eigVectors.size(), cap);
+      for (long j = 0; j < c; ++j)
+        for (long i = 0; i < n; ++i) V(i, j) = eigVectors[j].vData[i];
+      V.host_to_device();
+      V.setValidColumns(c);
+    }
+    const long f = call(V);
+    V.device_to_host();
+    eigVectors.assign(f, vStart);
+    for (long j = 0; j < f; ++j) {
+      eigVectors[j].vData.resize(n);
+      for (long i = 0; i < n; ++i) eigVectors[j].vData[i] = V(i, j);
+    }
+    return f;
+  }
 };
 
 }  // namespace rcg

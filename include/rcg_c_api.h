@@ -115,6 +115,20 @@ int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32
  * MFMA block products (FP64, or FP32 for the mixed-precision filter). */
 int rcg_operator_dense_create(rcg_context ctx, int64_t n, const double* A, int64_t lda, int mem,
                               rcg_operator* out);
+/* Operator ingest (SURVEY-listed formats around the path).  The reference's
+ * CSR index width is the includer's RAY_INT (RayleighChebyshevLMCuda.h:409ff):
+ * this takes 64-bit row pointers and columns and narrows them to the int32
+ * the kernels read (RCG_ERR_UNSUPPORTED beyond 2^31 - 1). */
+int rcg_operator_csr64_create(rcg_context ctx, int64_t n, int64_t nnz, const int64_t* rowptr,
+                              const int64_t* col, const void* val, int val_dtype,
+                              rcg_operator* out);
+/* Matrix Market coordinate files (real, integer, complex or pattern; general,
+ * symmetric or hermitian): rows sorted, duplicates summed, the stored
+ * triangle mirrored.  _info and _csr work on the host only (nnz counts the
+ * expanded entries; val holds nnz doubles, or 2 nnz interleaved for C128). */
+int rcg_matrix_market_info(const char* path, int64_t* n, int64_t* nnz, int* dtype);
+int rcg_matrix_market_csr(const char* path, int32_t* rowptr, int32_t* col, void* val);
+int rcg_operator_from_matrix_market(rcg_context ctx, const char* path, rcg_operator* out);
 /* No counterpart in the reference: a locality hint for operators on a
  * structured nx x ny x nz grid whose rows are numbered x + nx*(y + ny*z).
  * The CSR kernels may then visit x-lines in (y, z) tiles; results do not
@@ -198,7 +212,10 @@ typedef struct rcg_solve_params {
                              final FP64 refinement sweep */
   int32_t refine;         /* 1: final FP64 Rayleigh-Ritz over all found vectors */
   int32_t verbose;        /* setVerboseFlag: one line per inner iteration on stdout */
-  int32_t use_bounds;     /* 1: getMinEigenSystem(k, min_eig, max_eig, ...) (no Lanczos) */
+  int32_t use_bounds;     /* 1: the caller's bounds min_eig, max_eig, no Lanczos:
+                             getMinEigenSystem(k, min_eig, max_eig, ...) or, with
+                             interval, getMinIntervalEigenSystem(min_eig, lambda_max,
+                             max_eig, ...) (RayleighChebyshev.h:709-723, 812-821) */
   int32_t nonrandom_start;/* setNonRandomStartFlag: > 0 starts from the first
                              nonrandom_start columns of `evecs` (LM:957-985) */
   double min_eig;

@@ -78,6 +78,10 @@ def lib() -> C.CDLL:
         "rcg_operator_csr_create": ([vp, i64, i64, vp, vp, vp, i32, i32, P(vp)], i32),
         "rcg_operator_destroy": ([vp], i32),
         "rcg_operator_dense_create": ([vp, i64, vp, i64, i32, P(vp)], i32),
+        "rcg_operator_csr64_create": ([vp, i64, i64, vp, vp, vp, i32, P(vp)], i32),
+        "rcg_matrix_market_info": ([C.c_char_p, P(i64), P(i64), P(i32)], i32),
+        "rcg_matrix_market_csr": ([C.c_char_p, vp, vp, vp], i32),
+        "rcg_operator_from_matrix_market": ([vp, C.c_char_p, P(vp)], i32),
         "rcg_operator_set_grid": ([vp, i64, i64, i64], i32),
         "rcg_operator_apply": ([vp, vp, vp, d, d, i64], i32),
         "rcg_cheb_filter": ([vp, vp, vp, i64, i64, i64, d, d], i32),

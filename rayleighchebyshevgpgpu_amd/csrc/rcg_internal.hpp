@@ -42,6 +42,9 @@ struct Error : std::runtime_error {
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// The calling thread's message for rcg_get_last_error.
+void set_last_error(const std::string& msg);
+
 // -------------------------------------------------------------- launchers
 // All pointers are device pointers; every launch goes on `st`.
 

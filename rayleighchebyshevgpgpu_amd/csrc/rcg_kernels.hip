@@ -613,16 +613,21 @@ __device__ __forceinline__ d4v mfma16x16x4(double a, double b, d4v c) {
 template <bool SYM>
 __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ Y, int ldy,
-    int by, double* __restrict__ P, int64_t rows_per_blk, int nta) {
+    int by, double* __restrict__ P, int64_t rows_per_blk, int nta, int ntiles, int64_t nblk) {
   __shared__ double red[kTile * kTile];
-  const int tile = blockIdx.y;
+  // one 1-D grid: the tiles of a row block are consecutive logical workgroups,
+  // which the remap keeps on one XCD, so its rows are read from HBM once
+  const int64_t L = xcd_remap(blockIdx.x, gridDim.x);
+  if (L >= nblk * ntiles) return;
+  const int tile = (int)(L % ntiles);
+  const int64_t rblk = L / ntiles;
   const int a0 = (tile % nta) * kTile;
   const int b0 = (tile / nta) * kTile;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int kr = lane >> 4;
   const int cc = lane & 15;
-  const int64_t i0 = blockIdx.x * rows_per_blk;
+  const int64_t i0 = rblk * rows_per_blk;
   const int64_t i1 = min(n, i0 + rows_per_blk);
   d4v acc[4][4];
 #pragma unroll
@@ -678,7 +683,7 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     }
     __syncthreads();
   }
-  double* Pb = P + blockIdx.x * (int64_t)bx * by;
+  double* Pb = P + rblk * (int64_t)bx * by;
   for (int e = threadIdx.x; e < kTile * kTile; e += kThreads) {
     const int al = e % kTile;
     const int bl = e / kTile;
@@ -703,9 +708,14 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
 __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ C,
     int ldc, int by, double alpha, double beta, const double* Z, double* out, int ldo,
-    int64_t ntiles) {
+    int64_t ntiles, int ct, int64_t nstripe) {
   __shared__ double cs[kTile][kTile + 1];  // [k][c]
-  const int c0 = blockIdx.y * kTile;
+  // 1-D grid of nstripe x ct workgroups: the ct column tiles of a row stripe are
+  // consecutive logical workgroups on one XCD, so X rows are read from HBM once
+  const int64_t L = xcd_remap(blockIdx.x, gridDim.x);
+  if (L >= nstripe * ct) return;
+  const int c0 = (int)(L % ct) * kTile;
+  const int64_t stripe = L / ct;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int kk = lane >> 4;
@@ -723,7 +733,7 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
     stage(0);
     __syncthreads();
   }
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (int64_t tile = stripe; tile < ntiles; tile += nstripe) {
     const int64_t row = tile * kTile + wave * 16 + r;
     d4v acc[4];
 #pragma unroll
@@ -1450,11 +1460,12 @@ void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, in
   const int ntiles = nta * ((by + kTile - 1) / kTile);
   if constexpr (std::is_same<T, double>::value) {
     if (X == Y && bx == by && ldx == ldy && bx <= kTile)
-      hipLaunchKernelGGL(gram_mfma_kernel<true>, dim3((unsigned)nblk, 1), dim3(kThreads), 0, st, n,
-                         X, ldx, bx, Y, ldy, by, ws, rpb, nta);
+      hipLaunchKernelGGL(gram_mfma_kernel<true>, dim3((unsigned)round_up(nblk, kXcds)),
+                         dim3(kThreads), 0, st, n, X, ldx, bx, Y, ldy, by, ws, rpb, nta, 1, nblk);
     else
-      hipLaunchKernelGGL(gram_mfma_kernel<false>, dim3((unsigned)nblk, ntiles), dim3(kThreads), 0,
-                         st, n, X, ldx, bx, Y, ldy, by, ws, rpb, nta);
+      hipLaunchKernelGGL(gram_mfma_kernel<false>,
+                         dim3((unsigned)round_up(nblk * ntiles, kXcds)), dim3(kThreads), 0, st, n,
+                         X, ldx, bx, Y, ldy, by, ws, rpb, nta, ntiles, nblk);
   } else {
     hipLaunchKernelGGL(gram_partial_kernel<T>, dim3((unsigned)nblk, ntiles), dim3(kThreads), 0, st,
                        n, X, ldx, bx, Y, ldy, by, ws, rpb, ntiles);
@@ -1474,10 +1485,11 @@ void launch_tall_gemm(int64_t n, const T* X, int ldx, int bx, const T* C, int ld
   const int64_t grid = round_up(nrb, kXcds);
   const unsigned ct = (unsigned)((by + kTile - 1) / kTile);
   if constexpr (std::is_same<T, double>::value) {
-    // persistent over row tiles: 4 workgroups per CU of a 256-CU part
-    const int64_t pgrid = std::min<int64_t>(nrb, 1024);
-    hipLaunchKernelGGL(tall_gemm_mfma_kernel, dim3((unsigned)pgrid, ct), dim3(kThreads), 0, st, n,
-                       X, ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);
+    // persistent over row tiles: about 4 workgroups per CU of a 256-CU part
+    const int64_t nstripe = std::max<int64_t>(1, std::min<int64_t>(nrb, 1024 / (int64_t)ct));
+    hipLaunchKernelGGL(tall_gemm_mfma_kernel, dim3((unsigned)round_up(nstripe * ct, kXcds)),
+                       dim3(kThreads), 0, st, n, X, ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo,
+                       nrb, (int)ct, nstripe);
   } else {
     hipLaunchKernelGGL(tall_gemm_kernel<T>, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
                        ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);

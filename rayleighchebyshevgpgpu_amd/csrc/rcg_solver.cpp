@@ -95,6 +95,9 @@ struct rcg_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // set for server workers: waits block the thread on this event instead of
+  // spinning, so many workers share the host cores
+  hipEvent_t blocking_ev = nullptr;
   DevBuf ws;      // reduction partials
   DevBuf small;   // b x b matrices and b-vectors on the device
   HostBuf pinned; // host staging of the same
@@ -143,6 +146,12 @@ struct rcg_operator_s {
 namespace {
 
 thread_local std::string g_last_error;
+
+}  // namespace
+
+void rcg::set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
 
 template <class F>
 int guarded(F&& f) {
@@ -204,7 +213,7 @@ class FilterBatcher {
   FilterBatcher(int device, int window_us) : device_(device), window_us_(window_us) {
     RCG_HIP(hipSetDevice(device_));
     RCG_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-    RCG_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming));
+    RCG_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming | hipEventBlockingSync));
     thr_ = std::thread([this] { loop(); });
   }
   ~FilterBatcher() {
@@ -388,7 +397,14 @@ class Engine {
   int64_t ld() const { return ld_; }
   int64_t ldy() const { return ldy_; }
   hipStream_t stream() const { return st_; }
-  void sync() { RCG_HIP(hipStreamSynchronize(st_)); }
+  void sync() {
+    if (ctx_->blocking_ev) {
+      RCG_HIP(hipEventRecord(ctx_->blocking_ev, st_));
+      RCG_HIP(hipEventSynchronize(ctx_->blocking_ev));
+    } else {
+      RCG_HIP(hipStreamSynchronize(st_));
+    }
+  }
 
   // Scratch layout (device `small` and host `pinned` alike):
   //   [0, bb^2)          matrix A (Gram results, uploaded factors)
@@ -1235,10 +1251,10 @@ class Solver {
     // spectral bounds (getMinEigenSystem, LM:712-738; interval, LM:745-782)
     double minEigValue, maxEigValue, lambdaMax;
     const double tl0 = wall();
-    if (p_.use_bounds && !p_.interval) {
+    if (p_.use_bounds) {  // caller's bounds, no Lanczos (RayleighChebyshev.h:709-723, 812-821)
       minEigValue = p_.min_eig;
       maxEigValue = p_.max_eig;
-      lambdaMax = maxEigValue;
+      lambdaMax = p_.interval ? p_.lambda_max : maxEigValue;
     } else {
       double mn, mx;
       long it;
@@ -1681,11 +1697,11 @@ void server_worker(rcg_server_s* S, int w) {
         b->dtype = t->op->dtype;
         b->mem.ensure(rcg::dtype_size(b->dtype) * (size_t)b->n * (size_t)b->ld);
         s.export_vectors(b.get());
-        RCG_HIP(hipStreamSynchronize(ctx->stream));
         b->ctx = S->ctx;
         t->evecs = b.release();
       }
-      RCG_HIP(hipStreamSynchronize(ctx->stream));
+      RCG_HIP(hipEventRecord(ctx->blocking_ev, ctx->stream));
+      RCG_HIP(hipEventSynchronize(ctx->blocking_ev));
     } catch (const Error& e) {
       t->status = e.code;
       t->error = e.what();
@@ -2223,6 +2239,8 @@ int rcg_server_create(rcg_context ctx, int64_t hbm_budget, int32_t max_active, i
       c->device = ctx->device;
       RCG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       c->own_stream = true;
+      RCG_HIP(hipEventCreateWithFlags(&c->blocking_ev,
+                                      hipEventBlockingSync | hipEventDisableTiming));
       S->wctx.push_back(std::move(c));
     }
     rcg_server_s* raw = S.get();
@@ -2342,6 +2360,7 @@ int rcg_server_destroy(rcg_server s) {
       if (kv.second->evecs) delete kv.second->evecs;
     for (auto& c : s->wctx) {
       (void)hipStreamSynchronize(c->stream);
+      (void)hipEventDestroy(c->blocking_ev);
       (void)hipStreamDestroy(c->stream);
     }
     delete s;

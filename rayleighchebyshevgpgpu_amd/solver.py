@@ -138,6 +138,40 @@ class CsrOperator:
     def apply(self, X: Block, Y: Block, alpha: float = 1.0, beta: float = 0.0, cols=None):
         check(lib().rcg_operator_apply(self.h, X.h, Y.h, alpha, beta, X.cols if cols is None else cols))
 
+    @classmethod
+    def from_matrix_market(cls, path: str, ctx: Context | None = None) -> "CsrOperator":
+        """An operator read from a Matrix Market coordinate file by the library."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        n, nnz, dt = C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().rcg_matrix_market_info(str(path).encode(), C.byref(n), C.byref(nnz), C.byref(dt)))
+        self.n, self.nnz = n.value, nnz.value
+        self.is_complex = dt.value == capi.C128
+        self.dtype = np.dtype(np.complex128 if self.is_complex else np.float64)
+        h = C.c_void_p()
+        check(lib().rcg_operator_from_matrix_market(self.ctx.h, str(path).encode(), C.byref(h)))
+        self.h = h
+        return self
+
+    @classmethod
+    def from_csr64(cls, n, rowptr, col, val, ctx: Context | None = None) -> "CsrOperator":
+        """An operator from 64-bit CSR index arrays (narrowed by the library)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int64)
+        self.is_complex = np.iscomplexobj(val)
+        self.dtype = np.dtype(np.complex128 if self.is_complex else np.float64)
+        val = np.ascontiguousarray(val, dtype=self.dtype)
+        self.n, self.nnz = int(n), int(col.shape[0])
+        h = C.c_void_p()
+        check(lib().rcg_operator_csr64_create(self.ctx.h, self.n, self.nnz, rowptr.ctypes.data,
+                                              col.ctypes.data, val.ctypes.data,
+                                              capi.C128 if self.is_complex else capi.F64,
+                                              C.byref(h)))
+        self.h = h
+        return self
+
     def close(self):
         if getattr(self, "h", None):
             lib().rcg_operator_destroy(self.h)
@@ -337,9 +371,70 @@ class RayleighChebyshevLM:
                                bounds=(mn, mx))
         raise TypeError("getMinEigenSystem takes 9 or 11 arguments")
 
-    def getMinIntervalEigenSystem(self, lambdaMax, subspaceTol, inc, buf, maxEigensystemDim,
-                                  vStart, oP, randOp, eigValues, eigVectors):
-        """All eigenpairs below lambdaMax, at most maxEigensystemDim
-        (RayleighChebyshevLM.h:745-782)."""
-        return self._solve(maxEigensystemDim, subspaceTol, inc, buf, oP, randOp, eigValues,
-                           eigVectors, interval=True, lambda_max=lambdaMax)
+    def getMinIntervalEigenSystem(self, *args):
+        """All eigenpairs below lambdaMax, at most maxEigensystemDim:
+        getMinIntervalEigenSystem(lambdaMax, subspaceTol, inc, buf, maxEigensystemDim,
+        vStart, oP, randOp, eigValues, eigVectors) (RayleighChebyshevLM.h:745-782) or,
+        with the caller's spectral bounds and no Lanczos,
+        getMinIntervalEigenSystem(minEigValueEst, lambdaMax, maxEigValueBound,
+        subspaceTol, inc, buf, maxEigensystemDim, vStart, oP, randOp, eigValues,
+        eigVectors) (RayleighChebyshev.h:812-821)."""
+        if len(args) == 10:
+            lambdaMax, tol, inc, buf, dim, _v, oP, randOp, eigValues, eigVectors = args
+            return self._solve(dim, tol, inc, buf, oP, randOp, eigValues, eigVectors,
+                               interval=True, lambda_max=lambdaMax)
+        if len(args) == 12:
+            mn, lambdaMax, mx, tol, inc, buf, dim, _v, oP, randOp, eigValues, eigVectors = args
+            return self._solve(dim, tol, inc, buf, oP, randOp, eigValues, eigVectors,
+                               interval=True, lambda_max=lambdaMax, bounds=(mn, mx))
+        raise TypeError("getMinIntervalEigenSystem takes 10 or 12 arguments")
+
+
+class RayleighChebyshev(RayleighChebyshevLM):
+    """The older RayleighChebyshev template's calling convention
+    (RayleighChebyshev.h:210-821): eigenvectors come back as a list of host
+    vectors (numpy arrays) instead of a device block, and with the
+    non-random-start flag the list's vectors are the start block.  The solve is
+    the same device solve."""
+
+    def _host_solve(self, call, k, inc, buf, oP, eigVectors):
+        n = oP.n
+        cols = max(int(k), int(inc) + int(buf))
+        V = Block(n, cols, dtype=getattr(oP, "dtype", np.float64), ctx=oP.ctx)
+        if self.nonRandomStartFlag and eigVectors:
+            start = np.stack([np.asarray(v) for v in eigVectors[:cols]], axis=1)
+            V.upload(start)
+            V.valid_cols = start.shape[1]
+        f = call(V)
+        if eigVectors is not None:
+            X = V.download(f)
+            eigVectors.clear()
+            eigVectors.extend(X[:, j].copy() for j in range(f))
+        return f
+
+    def getMinEigenSystem(self, eigCount, *args):
+        if len(args) == 8:
+            tol, inc, buf, _v, oP, randOp, eigValues, eigVectors = args
+            return self._host_solve(lambda V: self._solve(eigCount, tol, inc, buf, oP, randOp,
+                                                          eigValues, V),
+                                    eigCount, inc, buf, oP, eigVectors)
+        if len(args) == 10:
+            mn, mx, tol, inc, buf, _v, oP, randOp, eigValues, eigVectors = args
+            return self._host_solve(lambda V: self._solve(eigCount, tol, inc, buf, oP, randOp,
+                                                          eigValues, V, bounds=(mn, mx)),
+                                    eigCount, inc, buf, oP, eigVectors)
+        raise TypeError("getMinEigenSystem takes 9 or 11 arguments")
+
+    def getMinIntervalEigenSystem(self, *args):
+        if len(args) == 10:
+            lambdaMax, tol, inc, buf, dim, _v, oP, randOp, eigValues, eigVectors = args
+            bounds = None
+        elif len(args) == 12:
+            mn, lambdaMax, mx, tol, inc, buf, dim, _v, oP, randOp, eigValues, eigVectors = args
+            bounds = (mn, mx)
+        else:
+            raise TypeError("getMinIntervalEigenSystem takes 10 or 12 arguments")
+        return self._host_solve(lambda V: self._solve(dim, tol, inc, buf, oP, randOp, eigValues, V,
+                                                      interval=True, lambda_max=lambdaMax,
+                                                      bounds=bounds),
+                                dim, inc, buf, oP, eigVectors)

@@ -91,6 +91,24 @@ int main(int argc, char** argv) {
         std::fabs(nrm - 1.0) > 1e-10)
       bad = 1;
   }
+  // The older RayleighChebyshev template: host vectors, and an interval solve
+  // with the caller's bounds (no Lanczos) for every eigenvalue below a gap.
+  rcg::RayleighChebyshev<rcg::AvectorClass<double>, rcg::CsrOperator, rcg::ArandomizeOpClass> rc3;
+  rc3.setStopCondition("RESIDUAL_ONLY");
+  long below = k;
+  while (below + 1 < (long)exact.size() && exact[below + 1] - exact[below] < 1e-6) ++below;
+  const double lambdaMax = 0.5 * (exact[below] + exact[below + 1]);
+  rcg::ArandomizeOpClass randOp3(3);
+  std::vector<double> eigValues3;
+  std::vector<rcg::AvectorClass<double>> eigVectors3;
+  const long found3 = rc3.getMinIntervalEigenSystem(0.0, lambdaMax, 8.0, tol, 8, 8, 4 * k, vStart,
+                                                    op, randOp3, eigValues3, eigVectors3);
+  if (found3 != below + 1 || (long)eigVectors3.size() != found3 ||
+      rc3.getSpectralBoundsIterCount() != 0)
+    bad = 1;
+  for (long j = 0; j < std::min(found3, below + 1); ++j)
+    if (std::fabs(eigValues3[j] - exact[j]) > 1e-10 * std::max(1.0, exact[j])) bad = 1;
+  std::printf("interval below %.6f: found %ld of %ld\n", lambdaMax, found3, below + 1);
   const auto cd = rc.getCountData();
   std::printf("found %ld ortho %ld apply %ld eig %ld lanczos %ld\n", found, cd.at("ortho"),
               cd.at("OpApply"), cd.at("eigenvalue"), rc.getSpectralBoundsIterCount());

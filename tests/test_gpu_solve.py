@@ -160,3 +160,52 @@ def test_warm_start_converges_immediately(rcg):
     assert f2 == 8
     assert np.max(np.abs(np.array(ev2) - ev)) < 1e-10
     assert rc2.getCountData()["eigenvalue"] < cold_iters
+
+
+def test_rc_adapter_vectors_and_interval_bounds(rcg):
+    """The older RayleighChebyshev calling convention (host vector list) and the
+    interval solve with caller bounds (RayleighChebyshev.h:709-821)."""
+    print("case: RayleighChebyshev adapter, 32x32 Dirichlet", flush=True)
+    A = ops.laplacian_2d(32)
+    exact = ops.laplacian_2d_spectrum(32)
+    op = rcg.CsrOperator(A)
+    rc = rcg.RayleighChebyshev()
+    rc.setStopCondition("RESIDUAL_ONLY")
+    ev, vecs = [], []
+    f = rc.getMinEigenSystem(8, 1e-10, 8, 8, None, op, rcg.RandomizeOp(0), ev, vecs)
+    assert f == 8 and len(vecs) == 8 and vecs[0].shape == (A.n,)
+    assert np.max(np.abs(np.array(ev) - exact[:8]) / exact[:8]) < 1e-10
+    S = A.to_scipy()
+    assert max(np.linalg.norm(S @ v - l * v) for l, v in zip(ev, vecs)) < 8e-10
+    # interval with the caller's bounds (no Lanczos): every eigenvalue below lambdaMax
+    lam_max = 0.5 * (exact[9] + exact[10])
+    ev2 = []
+    f2 = rc.getMinIntervalEigenSystem(0.0, lam_max, 8.0, 1e-10, 8, 8, 40, None, op,
+                                      rcg.RandomizeOp(1), ev2, [])
+    assert f2 == 10
+    assert rc.getSpectralBoundsIterCount() == 0
+    assert np.max(np.abs(np.array(ev2) - exact[:10]) / exact[:10]) < 1e-10
+    # warm start from the vectors found above converges at once
+    rc.setNonRandomStartFlag(True)
+    ev3 = []
+    f3 = rc.getMinEigenSystem(8, 1e-10, 8, 8, None, op, rcg.RandomizeOp(2), ev3, vecs)
+    assert f3 == 8 and np.max(np.abs(np.array(ev3) - exact[:8]) / exact[:8]) < 1e-10
+
+
+def test_matrix_market_and_csr64_operators(rcg, tmp_path):
+    import scipy.io
+    import scipy.sparse as sp
+    print("case: operators from a Matrix Market file and from 64-bit CSR", flush=True)
+    A = ops.laplacian_3d(10, bc="dirichlet", potential=ops.anderson_potential(1000, 2.0, 5))
+    S = A.to_scipy()
+    path = tmp_path / "a.mtx"
+    scipy.io.mmwrite(str(path), sp.tril(S).tocoo(), symmetry="symmetric")
+    op_mm = rcg.CsrOperator.from_matrix_market(path)
+    op64 = rcg.CsrOperator.from_csr64(A.n, A.rowptr.astype(np.int64), A.col.astype(np.int64), A.val)
+    ref = np.linalg.eigvalsh(S.toarray())[:6]
+    for op in (op_mm, op64):
+        rc = rcg.RayleighChebyshevLM()
+        rc.setStopCondition("RESIDUAL_ONLY")
+        ev = []
+        assert rc.getMinEigenSystem(6, 1e-10, 6, 6, None, op, rcg.RandomizeOp(0), ev, None) == 6
+        assert np.max(np.abs(np.array(ev) - ref) / np.abs(ref)) < 1e-10
