@@ -134,6 +134,43 @@ def cpu_baseline(A, gpu_counts, max_inner=1):
     }
 
 
+def cpu_baseline_dense(A, gpu_counts, b):
+    """The dense workload on the host cores: each filter degree is one n x n by
+    n x b product, timed with numpy's BLAS on the full block (two products),
+    the block orthonormalisation with the oracle's MGS; both scaled by the GPU
+    solve's counts (degrees, products of the Rayleigh-Ritz steps, Lanczos
+    matrix-vector products, orthonormalisations)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = A.shape[0]
+    A = np.ascontiguousarray(A)  # symmetric: row- and column-major agree
+    rng = np.random.default_rng(0)
+    X = rng.uniform(-1.0, 1.0, (n, b))
+    t0 = time.time()
+    for _ in range(2):
+        X = A @ X
+        X /= np.linalg.norm(X, axis=0)
+    per_deg = (time.time() - t0) / 2
+    t1 = time.time()
+    orc.mgs(X)
+    per_ortho = time.time() - t1
+    products = gpu_counts["degrees"] + 2 * gpu_counts["rr"]
+    est = (per_deg * products + per_ortho * gpu_counts["ortho"]
+           + per_deg / b * gpu_counts["lanczos"])
+    return {
+        "value": K / est,
+        "unit": "eigenpairs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"host cores ({threads} BLAS threads): two n x n by n x {b} products with "
+                   f"numpy ({per_deg:.2f} s each) and the oracle's MGS of the {n} x {b} block "
+                   f"({per_ortho:.2f} s), scaled by the GPU solve's {products} block products, "
+                   f"{gpu_counts['ortho']} orthonormalisations and {gpu_counts['lanczos']} "
+                   f"Lanczos steps to an estimated {est:.0f} s solve"),
+    }
+
+
 def kernel_only(args):
     """Filter steps alone (for rocprofv3 counter runs) plus a device-to-device
     copy of the 64-column block (a plain copy kernel of known size) whose
@@ -215,6 +252,12 @@ def serving_bench(args):
                         for o, r in zip(outs, reqs)]) * 1e3
         bad = [i for i, o in enumerate(outs) if o["status"] != 0 or o["found"] < reqs[i].k]
         found = sum(o["found"] for o in outs)
+        # per-request host wall time by phase (admitted to done), averaged
+        res = [o["result"] for o in outs]
+        phases = {k: round(1e3 * float(np.mean([getattr(r, k) for r in res])), 2)
+                  for k in ("t_total", "t_lanczos", "t_filter", "t_ortho", "t_rr")}
+        phases.update({k: round(float(np.mean([getattr(r, k) for r in res])), 1)
+                       for k in ("lanczos_iters", "count_eig", "filter_degree_sum")})
         for t in tickets:
             srv.release(t)
         srv.close()
@@ -223,7 +266,7 @@ def serving_bench(args):
                 "p99_ms": round(float(np.percentile(lat, 99)), 2), "failed": len(bad),
                 "filter_batches": st["filter_batches"],
                 "requests_per_batch": round(st["batched_requests"] / max(st["filter_batches"], 1), 2),
-                "hbm_peak_gb": round(st["hbm_peak"] / 2**30, 2)}
+                "hbm_peak_gb": round(st["hbm_peak"] / 2**30, 2), "per_request_mean": phases}
 
     log("serving (Poisson arrivals) ...")
     offered = serve(False)
@@ -490,7 +533,8 @@ def main():
         counts = {"degrees": int(last.filter_degree_sum), "ortho": int(last.count_ortho),
                   "rr": int(last.count_eig)}
         if WL.get("dense"):
-            line["cpu_baseline"] = cpu_baseline(np.asfortranarray(Ad), counts, args.cpu_iters)
+            counts["lanczos"] = int(last.lanczos_iters)
+            line["cpu_baseline"] = cpu_baseline_dense(Ad, counts, INC + BUF)
         else:
             line["cpu_baseline"] = cpu_baseline(A, counts, args.cpu_iters)
     print(json.dumps(line), flush=True)

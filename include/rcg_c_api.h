@@ -66,10 +66,15 @@ int rcg_device_count(int* count);
 /* ----------------------------------------------------------- context */
 /* Device + stream binding for every handle made from it.  hip_stream may be
  * NULL (the library creates a default-priority stream).  The reference runs
- * on the legacy default stream (LMC). */
+ * every launch on the legacy default stream (RayleighChebyshevLMCuda.h). */
 int rcg_context_create(int device, void* hip_stream, rcg_context* out);
 int rcg_context_destroy(rcg_context ctx);
 int rcg_context_synchronize(rcg_context ctx);
+/* No counterpart in the reference.  Solve buffers, workspaces and pinned
+ * staging are cached by the library after use (hipFree would synchronise
+ * the device and stall concurrent solves); at most 24 GiB per device are
+ * kept.  This frees every cached buffer now. */
+int rcg_memory_trim(void);
 
 /* ------------------------------------------------------------ blocks */
 /* AmatrixClassLMCuda(m, n) / resize / create_cuda_memory
@@ -115,7 +120,7 @@ int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32
  * MFMA block products (FP64, or FP32 for the mixed-precision filter). */
 int rcg_operator_dense_create(rcg_context ctx, int64_t n, const double* A, int64_t lda, int mem,
                               rcg_operator* out);
-/* Operator ingest (SURVEY-listed formats around the path).  The reference's
+/* Operator ingest.  The reference's
  * CSR index width is the includer's RAY_INT (RayleighChebyshevLMCuda.h:409ff):
  * this takes 64-bit row pointers and columns and narrows them to the int32
  * the kernels read (RCG_ERR_UNSUPPORTED beyond 2^31 - 1). */

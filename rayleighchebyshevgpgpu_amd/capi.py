@@ -68,6 +68,7 @@ def lib() -> C.CDLL:
         "rcg_context_create": ([i32, vp, P(vp)], i32),
         "rcg_context_destroy": ([vp], i32),
         "rcg_context_synchronize": ([vp], i32),
+        "rcg_memory_trim": ([], i32),
         "rcg_block_create": ([vp, i64, i64, i32, P(vp)], i32),
         "rcg_block_destroy": ([vp], i32),
         "rcg_block_upload": ([vp, vp, i64, i64], i32),

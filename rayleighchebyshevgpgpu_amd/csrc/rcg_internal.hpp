@@ -196,6 +196,13 @@ void launch_lz_apply_dot(int64_t n, const int* rowptr, const int* col, const dou
 void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* alpha_k,
                          double* beta_k, double* ws, size_t ws_elems, hipStream_t st);
 size_t lz_ws_elems(int64_t n);
+// One fused Lanczos step for real CSR operators: reads (v, w), writes (vn, wn),
+// alpha_k and beta_k; *cnt must be zero before the first call (each kernel
+// leaves it zero).  beta_prev == nullptr on the first step.
+void launch_lz_step(int64_t n, const int* rowptr, const int* col, const double* val,
+                    const double* v, const double* w, double* vn, double* wn,
+                    const double* beta_prev, double* alpha_k, double* beta_k, unsigned* cnt,
+                    double* ws, size_t ws_elems, hipStream_t st);
 // *out = w . v (two-phase, fixed order)
 void launch_lz_dot(int64_t n, const double* w, const double* v, double* out, double* ws,
                    size_t ws_elems, hipStream_t st);

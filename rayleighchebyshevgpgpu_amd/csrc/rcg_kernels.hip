@@ -1278,6 +1278,86 @@ __global__ __launch_bounds__(kThreads) void lz_axpy_norm_kernel(int64_t n, const
   if (threadIdx.x == 0) P[blockIdx.x] = s;
 }
 
+// The block's partial sum goes to P[blockIdx.x]; the last block to finish
+// (counted on *cnt, which it resets) adds the partials in block order -- the
+// order lz_finalize_kernel uses -- and writes the total (or its square root).
+// No block waits for another.
+__device__ void last_block_total(double s, double* P, unsigned* cnt, double* out, int take_sqrt,
+                                 double* red) {
+  __shared__ int am_last;
+  if (threadIdx.x == 0) {
+    P[blockIdx.x] = s;
+    __threadfence();
+    const unsigned t = atomicAdd(cnt, 1u);
+    am_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last) return;
+  __threadfence();
+  double a = 0.0;
+  for (int64_t b = threadIdx.x; b < (int64_t)gridDim.x; b += kThreads) a += P[b];
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) {
+    *out = take_sqrt ? sqrt(fabs(a)) : a;
+    *cnt = 0u;
+  }
+}
+
+// One Lanczos step on a ping-pong pair of vectors (LanczosMaxMinFinder.h:
+// 300-345), for real CSR operators.  The previous step's swap (when
+// |beta_{k-1}| > 1e-14: w' = v / beta, v' = -beta w) is folded into the
+// operator application, whose gathers read v / beta directly; the result
+// goes to the other pair (vn = v' + A w', wn = w'), and alpha_k = w' . vn.
+// Same arithmetic, in the same order, as lz_swap + lz_apply_dot + finalize.
+__global__ __launch_bounds__(kThreads) void lz_step_kernel(
+    int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const double* __restrict__ val, const double* __restrict__ v, const double* __restrict__ w,
+    double* __restrict__ vn, double* __restrict__ wn, const double* beta_prev, double* P,
+    unsigned* cnt, double* alpha_k) {
+  __shared__ double red[kThreads];
+  const double b = beta_prev ? *beta_prev : 0.0;
+  const bool sw = beta_prev != nullptr && fabs(b) > 1.0e-14;
+  const double rb = sw ? 1.0 / b : 1.0;
+  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
+  const int64_t i1 = min(n, i0 + kLzRows);
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    double a = 0.0;
+    if (sw) {
+      for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) a = fma(val[p], v[col[p]] * rb, a);
+    } else {
+      for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) a = fma(val[p], w[col[p]], a);
+    }
+    const double wi = sw ? v[i] * rb : w[i];
+    const double vi = sw ? w[i] * (-b) : v[i];
+    const double vnew = vi + a;
+    vn[i] = vnew;
+    wn[i] = wi;
+    s = fma(wi, vnew, s);
+  }
+  s = block_sum(s, red);
+  last_block_total(s, P, cnt, alpha_k, 0, red);
+}
+
+// v -= alpha w; beta_k = ||v|| (lz_axpy_norm + finalize in one launch)
+__global__ __launch_bounds__(kThreads) void lz_axpy_beta_kernel(int64_t n, const double* w,
+                                                                double* v, const double* alpha,
+                                                                double* P, unsigned* cnt,
+                                                                double* beta_k) {
+  __shared__ double red[kThreads];
+  const double a = -*alpha;
+  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
+  const int64_t i1 = min(n, i0 + kLzRows);
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const double vn = fma(w[i], a, v[i]);
+    v[i] = vn;
+    s = fma(vn, vn, s);
+  }
+  s = block_sum(s, red);
+  last_block_total(s, P, cnt, beta_k, 1, red);
+}
+
 inline unsigned grid_for(int64_t work) {
   return (unsigned)((work + kThreads - 1) / kThreads);
 }
@@ -1665,6 +1745,19 @@ void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* al
   hipLaunchKernelGGL(lz_axpy_norm_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, w, v,
                      alpha_k, ws);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, beta_k, 1);
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_lz_step(int64_t n, const int* rowptr, const int* col, const double* val,
+                    const double* v, const double* w, double* vn, double* wn,
+                    const double* beta_prev, double* alpha_k, double* beta_k, unsigned* cnt,
+                    double* ws, size_t ws_elems, hipStream_t st) {
+  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
+  hipLaunchKernelGGL(lz_step_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, rowptr, col,
+                     val, v, w, vn, wn, beta_prev, ws, cnt, alpha_k);
+  hipLaunchKernelGGL(lz_axpy_beta_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, wn, vn,
+                     alpha_k, ws, cnt, beta_k);
   RCG_HIP(hipGetLastError());
 }
 

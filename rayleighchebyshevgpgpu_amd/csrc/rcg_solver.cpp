@@ -40,15 +40,126 @@ inline double conj_s(double x) { return x; }
 inline cplx conj_s(cplx x) { return std::conj(x); }
 
 // ------------------------------------------------------------------ memory
+// Cached device and pinned-host allocations.  hipFree and hipHostFree
+// synchronise the whole device, so a solve that frees its buffers would wait
+// for every other solve in flight (the server runs many at once).  Buffers
+// bound to a stream are returned to a free list instead, in size classes of
+// 2^k * {1, 1.25, 1.5, 1.75}, after that stream has drained; later requests
+// of the same class (on any thread) reuse them.  At most kMaxCached bytes
+// per device (and of pinned memory) are kept; rcg_memory_trim frees them.
+class MemPool {
+ public:
+  static MemPool& get() {
+    static MemPool* p = new MemPool;  // never destroyed: no HIP calls at exit
+    return *p;
+  }
+  static size_t size_class(size_t b) {
+    if (b <= 4096) return 4096;
+    const int k = 63 - __builtin_clzll((unsigned long long)(b - 1));  // 2^k < b <= 2^(k+1)
+    const size_t base = size_t(1) << k, step = base / 4;
+    return base + ((b - base + step - 1) / step) * step;
+  }
+  // device < 0: pinned host memory
+  void* alloc(int device, size_t bytes, size_t* cap) {
+    *cap = size_class(bytes);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto it = free_.find({device, *cap});
+      if (it != free_.end() && !it->second.empty()) {
+        void* q = it->second.back();
+        it->second.pop_back();
+        cached_[device] -= *cap;
+        return q;
+      }
+    }
+    void* q = nullptr;
+    hipError_t e = raw_alloc(device, &q, *cap);
+    if (e != hipSuccess) {  // out of memory: give the cached buffers back and retry once
+      (void)hipGetLastError();
+      trim(device);
+      e = raw_alloc(device, &q, *cap);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(RCG_ERR_ALLOC, std::string(device < 0 ? "hipHostMalloc" : "hipMalloc") +
+                                       " of " + std::to_string(*cap) + " bytes failed: " +
+                                       hipGetErrorString(e));
+      }
+    }
+    return q;
+  }
+  void release(int device, void* q, size_t cap) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (cached_[device] + cap <= kMaxCached) {
+        free_[{device, cap}].push_back(q);
+        cached_[device] += cap;
+        return;
+      }
+    }
+    raw_free(device, q);
+  }
+  // frees the cached buffers of one device (device < 0: pinned), or of all
+  // (device == kAll)
+  void trim(int device = kAll) {
+    std::vector<std::pair<int, void*>> out;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      for (auto it = free_.begin(); it != free_.end();) {
+        if (device == kAll || it->first.first == device) {
+          for (void* q : it->second) out.push_back({it->first.first, q});
+          cached_[it->first.first] = 0;
+          it = free_.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    for (auto& d : out) raw_free(d.first, d.second);
+  }
+  static constexpr int kAll = -1000;
+
+ private:
+  static constexpr size_t kMaxCached = size_t(24) << 30;
+  static hipError_t raw_alloc(int device, void** q, size_t b) {
+    if (device < 0) return hipHostMalloc(q, b, hipHostMallocDefault);
+    return hipMalloc(q, b);
+  }
+  static void raw_free(int device, void* q) {
+    if (device < 0) (void)hipHostFree(q);
+    else (void)hipFree(q);
+  }
+  std::mutex mu_;
+  std::map<std::pair<int, size_t>, std::vector<void*>> free_;
+  std::map<int, size_t> cached_;
+};
+
+// A device buffer.  Bound to a stream (use_stream), it comes from and goes
+// back to the cache; the stream is drained before the memory is handed on.
+// Unbound, it is a plain hipMalloc / hipFree allocation.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  int dev = 0;
+  bool pooled = false;
+  hipStream_t stream = nullptr;
   DevBuf() = default;
+  explicit DevBuf(hipStream_t s) { use_stream(s); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
+  void use_stream(hipStream_t s) {
+    pooled = true;
+    stream = s;
+  }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pooled) {
+        (void)hipStreamSynchronize(stream);
+        MemPool::get().release(dev, p, bytes);
+      } else {
+        (void)hipFree(p);
+      }
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -56,6 +167,11 @@ struct DevBuf {
     if (b <= bytes) return;
     release();
     if (b == 0) return;
+    if (pooled) {
+      RCG_HIP(hipGetDevice(&dev));
+      p = MemPool::get().alloc(dev, b, &bytes);
+      return;
+    }
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
       p = nullptr;
@@ -69,20 +185,39 @@ struct DevBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// Pinned host staging; cached like DevBuf when bound to a stream.
 struct HostBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool pooled = false;
+  hipStream_t stream = nullptr;
   HostBuf() = default;
   HostBuf(const HostBuf&) = delete;
   HostBuf& operator=(const HostBuf&) = delete;
-  ~HostBuf() {
-    if (p) (void)hipHostFree(p);
+  ~HostBuf() { release(); }
+  void use_stream(hipStream_t s) {
+    pooled = true;
+    stream = s;
+  }
+  void release() {
+    if (p) {
+      if (pooled) {
+        (void)hipStreamSynchronize(stream);
+        MemPool::get().release(-1, p, bytes);
+      } else {
+        (void)hipHostFree(p);
+      }
+    }
+    p = nullptr;
+    bytes = 0;
   }
   void ensure(size_t b) {
     if (b <= bytes) return;
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    bytes = 0;
+    release();
+    if (pooled) {
+      p = MemPool::get().alloc(-1, b, &bytes);
+      return;
+    }
     RCG_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
     bytes = b;
   }
@@ -101,6 +236,16 @@ struct rcg_context_s {
   DevBuf ws;      // reduction partials
   DevBuf small;   // b x b matrices and b-vectors on the device
   HostBuf pinned; // host staging of the same
+  void bind_scratch() {
+    ws.use_stream(stream);
+    small.use_stream(stream);
+    pinned.use_stream(stream);
+  }
+  void release_scratch() {
+    ws.release();
+    small.release();
+    pinned.release();
+  }
 };
 
 struct rcg_block_s {
@@ -213,6 +358,10 @@ class FilterBatcher {
   FilterBatcher(int device, int window_us) : device_(device), window_us_(window_us) {
     RCG_HIP(hipSetDevice(device_));
     RCG_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    hdesc_.use_stream(st_);
+    htask_.use_stream(st_);
+    ddesc_.use_stream(st_);
+    dtask_.use_stream(st_);
     RCG_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming | hipEventBlockingSync));
     thr_ = std::thread([this] { loop(); });
   }
@@ -224,6 +373,8 @@ class FilterBatcher {
     cv_.notify_all();
     if (thr_.joinable()) thr_.join();
     (void)hipStreamSynchronize(st_);
+    for (DevBuf* b : {&ddesc_, &dtask_}) b->release();  // bound to st_
+    for (HostBuf* b : {&hdesc_, &htask_}) b->release();
     (void)hipEventDestroy(uploaded_);
     (void)hipStreamDestroy(st_);
   }
@@ -361,6 +512,8 @@ class Engine {
     st_ = ctx->stream;
     cx_ = op && op->dtype == RCG_C128;
     cw_ = cx_ ? 2 : 1;
+    coef_.use_stream(st_);
+    hcoef_.use_stream(st_);
   }
   bool cx() const { return cx_; }
   int cw() const { return cw_; }
@@ -1042,8 +1195,10 @@ class Engine {
     // complex vectors are 2n interleaved reals: the start draws, norms,
     // scalings and the real-alpha axpy act on them as real vectors
     const int64_t nr = cw_ * n_;
-    DevBuf vecs, ab;
-    vecs.ensure(sizeof(double) * (size_t)nr * 2);
+    // real CSR operators run the fused step on a ping-pong pair (v, w), (v2, w2)
+    const bool fused = !cx_ && !op_->is_dense();
+    DevBuf vecs(st_), ab(st_);
+    vecs.ensure(sizeof(double) * (size_t)nr * (fused ? 4 : 2));
     ab.ensure(sizeof(double) * (size_t)(3 * itmax + 8));
     double* v = vecs.as<double>();
     double* w = v + nr;
@@ -1051,9 +1206,62 @@ class Engine {
     double* beta = alpha + itmax;
     double* alpha_im = beta + itmax;
     double* scratch = alpha_im + itmax;
+    unsigned* cnt = reinterpret_cast<unsigned*>(scratch + 4);
+    RCG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), st_));
     rcg::launch_randomize_cols<double>(nr, v, 1, 0, 1, seed, *counter, st_);
     *counter += (uint64_t)nr;
     rcg::launch_lz_start(nr, v, w, ws(), ws_elems_, scratch, st_);
+    double* pv[2] = {v, fused ? w + nr : nullptr};
+    double* pw[2] = {w, fused ? w + 2 * nr : nullptr};
+    auto step = [&](long k) {
+      if (fused) {
+        const int c = (int)(k & 1);
+        rcg::launch_lz_step(n_, op_->rowptr, op_->col, op_->val, pv[c], pw[c], pv[1 - c],
+                            pw[1 - c], k == 0 ? nullptr : beta + (k - 1), alpha + k, beta + k, cnt,
+                            ws(), ws_elems_, st_);
+        return;
+      }
+      if (k != 0) rcg::launch_lz_swap(nr, v, w, beta + (k - 1), st_);
+      if (op_->is_dense()) {
+        rcg::launch_dense_step<double>(n_, op_->dense, op_->lda, w, nullptr, v, 1, 1, 1.0, 1.0,
+                                       0.0, rcg::kStepApplyBeta, nullptr, nullptr, st_);
+        rcg::launch_lz_dot(n_, w, v, alpha + k, ws(), ws_elems_, st_);
+      } else {
+        rcg::launch_lz_apply_dot_c(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k,
+                                   alpha_im + k, ws(), ws_elems_, st_);
+      }
+      rcg::launch_lz_axpy_norm(nr, w, v, alpha + k, beta + k, ws(), ws_elems_, st_);
+    };
+    // Steps go out in chunks of 5 (the convergence test's period); each full
+    // chunk's alpha/beta are copied to one of two host slots behind an
+    // event.  The next chunk is issued before the host waits for the
+    // current one, so the device keeps stepping while the host tests; steps
+    // past the stopping point only move the internal vectors.
+    hipEvent_t cev[2] = {nullptr, nullptr};
+    const unsigned evf = hipEventDisableTiming | (ctx_->blocking_ev ? hipEventBlockingSync : 0u);
+    for (auto& e : cev) RCG_HIP(hipEventCreateWithFlags(&e, evf));
+    struct EvGuard {
+      hipEvent_t* e;
+      ~EvGuard() {
+        for (int i = 0; i < 2; ++i)
+          if (e[i]) (void)hipEventDestroy(e[i]);
+      }
+    } evg{cev};
+    long issued = 0;
+    auto issue_chunk = [&]() {
+      const long k0 = issued, k1 = std::min(itmax, k0 + 5);
+      for (long q = k0; q < k1; ++q) step(q);
+      issued = k1;
+      if (k1 - k0 == 5) {
+        double* h = hlz() + 24 * ((k0 / 5) & 1);
+        RCG_HIP(hipMemcpyAsync(h, alpha + k0, sizeof(double) * 5, hipMemcpyDeviceToHost, st_));
+        RCG_HIP(hipMemcpyAsync(h + 8, beta + k0, sizeof(double) * 5, hipMemcpyDeviceToHost, st_));
+        if (cx_)
+          RCG_HIP(hipMemcpyAsync(h + 16, alpha_im + k0, sizeof(double) * 5, hipMemcpyDeviceToHost,
+                                 st_));
+        RCG_HIP(hipEventRecord(cev[(k0 / 5) & 1], st_));
+      }
+    };
     std::vector<double> ha(itmax), hb(itmax);
     double minDiff = 1.0e10, maxDiff = 1.0e10, lmin = 0.0, lmax = 0.0;
     double rfMax = 9.0, rfMin = 9.0, rateMax = 1.0, rateMin = 1.0;
@@ -1061,32 +1269,19 @@ class Engine {
     int diffFlag = 0, rateFlag = 0;
     long k = 0;
     bool done = false;
+    if (itmax > 0) issue_chunk();
     while (k < itmax && !done) {
-      if (k != 0) rcg::launch_lz_swap(nr, v, w, beta + (k - 1), st_);
-      if (op_->is_dense()) {
-        rcg::launch_dense_step<double>(n_, op_->dense, op_->lda, w, nullptr, v, 1, 1, 1.0, 1.0,
-                                       0.0, rcg::kStepApplyBeta, nullptr, nullptr, st_);
-        rcg::launch_lz_dot(n_, w, v, alpha + k, ws(), ws_elems_, st_);
-      } else if (cx_)
-        rcg::launch_lz_apply_dot_c(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k,
-                                   alpha_im + k, ws(), ws_elems_, st_);
-      else
-        rcg::launch_lz_apply_dot(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k, ws(),
-                                 ws_elems_, st_);
-      rcg::launch_lz_axpy_norm(nr, w, v, alpha + k, beta + k, ws(), ws_elems_, st_);
-      k += 1;
-      if (k % 5 == 0) {
-        RCG_HIP(hipMemcpyAsync(hlz(), alpha + (k - 5), sizeof(double) * 5, hipMemcpyDeviceToHost, st_));
-        RCG_HIP(hipMemcpyAsync(hlz() + 8, beta + (k - 5), sizeof(double) * 5, hipMemcpyDeviceToHost, st_));
-        if (cx_)
-          RCG_HIP(hipMemcpyAsync(hlz() + 16, alpha_im + (k - 5), sizeof(double) * 5,
-                                 hipMemcpyDeviceToHost, st_));
-        sync();
+      const long k0 = k;
+      if (issued < itmax) issue_chunk();  // the next chunk, ahead of the test
+      k = std::min(itmax, k0 + 5);
+      if (k - k0 == 5) {
+        const double* h = hlz() + 24 * ((k0 / 5) & 1);
+        RCG_HIP(hipEventSynchronize(cev[(k0 / 5) & 1]));
         for (int q = 0; q < 5; ++q) {
-          ha[k - 5 + q] = hlz()[q];
-          hb[k - 5 + q] = hlz()[8 + q];
+          ha[k - 5 + q] = h[q];
+          hb[k - 5 + q] = h[8 + q];
           // LanczosMaxMinFinder.h:350-356: a non-real <w, Aw> means A is not Hermitian
-          if (cx_ && std::abs(hlz()[16 + q]) > (1.0 + std::abs(hlz()[q])) * 1.0e-12)
+          if (cx_ && std::abs(h[16 + q]) > (1.0 + std::abs(h[q])) * 1.0e-12)
             throw Error(RCG_ERR_NOT_HERMITIAN, "Lanczos max/min finder: operator is not Hermitian");
         }
         std::vector<double> D(ha.begin(), ha.begin() + k), E(hb.begin(), hb.begin() + (k - 1));
@@ -1179,6 +1374,7 @@ class Solver {
          FilterBatcher* batcher = nullptr)
       : op_(op), p_(p), eng_(ctx ? ctx : op->ctx, op, op->n) {
     eng_.set_batcher(batcher);
+    for (DevBuf* b : {&Vb_, &Tb_, &Wb_, &Yb_, &Mb_}) b->use_stream(eng_.stream());
   }
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
@@ -1561,7 +1757,7 @@ class Solver {
     const int64_t n = op_->n;
     const int64_t ldy = eng_.ldy();
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)ldy;
-    DevBuf AY, T2;
+    DevBuf AY(eng_.stream()), T2(eng_.stream());
     AY.ensure(bytes);
     T2.ensure(bytes);
     Engine e2(eng_.context(), op_, n);
@@ -1695,9 +1891,11 @@ void server_worker(rcg_server_s* S, int w) {
         b->cols = t->res.found;
         b->ld = block_ld(b->cols);
         b->dtype = t->op->dtype;
+        b->mem.use_stream(ctx->stream);
         b->mem.ensure(rcg::dtype_size(b->dtype) * (size_t)b->n * (size_t)b->ld);
         s.export_vectors(b.get());
         b->ctx = S->ctx;
+        b->mem.stream = S->ctx->stream;  // the caller's work on it goes there from now
         t->evecs = b.release();
       }
       RCG_HIP(hipEventRecord(ctx->blocking_ev, ctx->stream));
@@ -1791,6 +1989,7 @@ int rcg_context_create(int device, void* hip_stream, rcg_context* out) {
       RCG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       c->own_stream = true;
     }
+    c->bind_scratch();
     *out = c.release();
   });
 }
@@ -1800,9 +1999,14 @@ int rcg_context_destroy(rcg_context ctx) {
     if (!ctx) return;
     DeviceGuard g(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->release_scratch();  // while the stream they are bound to exists
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });
+}
+
+int rcg_memory_trim(void) {
+  return guarded([&] { MemPool::get().trim(); });
 }
 
 int rcg_context_synchronize(rcg_context ctx) {
@@ -2067,7 +2271,7 @@ int rcg_cheb_filter_mixed(rcg_operator op, rcg_block X, rcg_block tmp, int64_t c
     pc.eng.attach(static_cast<double*>(X->d()), static_cast<double*>(tmp->d()),
                   static_cast<double*>(tmp->d()), X->ld, nullptr, 0);
     const size_t fb = sizeof(float) * (size_t)X->n * (size_t)X->ld;
-    DevBuf mb;
+    DevBuf mb(pc.eng.stream());
     mb.ensure(3 * fb);
     RCG_HIP(hipMemsetAsync(mb.p, 0, 3 * fb, pc.eng.stream()));
     float* m0 = mb.as<float>();
@@ -2241,6 +2445,7 @@ int rcg_server_create(rcg_context ctx, int64_t hbm_budget, int32_t max_active, i
       c->own_stream = true;
       RCG_HIP(hipEventCreateWithFlags(&c->blocking_ev,
                                       hipEventBlockingSync | hipEventDisableTiming));
+      c->bind_scratch();
       S->wctx.push_back(std::move(c));
     }
     rcg_server_s* raw = S.get();
@@ -2360,6 +2565,7 @@ int rcg_server_destroy(rcg_server s) {
       if (kv.second->evecs) delete kv.second->evecs;
     for (auto& c : s->wctx) {
       (void)hipStreamSynchronize(c->stream);
+      c->release_scratch();
       (void)hipEventDestroy(c->blocking_ev);
       (void)hipStreamDestroy(c->stream);
     }

@@ -42,7 +42,10 @@ def test_cpp_caller_solves():
     print(p.stdout, p.stderr)
     assert p.returncode == 0
     lines = p.stdout.strip().splitlines()
-    rows = np.array([[float(t) for t in ln.split()] for ln in lines[:-1]])
+    rows = np.array([[float(t) for t in ln.split()] for ln in lines[:-2]])
     assert rows.shape == (10, 5)
     assert np.max(np.abs(rows[:, 0] - rows[:, 1])) < 1e-10
+    # the RayleighChebyshev (host-vector) interval solve with the caller's bounds
+    words = lines[-2].split()
+    assert words[0] == "interval" and words[4] == words[6]
     assert lines[-1].startswith("found 10 ")

@@ -201,3 +201,15 @@ def test_lanczos_bounds_vs_oracle(rcg, orc):
             assert cnt.value == A.n
             assert abs(mx.value - omx) < 1e-11 * abs(omx)
             assert abs(mn.value - omn) < 1e-9 * max(1.0, abs(omn))
+        # an iteration cap that is not a multiple of the test period (5): the
+        # last partial chunk runs untested, as in the reference loop
+        for itmax in (7, 12):
+            cnt = C.c_uint64(0)
+            mn, mx, it = C.c_double(), C.c_double(), C.c_int64()
+            capi.check(capi.lib().rcg_lanczos_bounds(op.h, 1e-6, 3, itmax, 0, C.byref(cnt),
+                                                     C.byref(mn), C.byref(mx), C.byref(it)))
+            omn, omx, oit = orc.lanczos_bounds(A, seed=0, tol=1e-6, max_flag=True, min_flag=True,
+                                               itmax=itmax)
+            assert it.value == oit == itmax
+            assert abs(mx.value - omx) <= 1e-11 * abs(omx)
+            assert abs(mn.value - omn) <= 1e-9 * max(1.0, abs(omn))
