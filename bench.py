@@ -245,6 +245,7 @@ def serving_bench(args):
         outs = [srv.wait(t) for t in tickets]
         makespan = time.perf_counter() - start
         st = srv.stats()
+        fp = srv.filter_profile()
         # latency = completion - arrival, on the library's steady clock
         t_first = min(o["t_submit"] for o in outs)
         arrival0 = t_first - (0.0 if saturate else reqs[0].arrival)
@@ -266,7 +267,10 @@ def serving_bench(args):
                 "p99_ms": round(float(np.percentile(lat, 99)), 2), "failed": len(bad),
                 "filter_batches": st["filter_batches"],
                 "requests_per_batch": round(st["batched_requests"] / max(st["filter_batches"], 1), 2),
-                "hbm_peak_gb": round(st["hbm_peak"] / 2**30, 2), "per_request_mean": phases}
+                "hbm_peak_gb": round(st["hbm_peak"] / 2**30, 2), "per_request_mean": phases,
+                "shared_filter": {"device_s": round(fp["device_s"], 3), "launches": fp["launches"],
+                                  "alg_GBps": round(fp["alg_bytes"] / max(fp["device_s"], 1e-9) / 1e9, 1),
+                                  "busy_fraction": round(fp["device_s"] / makespan, 3)}}
 
     log("serving (Poisson arrivals) ...")
     offered = serve(False)

@@ -285,6 +285,10 @@ int rcg_server_release(rcg_server s, int64_t ticket);
 /* stats[6]: completed, filter batches, requests in them, HBM in use, HBM
  * peak, queued. */
 int rcg_server_stats(rcg_server s, int64_t* stats);
+/* out[4] about the shared filter launches: device seconds of the finished
+ * batches (first launch to last), kernel launches, algorithmic bytes of the
+ * launched steps, batches timed. */
+int rcg_server_filter_profile(rcg_server s, double* out);
 /* Finishes the running solves, drops queued ones. */
 int rcg_server_destroy(rcg_server s);
 

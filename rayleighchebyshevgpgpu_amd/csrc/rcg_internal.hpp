@@ -221,4 +221,5 @@ void launch_pair_sums(double* s, int nc, hipStream_t st);
 // real Gram of interleaved blocks (2f x 2b) -> real embedding of Y^H V, in place.
 void launch_herm_embed(double* G, int f, int b, hipStream_t st);
 
+
 }  // namespace rcg

@@ -342,27 +342,43 @@ double wall() {
 // host thread) and runs them together: for every step j one launch per
 // launch class covers the row blocks of all requests whose degree reaches j.
 // A request's producer stream records `ready`; the batcher's stream waits on
-// it, runs the steps and records `done`, on which the producer stream waits
-// in turn -- no host synchronisation on either side.
+// it, runs the steps and records `done` after the request's last step, on
+// which the producer stream waits in turn -- no host synchronisation on
+// either side.  Up to kDepth batches are in flight on the batcher's stream,
+// each with its own slot of task tables, so the next batch is gathered and
+// enqueued while the previous one runs; with nothing in flight the batcher
+// waits up to window_us for more requests.
 struct FilterReq {
   rcg::BatchDesc desc{};
   int cls = 0;
   bool f32 = false, src = false, cx = false;
   int jstart = 1;
+  double step_bytes = 0;  // algorithmic bytes of one step (instrumentation)
   hipEvent_t ready = nullptr, done = nullptr;
   bool enqueued = false;
 };
 
 class FilterBatcher {
+  static constexpr int kDepth = 2;            // batches in flight
+  static constexpr int kSlots = kDepth + 1;  // task-table slots
+  struct Slot {
+    DevBuf ddesc, dtask;
+    HostBuf hdesc, htask;
+    hipEvent_t end = nullptr;  // the slot's last batch has finished
+    bool used = false;
+  };
+
  public:
   FilterBatcher(int device, int window_us) : device_(device), window_us_(window_us) {
     RCG_HIP(hipSetDevice(device_));
     RCG_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-    hdesc_.use_stream(st_);
-    htask_.use_stream(st_);
-    ddesc_.use_stream(st_);
-    dtask_.use_stream(st_);
-    RCG_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming | hipEventBlockingSync));
+    for (Slot& sl : slots_) {
+      sl.hdesc.use_stream(st_);
+      sl.htask.use_stream(st_);
+      sl.ddesc.use_stream(st_);
+      sl.dtask.use_stream(st_);
+      RCG_HIP(hipEventCreateWithFlags(&sl.end, hipEventDisableTiming | hipEventBlockingSync));
+    }
     thr_ = std::thread([this] { loop(); });
   }
   ~FilterBatcher() {
@@ -373,12 +389,23 @@ class FilterBatcher {
     cv_.notify_all();
     if (thr_.joinable()) thr_.join();
     (void)hipStreamSynchronize(st_);
-    for (DevBuf* b : {&ddesc_, &dtask_}) b->release();  // bound to st_
-    for (HostBuf* b : {&hdesc_, &htask_}) b->release();
-    (void)hipEventDestroy(uploaded_);
+    {
+      std::lock_guard<std::mutex> g(prof_mu_);
+      harvest_locked(true);
+      for (hipEvent_t e : ev_free_) (void)hipEventDestroy(e);
+      ev_free_.clear();
+    }
+    for (Slot& sl : slots_) {  // bound to st_
+      sl.ddesc.release();
+      sl.dtask.release();
+      sl.hdesc.release();
+      sl.htask.release();
+      (void)hipEventDestroy(sl.end);
+    }
     (void)hipStreamDestroy(st_);
   }
-  // Solves that may submit; the batcher launches as soon as all of them wait.
+  // Solves that may submit; with nothing in flight the batcher launches as
+  // soon as all of them wait.
   void attach() {
     std::lock_guard<std::mutex> g(mu_);
     ++members_;
@@ -401,27 +428,61 @@ class FilterBatcher {
   }
   int64_t batches() const { return batches_; }
   int64_t requests() const { return requests_; }
+  // out[4]: device seconds spent in the timed batches (first launch to last),
+  // kernel launches, algorithmic bytes of the launched steps, timed batches
+  void profile(double* out) {
+    std::lock_guard<std::mutex> g(prof_mu_);
+    harvest_locked(false);
+    out[0] = busy_s_;
+    out[1] = (double)launches_;
+    out[2] = bytes_;
+    out[3] = (double)timed_;
+  }
 
  private:
+  // drops finished batches from the in-flight list (the oldest first)
+  void retire_finished() {
+    while (!inflight_.empty() && hipEventQuery(slots_[inflight_.front()].end) == hipSuccess)
+      inflight_.pop_front();
+    (void)hipGetLastError();
+  }
+
   void loop() {
     (void)hipSetDevice(device_);
     std::unique_lock<std::mutex> g(mu_);
+    int next_slot = 0;
     while (true) {
       cv_.wait(g, [&] { return stop_ || !pending_.empty(); });
       if (stop_ && pending_.empty()) return;
-      // gather: until every attached solve waits here, or the window closes
-      const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(window_us_);
-      cv_.wait_until(g, until, [&] { return stop_ || (int)pending_.size() >= members_; });
+      retire_finished();
+      if ((int)inflight_.size() >= kDepth) {
+        // the device has work queued: wait for the oldest batch while more
+        // requests gather
+        const hipEvent_t e = slots_[inflight_.front()].end;
+        g.unlock();
+        (void)hipEventSynchronize(e);
+        g.lock();
+        continue;
+      }
+      if (inflight_.empty()) {
+        // the device is idle: gather until every attached solve waits here,
+        // or the window closes
+        const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(window_us_);
+        cv_.wait_until(g, until, [&] { return stop_ || (int)pending_.size() >= members_; });
+      }
       std::vector<FilterReq*> batch;
       batch.swap(pending_);
+      const int slot = next_slot;
+      next_slot = (next_slot + 1) % kSlots;
       g.unlock();
       std::string err;
       try {
-        run(batch);
+        run(batch, slots_[slot]);
       } catch (const std::exception& e) {
         err = e.what();
       }
       g.lock();
+      if (err.empty()) inflight_.push_back(slot);
       if (!err.empty()) error_ = err;
       for (FilterReq* r : batch) r->enqueued = err.empty();
       ++batches_;
@@ -430,8 +491,21 @@ class FilterBatcher {
     }
   }
 
-  void run(std::vector<FilterReq*>& batch) {
+  void run(std::vector<FilterReq*>& batch, Slot& sl) {
     for (FilterReq* r : batch) RCG_HIP(hipStreamWaitEvent(st_, r->ready, 0));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    {
+      std::lock_guard<std::mutex> g(prof_mu_);
+      harvest_locked(false);
+      for (hipEvent_t* e : {&t0, &t1}) {
+        if (!ev_free_.empty()) {
+          *e = ev_free_.back();
+          ev_free_.pop_back();
+        } else {
+          RCG_HIP(hipEventCreate(e));
+        }
+      }
+    }
     // groups of one launch kind, requests by degree descending
     std::stable_sort(batch.begin(), batch.end(), [](const FilterReq* a, const FilterReq* b) {
       const int ka = (a->f32 ? 1000 : 0) + (a->cx ? 100 : 0) + a->cls;
@@ -450,43 +524,90 @@ class FilterBatcher {
     first_task[batch.size()] = (int)tasks.size();
     const size_t db = sizeof(rcg::BatchDesc) * batch.size();
     const size_t tb = sizeof(rcg::BatchTask) * tasks.size();
-    RCG_HIP(hipEventSynchronize(uploaded_));  // the staging copy of the previous batch is done
-    // growing the device tables frees them: the previous batch's steps must be done
-    if (db > ddesc_.bytes || tb > dtask_.bytes) RCG_HIP(hipStreamSynchronize(st_));
-    hdesc_.ensure(db);
-    htask_.ensure(tb);
-    ddesc_.ensure(db);
-    dtask_.ensure(tb);
-    for (size_t q = 0; q < batch.size(); ++q) hdesc_.as<rcg::BatchDesc>()[q] = batch[q]->desc;
-    std::memcpy(htask_.p, tasks.data(), tb);
-    RCG_HIP(hipMemcpyAsync(ddesc_.p, hdesc_.p, db, hipMemcpyHostToDevice, st_));
-    RCG_HIP(hipMemcpyAsync(dtask_.p, htask_.p, tb, hipMemcpyHostToDevice, st_));
-    RCG_HIP(hipEventRecord(uploaded_, st_));
-    const rcg::BatchTask* dt = dtask_.as<rcg::BatchTask>();
+    // the slot's previous batch (kSlots batches ago) has finished: at most
+    // kDepth are in flight
+    if (sl.used) RCG_HIP(hipEventSynchronize(sl.end));
+    sl.used = true;
+    sl.hdesc.ensure(db);
+    sl.htask.ensure(tb);
+    sl.ddesc.ensure(db);
+    sl.dtask.ensure(tb);
+    for (size_t q = 0; q < batch.size(); ++q) sl.hdesc.as<rcg::BatchDesc>()[q] = batch[q]->desc;
+    std::memcpy(sl.htask.p, tasks.data(), tb);
+    RCG_HIP(hipMemcpyAsync(sl.ddesc.p, sl.hdesc.p, db, hipMemcpyHostToDevice, st_));
+    RCG_HIP(hipMemcpyAsync(sl.dtask.p, sl.htask.p, tb, hipMemcpyHostToDevice, st_));
+    const rcg::BatchTask* dt = sl.dtask.as<rcg::BatchTask>();
+    const rcg::BatchDesc* dd = sl.ddesc.as<rcg::BatchDesc>();
+    // groups of one launch kind; within a group, requests by degree
+    // descending, so step j covers a prefix of the group
+    struct Group {
+      size_t g0, end;
+      int jstart;
+    };
+    std::vector<Group> groups;
+    int mmax = 0, jmin = 1 << 30;
     for (size_t g0 = 0; g0 < batch.size();) {
       size_t g1 = g0;
       const FilterReq* h = batch[g0];
       while (g1 < batch.size() && batch[g1]->f32 == h->f32 && batch[g1]->cx == h->cx &&
              batch[g1]->cls == h->cls)
         ++g1;
-      // requests g0..g1-1 by degree descending: step j covers a prefix
-      const int mmax = batch[g0]->desc.m;
-      size_t end = g1;
-      for (int j = h->jstart; j <= mmax; ++j) {
-        while (end > g0 && batch[end - 1]->desc.m < j) --end;
-        const int nt = first_task[end] - first_task[g0];
-        rcg::launch_cheb_batch(ddesc_.as<rcg::BatchDesc>(), dt + first_task[g0], nt, j, h->f32,
-                               h->src, h->cx, h->cls, st_);
-      }
+      groups.push_back({g0, g1, h->jstart});
+      mmax = std::max(mmax, h->desc.m);
+      jmin = std::min(jmin, h->jstart);
       g0 = g1;
     }
-    for (FilterReq* r : batch) RCG_HIP(hipEventRecord(r->done, st_));
+    // a request's `done` is recorded right after its own last step, so a
+    // short sweep does not wait for the longest one in its batch
+    for (FilterReq* r : batch)
+      if (r->desc.m < r->jstart) RCG_HIP(hipEventRecord(r->done, st_));
+    RCG_HIP(hipEventRecord(t0, st_));
+    int64_t nl = 0;
+    double nb = 0;
+    for (int j = jmin; j <= mmax; ++j) {
+      for (Group& gr : groups) {
+        if (j < gr.jstart) continue;
+        while (gr.end > gr.g0 && batch[gr.end - 1]->desc.m < j) --gr.end;
+        if (gr.end == gr.g0) continue;
+        const FilterReq* h = batch[gr.g0];
+        const int nt = first_task[gr.end] - first_task[gr.g0];
+        rcg::launch_cheb_batch(dd, dt + first_task[gr.g0], nt, j, h->f32, h->src, h->cx, h->cls,
+                               st_);
+        ++nl;
+        for (size_t q = gr.g0; q < gr.end; ++q) nb += batch[q]->step_bytes;
+      }
+      for (FilterReq* r : batch)
+        if (r->desc.m == j && j >= r->jstart) RCG_HIP(hipEventRecord(r->done, st_));
+    }
+    RCG_HIP(hipEventRecord(t1, st_));
+    RCG_HIP(hipEventRecord(sl.end, st_));
+    std::lock_guard<std::mutex> g(prof_mu_);
+    timing_.push_back({t0, t1});
+    launches_ += nl;
+    bytes_ += nb;
+  }
+
+  // adds the intervals of finished batches (all of them when `wait`)
+  void harvest_locked(bool wait) {
+    while (!timing_.empty()) {
+      auto pr = timing_.front();
+      if (wait) (void)hipEventSynchronize(pr.second);
+      if (hipEventQuery(pr.second) != hipSuccess) break;
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+        busy_s_ += 1e-3 * ms;
+        ++timed_;
+      }
+      (void)hipGetLastError();
+      ev_free_.push_back(pr.first);
+      ev_free_.push_back(pr.second);
+      timing_.pop_front();
+    }
   }
 
   int device_;
   int window_us_;
   hipStream_t st_ = nullptr;
-  hipEvent_t uploaded_ = nullptr;
   std::thread thr_;
   std::mutex mu_;
   std::condition_variable cv_, cv_done_;
@@ -495,8 +616,13 @@ class FilterBatcher {
   bool stop_ = false;
   std::string error_;
   int64_t batches_ = 0, requests_ = 0;
-  DevBuf ddesc_, dtask_;
-  HostBuf hdesc_, htask_;
+  Slot slots_[kSlots];
+  std::deque<int> inflight_;  // slots of enqueued, unfinished batches
+  std::mutex prof_mu_;
+  std::deque<std::pair<hipEvent_t, hipEvent_t>> timing_;
+  std::vector<hipEvent_t> ev_free_;
+  double busy_s_ = 0, bytes_ = 0;
+  int64_t launches_ = 0, timed_ = 0;
 };
 
 // ------------------------------------------------------------------ engine
@@ -573,6 +699,7 @@ class Engine {
   double* hvecUp() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + bb_; }
   double* hvecDown() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + 2 * bb_; }
   double* hlz() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + 3 * bb_; }
+
   double* ws() { return ctx_->ws.as<double>(); }
 
   // Candidate row-block orders: contiguous blocks with and without the XCD
@@ -757,6 +884,13 @@ class Engine {
     r.cx = cx;
     r.jstart = jstart;
     r.cls = rcg::csr_class(d.ncv);
+    {
+      const double vs = (f32 ? 4.0 : 8.0) * (cx ? 2.0 : 1.0);  // bytes per matrix value
+      const double xs = f32 ? 4.0 : 8.0;                       // bytes per block element
+      const double cols = (double)last_panel_;
+      r.step_bytes = (double)op_->nnz * (vs + 4.0) + 4.0 * (double)(n_ + 1) +
+                     (double)n_ * cols * xs * (f32 ? 4.0 : 3.0);
+    }
     r.ready = ev_ready_;
     r.done = ev_done_;
     RCG_HIP(hipEventRecord(ev_ready_, st_));
@@ -839,7 +973,9 @@ class Engine {
     // kap rows j = 0..m [ld floats each]
     const size_t nd = 2 * (size_t)rb + (size_t)ld_;
     const size_t bytes = sizeof(double) * nd + sizeof(float) * (size_t)(m + 1) * (size_t)ld_;
-    sync();  // the pinned table of the previous call is no longer in flight
+    // the pinned table of the previous call is no longer in flight
+    if (coef_ev_) RCG_HIP(hipEventSynchronize(coef_ev_));
+    else RCG_HIP(hipEventCreateWithFlags(&coef_ev_, hipEventDisableTiming));
     coef_.ensure(bytes);
     hcoef_.ensure(bytes);
     double* hd = hcoef_.as<double>();
@@ -861,6 +997,7 @@ class Engine {
       hd[2 * rb + col] = u1;  // u_m
     }
     RCG_HIP(hipMemcpyAsync(coef_.p, hcoef_.p, bytes, hipMemcpyHostToDevice, st_));
+    RCG_HIP(hipEventRecord(coef_ev_, st_));
     const double* dcp = coef_.as<double>();
     const double* dum = dcp + 2 * rb;
     const float* dk = reinterpret_cast<const float*>(dcp + nd);
@@ -966,6 +1103,7 @@ class Engine {
       if (e) (void)hipEventDestroy(e);
     if (ev_ready_) (void)hipEventDestroy(ev_ready_);
     if (ev_done_) (void)hipEventDestroy(ev_done_);
+    if (coef_ev_) (void)hipEventDestroy(coef_ev_);
   }
   int last_panel() const { return last_panel_; }
   const std::string& last_sched() const { return last_sched_; }
@@ -1064,6 +1202,7 @@ class Engine {
     cholqr_pass<S>(b, true);
     if (!clean) cholqr_pass<S>(b, false);
   }
+
 
   // OrthogonalizeAtoB (RayleighChebyshevLM.h:1844-1917) against Y[:, :found]:
   // V -= Y (Y^H V), twice.  The normalisation of the reference is implied by
@@ -1354,6 +1493,7 @@ class Engine {
   float* R32_ = nullptr;              // FP32 residual block (mixed filter)
   std::vector<double> last_norms_;    // column norms of V at the last residuals
   DevBuf coef_;                       // mixed filter coefficient table
+  hipEvent_t coef_ev_ = nullptr;      // its upload from hcoef_ has finished
   HostBuf hcoef_;
   hipEvent_t fev_[2] = {nullptr, nullptr};
   FilterBatcher* batcher_ = nullptr;
@@ -2547,6 +2687,13 @@ int rcg_server_stats(rcg_server s, int64_t* stats) {
     stats[3] = s->used;
     stats[4] = s->peak;
     stats[5] = (int64_t)s->queue.size();
+  });
+}
+
+int rcg_server_filter_profile(rcg_server s, double* out) {
+  return guarded([&] {
+    need(s && out, "NULL argument");
+    s->batcher->profile(out);
   });
 }
 

@@ -138,6 +138,13 @@ class Server:
         return {"completed": int(s[0]), "filter_batches": int(s[1]), "batched_requests": int(s[2]),
                 "hbm_in_use": int(s[3]), "hbm_peak": int(s[4]), "queued": int(s[5])}
 
+    def filter_profile(self) -> dict:
+        """Device time, launches and algorithmic bytes of the shared filter launches."""
+        o = np.zeros(4)
+        check(lib().rcg_server_filter_profile(self.h, o.ctypes.data_as(C.c_void_p)))
+        return {"device_s": float(o[0]), "launches": int(o[1]), "alg_bytes": float(o[2]),
+                "batches_timed": int(o[3])}
+
     def close(self):
         if getattr(self, "h", None):
             lib().rcg_server_destroy(self.h)
