@@ -529,7 +529,10 @@ int64_t gram_blocks(int64_t n, int bx, int by, int64_t* rows_per_blk) {
   const int ntiles = ((bx + kTile - 1) / kTile) * ((by + kTile - 1) / kTile);
   int64_t target = std::max<int64_t>(1, 1024 / ntiles);
   int64_t rpb = round_up((n + target - 1) / target, kRb);
-  rpb = std::max<int64_t>(rpb, kRb);
+  // at least 4 rows per output column, so a block's bx x by partial is at
+  // most a quarter of the rows it reads (small n would otherwise write about
+  // as many partial bytes as it reads)
+  rpb = std::max<int64_t>(rpb, round_up(4 * (int64_t)std::max(bx, by), kRb));
   *rows_per_blk = rpb;
   return (n + rpb - 1) / rpb;
 }
@@ -635,10 +638,11 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
   // four 16-row groups per pass: all of their loads are issued before the
-  // MFMAs that consume them
+  // MFMAs that consume them, and the next pass's loads before this pass's
+  // MFMAs
   constexpr int U = 4;
-  for (int64_t ib = i0 + 4 * wave; ib < i1; ib += 16 * U) {
-    double xa[U][4], yb[U][4];
+  double xa[U][4], yb[U][4], xn[U][4], yn[U][4];
+  auto load = [&](int64_t ib, double (&xv)[U][4], double (&yv)[U][4]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t row = ib + 16 * u + kr;
@@ -646,16 +650,22 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int c = a0 + 16 * t + cc;
-        xa[u][t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
+        xv[u][t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
       }
       if constexpr (!SYM) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int c = b0 + 16 * t + cc;
-          yb[u][t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
+          yv[u][t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
         }
       }
     }
+  };
+  int64_t ib = i0 + 4 * wave;
+  if (ib < i1) load(ib, xa, yb);
+  for (; ib < i1; ib += 16 * U) {
+    const bool more = ib + 16 * U < i1;
+    if (more) load(ib + 16 * U, xn, yn);
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -665,6 +675,15 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
           if (SYM && p > q) continue;
           acc[p][q] = mfma16x16x4(xa[u][p], SYM ? xa[u][q] : yb[u][q], acc[p][q]);
         }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          xa[u][t] = xn[u][t];
+          if constexpr (!SYM) yb[u][t] = yn[u][t];
+        }
+    }
   }
   // the four waves' tiles are summed in LDS in wave order
   for (int w = 0; w < 4; ++w) {
@@ -1143,7 +1162,7 @@ __global__ void herm_embed_kernel(double* G, int f, int b) {
 }
 
 // ----------------------------------------------------------------- Lanczos
-constexpr int kLzRows = 2048;  // rows per partial block
+constexpr int kLzRows = 512;  // rows per partial block
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   red[threadIdx.x] = v;
