@@ -341,10 +341,9 @@ double wall() {
 // Collects the filter sweeps of concurrent solves (each on its own stream and
 // host thread) and runs them together: for every step j one launch per
 // launch class covers the row blocks of all requests whose degree reaches j.
-// A request's producer stream records `ready`; the batcher's stream waits on
-// it, runs the steps and records `done` after the request's last step, on
-// which the producer stream waits in turn -- no host synchronisation on
-// either side.  Up to kDepth batches are in flight on the batcher's stream,
+// A solve submits once its stream has drained (or, with `ready`, the
+// batcher's stream waits on that event); the batcher records `done` after the
+// request's last step.  Up to kDepth batches are in flight on the batcher's stream,
 // each with its own slot of task tables, so the next batch is gathered and
 // enqueued while the previous one runs; with nothing in flight the batcher
 // waits up to window_us for more requests.
@@ -492,7 +491,8 @@ class FilterBatcher {
   }
 
   void run(std::vector<FilterReq*>& batch, Slot& sl) {
-    for (FilterReq* r : batch) RCG_HIP(hipStreamWaitEvent(st_, r->ready, 0));
+    for (FilterReq* r : batch)
+      if (r->ready) RCG_HIP(hipStreamWaitEvent(st_, r->ready, 0));
     hipEvent_t t0 = nullptr, t1 = nullptr;
     {
       std::lock_guard<std::mutex> g(prof_mu_);
@@ -870,13 +870,13 @@ class Engine {
     step_end();
   }
 
-  // Serving: the steps run in the shared batcher, ordered against this
-  // engine's stream by events.
+  // Serving: the steps run in the shared batcher.  The two sides are
+  // ordered by host waits, not by device-side event waits: a stream waiting
+  // on an event holds up the other streams that share its hardware queue.
   void batch_submit(const rcg::BatchDesc& d, bool f32, bool cx, int jstart) {
-    if (!ev_ready_) {
-      RCG_HIP(hipEventCreateWithFlags(&ev_ready_, hipEventDisableTiming));
-      RCG_HIP(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming));
-    }
+    if (!ev_done_)
+      RCG_HIP(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming | hipEventBlockingSync));
+    sync();  // the block is ready
     FilterReq r;
     r.desc = d;
     r.f32 = f32;
@@ -891,11 +891,9 @@ class Engine {
       r.step_bytes = (double)op_->nnz * (vs + 4.0) + 4.0 * (double)(n_ + 1) +
                      (double)n_ * cols * xs * (f32 ? 4.0 : 3.0);
     }
-    r.ready = ev_ready_;
     r.done = ev_done_;
-    RCG_HIP(hipEventRecord(ev_ready_, st_));
     batcher_->submit(&r);
-    RCG_HIP(hipStreamWaitEvent(st_, ev_done_, 0));
+    RCG_HIP(hipEventSynchronize(ev_done_));
   }
   rcg::BatchDesc batch_desc(long m, double g1, double g2, int64_t rb, int w) {
     rcg::BatchDesc d{};
@@ -1101,7 +1099,6 @@ class Engine {
   ~Engine() {
     for (auto& e : fev_)
       if (e) (void)hipEventDestroy(e);
-    if (ev_ready_) (void)hipEventDestroy(ev_ready_);
     if (ev_done_) (void)hipEventDestroy(ev_done_);
     if (coef_ev_) (void)hipEventDestroy(coef_ev_);
   }
@@ -1497,7 +1494,7 @@ class Engine {
   HostBuf hcoef_;
   hipEvent_t fev_[2] = {nullptr, nullptr};
   FilterBatcher* batcher_ = nullptr;
-  hipEvent_t ev_ready_ = nullptr, ev_done_ = nullptr;
+  hipEvent_t ev_done_ = nullptr;  // the batcher has run this engine's steps
   bool step_pending_ = false;
   long pend_launches_ = 0, step_launches_ = 0;
   double pend_bytes_ = 0.0, step_bytes_ = 0.0, step_seconds_ = 0.0;
