@@ -210,6 +210,29 @@ def kernel_only(args):
                       "copy_bytes_read": blk, "copy_bytes_written": blk}), flush=True)
 
 
+def serving_cpu_baseline(reqs, inc_buf, budget_s=15.0):
+    """The oracle (FP64, OpenMP) on the stream's requests in order until
+    budget_s of CPU solving has passed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    found = done = 0
+    for r in reqs:
+        inc, buf = inc_buf(r.k)
+        o = orc.solve(r.operator(), r.k, r.tol, inc, buf, seed=0, stop="RESIDUAL_ONLY")
+        found += int(o["found"])
+        done += 1
+        if time.time() - t0 > budget_s:
+            break
+    wall = time.time() - t0
+    return {"value": round(found / wall, 3), "unit": "eigenpairs/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"CPU oracle (restated RayleighChebyshevLM, OpenMP, {threads} threads), the "
+                       f"first {done} requests of the same stream one by one, FP64: {found} "
+                       f"eigenpairs in {wall:.1f} s (operator construction included)")}
+
+
 def serving_bench(args):
     """Batched serving: the request stream through rcg_server (mixed precision,
     continuous batching) against the plain FP64 path solving requests one by
@@ -272,12 +295,14 @@ def serving_bench(args):
                                   "alg_GBps": round(fp["alg_bytes"] / max(fp["device_s"], 1e-9) / 1e9, 1),
                                   "busy_fraction": round(fp["device_s"] / makespan, 3)}}
 
-    log("serving (Poisson arrivals) ...")
-    offered = serve(False)
-    log(f"offered-load run: {offered}")
-    saturated = serve(True) if args.saturate else None
-    if saturated:
-        log(f"saturated run: {saturated}")
+    offered = saturated = None
+    if not args.plain_only:
+        log("serving (Poisson arrivals) ...")
+        offered = serve(False)
+        log(f"offered-load run: {offered}")
+        saturated = serve(True) if args.saturate else None
+        if saturated:
+            log(f"saturated run: {saturated}")
     # plain FP64 path: the first requests one by one through rcg_solve_lowest
     m = min(args.plain_requests, len(reqs))
     tp = time.perf_counter()
@@ -292,7 +317,15 @@ def serving_bench(args):
     ctx.synchronize()
     plain_s = time.perf_counter() - tp
     plain = found_plain / plain_s
+    log(f"plain FP64: {m} requests in {plain_s:.3f} s, {plain:.1f} eigenpairs/s")
+    if args.plain_only:
+        print(json.dumps({"plain_fp64": {"requests": m, "eigenpairs": found_plain,
+                                         "seconds": round(plain_s, 3), "value": round(plain, 2)}}),
+              flush=True)
+        return
     best = saturated or offered
+    sf = best["shared_filter"]
+    cpu = None if args.no_cpu else serving_cpu_baseline(reqs, inc_buf)
     line = {
         "metric": METRIC,
         "value": best["value"],
@@ -317,8 +350,13 @@ def serving_bench(args):
         "plain_fp64": {"requests": m, "eigenpairs": found_plain, "seconds": round(plain_s, 3),
                        "value": round(plain, 2)},
         "speedup_vs_plain_fp64": round(best["value"] / plain, 2) if plain > 0 else None,
-        "roofline": None,
-        "cpu_baseline": None,
+        # the shared filter launches: algorithmic bytes over their device time
+        "roofline": {"bound": "hbm", "achieved": sf["alg_GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(sf["alg_GBps"] / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "csr_batch_kernel<float,...,SRC> (batched FP32 correction steps)",
+                     "launches": sf["launches"], "device_s": sf["device_s"]},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
@@ -340,6 +378,8 @@ def main():
     ap.add_argument("--hbm-budget", type=int, default=200, help="serving: GiB for admitted solves")
     ap.add_argument("--saturate", action="store_true",
                     help="serving: also run with every request queued at once (capacity)")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="serving: only the plain FP64 path (one request at a time)")
     ap.add_argument("--plain-requests", type=int, default=256,
                     help="serving: requests solved one by one on the plain FP64 path")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solves")

@@ -273,6 +273,13 @@ int rcg_server_create(rcg_context ctx, int64_t hbm_budget, int32_t max_active, i
 /* Queue a solve (params as rcg_solve_lowest; random start vectors). */
 int rcg_server_submit(rcg_server s, rcg_operator op, const rcg_solve_params* params,
                       int64_t* ticket);
+/* Warm start (setNonRandomStartFlag across requests, LM:957-985): the solve
+ * starts from the resident eigenvectors of the finished ticket start_ticket
+ * (same operator size and dtype), e.g. a nearby operator of a parameter
+ * sweep.  start_ticket cannot be released until this solve has finished.
+ * params->nonrandom_start must be 0 (the count comes from the start ticket). */
+int rcg_server_submit_warm(rcg_server s, rcg_operator op, const rcg_solve_params* params,
+                           int64_t start_ticket, int64_t* ticket);
 /* Wait for a ticket: result, eigenvalues and residuals (capacity cap), and
  * times[3] = submit, admit, completion (seconds, steady clock).  Returns the
  * solve's own status. */

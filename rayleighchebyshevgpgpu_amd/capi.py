@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
         "rcg_server_vectors": ([vp, i64, P(vp)], i32),
         "rcg_server_release": ([vp, i64], i32),
         "rcg_server_stats": ([vp, vp], i32),
+        "rcg_server_submit_warm": ([vp, vp, vp, i64, vp], i32),
         "rcg_server_filter_profile": ([vp, vp], i32),
         "rcg_server_destroy": ([vp], i32),
         "rcg_sizeof_solve_params": ([], i64),

@@ -1162,7 +1162,9 @@ __global__ void herm_embed_kernel(double* G, int f, int b) {
 }
 
 // ----------------------------------------------------------------- Lanczos
-constexpr int kLzRows = 512;  // rows per partial block
+// rows per partial block of the Lanczos kernels: 512 below 2^20 rows (more
+// blocks for small operators), 2048 from there (fewer partials to reduce)
+inline int64_t lz_rows(int64_t n) { return n >= (int64_t(1) << 20) ? 2048 : 512; }
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   red[threadIdx.x] = v;
@@ -1177,10 +1179,10 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // v = v/||v||, w = v, v = 0 (LanczosMaxMinFinder.h:272-279) -- two phases.
-__global__ __launch_bounds__(kThreads) void lz_sumsq_kernel(int64_t n, const double* v, double* P) {
+__global__ __launch_bounds__(kThreads) void lz_sumsq_kernel(int64_t n, const double* v, double* P, int64_t rpb) {
   __shared__ double red[kThreads];
-  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
-  const int64_t i1 = min(n, i0 + kLzRows);
+  const int64_t i0 = blockIdx.x * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
   double s = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s += v[i] * v[i];
   s = block_sum(s, red);
@@ -1220,10 +1222,10 @@ __global__ __launch_bounds__(kThreads) void lz_apply_dot_kernel(int64_t n,
                                                                 const int* __restrict__ col,
                                                                 const double* __restrict__ val,
                                                                 const double* __restrict__ w,
-                                                                double* v, double* P) {
+                                                                double* v, double* P, int64_t rpb) {
   __shared__ double red[kThreads];
-  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
-  const int64_t i1 = min(n, i0 + kLzRows);
+  const int64_t i0 = blockIdx.x * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
   double s = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
     double a = 0.0;
@@ -1239,10 +1241,10 @@ __global__ __launch_bounds__(kThreads) void lz_apply_dot_kernel(int64_t n,
 __global__ __launch_bounds__(kThreads) void lz_apply_dot_c_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const double* __restrict__ val, const double* __restrict__ w, double* v, double* P,
-    int64_t nblk) {
+    int64_t nblk, int64_t rpb) {
   __shared__ double red[kThreads];
-  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
-  const int64_t i1 = min(n, i0 + kLzRows);
+  const int64_t i0 = blockIdx.x * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
   double sr = 0.0, si = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
     double ar = 0.0, ai = 0.0;
@@ -1269,10 +1271,10 @@ __global__ __launch_bounds__(kThreads) void lz_apply_dot_c_kernel(
 
 // partial sum w . v (the dense operator's Lanczos alpha)
 __global__ __launch_bounds__(kThreads) void lz_dot_kernel(int64_t n, const double* w,
-                                                          const double* v, double* P) {
+                                                          const double* v, double* P, int64_t rpb) {
   __shared__ double red[kThreads];
-  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
-  const int64_t i1 = min(n, i0 + kLzRows);
+  const int64_t i0 = blockIdx.x * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
   double s = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(w[i], v[i], s);
   s = block_sum(s, red);
@@ -1282,11 +1284,11 @@ __global__ __launch_bounds__(kThreads) void lz_dot_kernel(int64_t n, const doubl
 // v += (-alpha) w; partial |v|^2
 __global__ __launch_bounds__(kThreads) void lz_axpy_norm_kernel(int64_t n, const double* w,
                                                                 double* v, const double* alpha,
-                                                                double* P) {
+                                                                double* P, int64_t rpb) {
   __shared__ double red[kThreads];
   const double a = -*alpha;
-  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
-  const int64_t i1 = min(n, i0 + kLzRows);
+  const int64_t i0 = blockIdx.x * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
   double s = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
     const double vn = fma(w[i], a, v[i]);
@@ -1332,13 +1334,13 @@ __global__ __launch_bounds__(kThreads) void lz_step_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const double* __restrict__ val, const double* __restrict__ v, const double* __restrict__ w,
     double* __restrict__ vn, double* __restrict__ wn, const double* beta_prev, double* P,
-    unsigned* cnt, double* alpha_k) {
+    unsigned* cnt, double* alpha_k, int64_t rpb) {
   __shared__ double red[kThreads];
   const double b = beta_prev ? *beta_prev : 0.0;
   const bool sw = beta_prev != nullptr && fabs(b) > 1.0e-14;
   const double rb = sw ? 1.0 / b : 1.0;
-  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
-  const int64_t i1 = min(n, i0 + kLzRows);
+  const int64_t i0 = blockIdx.x * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
   double s = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
     double a = 0.0;
@@ -1362,11 +1364,11 @@ __global__ __launch_bounds__(kThreads) void lz_step_kernel(
 __global__ __launch_bounds__(kThreads) void lz_axpy_beta_kernel(int64_t n, const double* w,
                                                                 double* v, const double* alpha,
                                                                 double* P, unsigned* cnt,
-                                                                double* beta_k) {
+                                                                double* beta_k, int64_t rpb) {
   __shared__ double red[kThreads];
   const double a = -*alpha;
-  const int64_t i0 = blockIdx.x * (int64_t)kLzRows;
-  const int64_t i1 = min(n, i0 + kLzRows);
+  const int64_t i0 = blockIdx.x * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
   double s = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
     const double vn = fma(w[i], a, v[i]);
@@ -1683,13 +1685,14 @@ void launch_fill_cols(int64_t n, T* X, int ld, int c0, int cnt, T value, hipStre
   RCG_HIP(hipGetLastError());
 }
 
-size_t lz_ws_elems(int64_t n) { return 2 * (size_t)((n + kLzRows - 1) / kLzRows) + 4; }
+size_t lz_ws_elems(int64_t n) { return 2 * (size_t)((n + lz_rows(n) - 1) / lz_rows(n)) + 4; }
 
 void launch_lz_dot(int64_t n, const double* w, const double* v, double* out, double* ws,
                    size_t ws_elems, hipStream_t st) {
-  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  const int64_t rpb = lz_rows(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
   if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
-  hipLaunchKernelGGL(lz_dot_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, w, v, ws);
+  hipLaunchKernelGGL(lz_dot_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, w, v, ws, rpb);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, out, 0);
   RCG_HIP(hipGetLastError());
 }
@@ -1723,10 +1726,11 @@ void launch_herm_embed(double* G, int f, int b, hipStream_t st) {
 void launch_lz_apply_dot_c(int64_t n, const int* rowptr, const int* col, const double* val,
                            const double* w, double* v, double* alpha_k, double* alpha_im_k,
                            double* ws, size_t ws_elems, hipStream_t st) {
-  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  const int64_t rpb = lz_rows(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
   if ((size_t)(2 * nblk) > ws_elems) throw Error(3, "Lanczos workspace too small");
   hipLaunchKernelGGL(lz_apply_dot_c_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, rowptr,
-                     col, val, w, v, ws, nblk);
+                     col, val, w, v, ws, nblk, rpb);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, alpha_k, 0);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws + nblk, nblk, alpha_im_k, 0);
   RCG_HIP(hipGetLastError());
@@ -1736,9 +1740,10 @@ void launch_lz_apply_dot_c(int64_t n, const int* rowptr, const int* col, const d
 
 void launch_lz_start(int64_t n, double* v, double* w, double* ws, size_t ws_elems,
                      double* scratch, hipStream_t st) {
-  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  const int64_t rpb = lz_rows(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
   if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
-  hipLaunchKernelGGL(lz_sumsq_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, v, ws);
+  hipLaunchKernelGGL(lz_sumsq_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, v, ws, rpb);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, scratch, 1);
   hipLaunchKernelGGL(lz_init_kernel, dim3(grid_for(n)), dim3(kThreads), 0, st, n, v, w, scratch);
   RCG_HIP(hipGetLastError());
@@ -1750,19 +1755,21 @@ void launch_lz_swap(int64_t n, double* v, double* w, const double* beta_prev, hi
 void launch_lz_apply_dot(int64_t n, const int* rowptr, const int* col, const double* val,
                          const double* w, double* v, double* alpha_k, double* ws, size_t ws_elems,
                          hipStream_t st) {
-  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  const int64_t rpb = lz_rows(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
   if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
   hipLaunchKernelGGL(lz_apply_dot_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, rowptr,
-                     col, val, w, v, ws);
+                     col, val, w, v, ws, rpb);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, alpha_k, 0);
   RCG_HIP(hipGetLastError());
 }
 void launch_lz_axpy_norm(int64_t n, const double* w, double* v, const double* alpha_k,
                          double* beta_k, double* ws, size_t ws_elems, hipStream_t st) {
-  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  const int64_t rpb = lz_rows(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
   if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
   hipLaunchKernelGGL(lz_axpy_norm_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, w, v,
-                     alpha_k, ws);
+                     alpha_k, ws, rpb);
   hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, beta_k, 1);
   RCG_HIP(hipGetLastError());
 }
@@ -1771,12 +1778,13 @@ void launch_lz_step(int64_t n, const int* rowptr, const int* col, const double* 
                     const double* v, const double* w, double* vn, double* wn,
                     const double* beta_prev, double* alpha_k, double* beta_k, unsigned* cnt,
                     double* ws, size_t ws_elems, hipStream_t st) {
-  const int64_t nblk = (n + kLzRows - 1) / kLzRows;
+  const int64_t rpb = lz_rows(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
   if ((size_t)nblk > ws_elems) throw Error(3, "Lanczos workspace too small");
   hipLaunchKernelGGL(lz_step_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, rowptr, col,
-                     val, v, w, vn, wn, beta_prev, ws, cnt, alpha_k);
+                     val, v, w, vn, wn, beta_prev, ws, cnt, alpha_k, rpb);
   hipLaunchKernelGGL(lz_axpy_beta_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, wn, vn,
-                     alpha_k, ws, cnt, beta_k);
+                     alpha_k, ws, cnt, beta_k, rpb);
   RCG_HIP(hipGetLastError());
 }
 

@@ -1957,6 +1957,8 @@ struct Ticket {
   rcg_block_s* evecs = nullptr;
   int status = RCG_OK;
   std::string error;
+  Ticket* start = nullptr;  // warm start: a finished ticket whose vectors start this solve
+  int pins = 0;             // running solves that start from this ticket's vectors
 };
 
 int64_t solve_footprint(const rcg_operator_s* op, const rcg_solve_params& p) {
@@ -2015,6 +2017,11 @@ void server_worker(rcg_server_s* S, int w) {
     }
     try {
       Solver s(t->op, t->p, ctx, S->batcher.get());
+      if (t->start) {  // resident vectors of a finished request (LM:957-985)
+        s.start_ = static_cast<const double*>(t->start->evecs->d());
+        s.start_ld_ = t->start->evecs->ld;
+        s.start_cols_ = t->p.nonrandom_start;
+      }
       std::vector<double> ev, rr;
       s.run(&t->res, ev, rr);
       t->res.trace_len = (int64_t)s.trace_.size();
@@ -2047,6 +2054,10 @@ void server_worker(rcg_server_s* S, int w) {
     S->batcher->detach();
     {
       std::lock_guard<std::mutex> g(S->mu);
+      if (t->start) {
+        --t->start->pins;
+        t->start = nullptr;
+      }
       S->used -= t->footprint;
       t->state = 2;
       t->t_done = now_s();
@@ -2591,30 +2602,58 @@ int rcg_server_create(rcg_context ctx, int64_t hbm_budget, int32_t max_active, i
   });
 }
 
+namespace {
+void server_enqueue(rcg_server s, rcg_operator op, const rcg_solve_params* params,
+                    int64_t start_ticket, int64_t* ticket) {
+  need(s && op && params && ticket, "NULL argument");
+  need(params->k >= 1, "eigCount must be >= 1");
+  need(params->inc >= 0 && params->buf >= 0 && params->inc + params->buf >= 1,
+       "subspace increment + buffer must be >= 1");
+  need(params->filter_dtype == RCG_F64 || params->filter_dtype == RCG_F32,
+       "filter_dtype must be RCG_F64 or RCG_F32");
+  need(params->nonrandom_start == 0,
+       "served solves start from random vectors or, with rcg_server_submit_warm, from a "
+       "finished ticket's vectors");
+  need(op->ctx->device == s->ctx->device, "operator on another device");
+  auto t = std::make_unique<Ticket>();
+  t->op = op;
+  t->p = *params;
+  t->footprint = solve_footprint(op, *params);
+  t->t_submit = now_s();
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (start_ticket > 0) {
+      auto it = s->tickets.find(start_ticket);
+      need(it != s->tickets.end(), "unknown start ticket");
+      Ticket* st = it->second.get();
+      need(st->state == 2 && st->status == RCG_OK && st->evecs != nullptr,
+           "the start ticket has not finished with eigenvectors");
+      need(st->evecs->n == op->n && st->evecs->dtype == op->dtype,
+           "the start ticket's vectors do not match the operator (rows, dtype)");
+      t->start = st;
+      t->p.nonrandom_start = (int32_t)std::min<int64_t>(
+          st->evecs->cols, std::max<int64_t>(params->k, params->inc + params->buf));
+      ++st->pins;
+    }
+    t->id = s->next_id++;
+    *ticket = t->id;
+    s->queue.push_back(t.get());
+    s->tickets[t->id] = std::move(t);
+  }
+  s->cv.notify_all();
+}
+}  // namespace
+
 int rcg_server_submit(rcg_server s, rcg_operator op, const rcg_solve_params* params,
                       int64_t* ticket) {
+  return guarded([&] { server_enqueue(s, op, params, 0, ticket); });
+}
+
+int rcg_server_submit_warm(rcg_server s, rcg_operator op, const rcg_solve_params* params,
+                           int64_t start_ticket, int64_t* ticket) {
   return guarded([&] {
-    need(s && op && params && ticket, "NULL argument");
-    need(params->k >= 1, "eigCount must be >= 1");
-    need(params->inc >= 0 && params->buf >= 0 && params->inc + params->buf >= 1,
-         "subspace increment + buffer must be >= 1");
-    need(params->filter_dtype == RCG_F64 || params->filter_dtype == RCG_F32,
-         "filter_dtype must be RCG_F64 or RCG_F32");
-    need(params->nonrandom_start == 0, "served solves start from random vectors");
-    need(op->ctx->device == s->ctx->device, "operator on another device");
-    auto t = std::make_unique<Ticket>();
-    t->op = op;
-    t->p = *params;
-    t->footprint = solve_footprint(op, *params);
-    t->t_submit = now_s();
-    {
-      std::lock_guard<std::mutex> g(s->mu);
-      t->id = s->next_id++;
-      *ticket = t->id;
-      s->queue.push_back(t.get());
-      s->tickets[t->id] = std::move(t);
-    }
-    s->cv.notify_all();
+    need(start_ticket > 0, "bad start ticket");
+    server_enqueue(s, op, params, start_ticket, ticket);
   });
 }
 
@@ -2667,6 +2706,7 @@ int rcg_server_release(rcg_server s, int64_t ticket) {
       auto it = s->tickets.find(ticket);
       need(it != s->tickets.end(), "unknown ticket");
       need(it->second->state == 2, "the solve has not finished");
+      need(it->second->pins == 0, "the ticket's vectors are the start of a queued or running solve");
       t = std::move(it->second);
       s->tickets.erase(it);
     }

@@ -100,12 +100,19 @@ class Server:
         self.h = h
 
     def submit(self, op, k: int, tol: float, inc: int, buf: int, precision: str = "mixed",
-               seed: int = 0, stop: str = "RESIDUAL_ONLY", refine: bool = False) -> int:
+               seed: int = 0, stop: str = "RESIDUAL_ONLY", refine: bool = False,
+               start_ticket: int | None = None) -> int:
+        """Queue a solve; with start_ticket, it starts from that finished
+        ticket's resident eigenvectors (warm start)."""
         p = capi.SolveParams(k=int(k), inc=int(inc), buf=int(buf), tol=float(tol), seed=int(seed),
                              stop=capi.STOP[stop], filter_dtype=capi.F32 if precision == "mixed"
                              else capi.F64, refine=int(refine))
         t = C.c_int64()
-        check(lib().rcg_server_submit(self.h, op.h, C.byref(p), C.byref(t)))
+        if start_ticket is None:
+            check(lib().rcg_server_submit(self.h, op.h, C.byref(p), C.byref(t)))
+        else:
+            check(lib().rcg_server_submit_warm(self.h, op.h, C.byref(p), int(start_ticket),
+                                               C.byref(t)))
         return t.value
 
     def wait(self, ticket: int, cap: int = 4096) -> dict:

@@ -128,3 +128,50 @@ def test_failing_request_does_not_stop_others(rcg):
     assert r2["status"] == 4 and "Hermitian" in r2["error"]
     assert np.array_equal(r1["eigenvalues"], r3["eigenvalues"])
     srv.close()
+
+
+def test_warm_start_from_resident_vectors(rcg):
+    """A request started from a finished ticket's resident eigenvectors
+    (setNonRandomStartFlag across requests): the same eigenvalues in fewer
+    inner iterations, also for a nearby operator; the start ticket cannot be
+    released while the warm solve is queued or running."""
+    print("case: warm start from a finished ticket, 48x48 Dirichlet + potential", flush=True)
+    side, k = 48, 12
+    A0 = ops.laplacian_2d(side, potential=ops.anderson_potential(side * side, 1.0, 7))
+    A1 = ops.laplacian_2d(side, potential=1.001 * ops.anderson_potential(side * side, 1.0, 7))
+    op0, op1 = rcg.CsrOperator(A0), rcg.CsrOperator(A1)
+    srv = serving.Server(max_active=2, window_us=0)
+    cold = srv.wait(srv.submit(op0, k, 1e-10, k, 6, precision="fp64"))
+    assert cold["status"] == 0 and cold["found"] == k
+    t0 = srv.submit(op0, k, 1e-10, k, 6, precision="fp64")
+    first = srv.wait(t0)
+    tw = srv.submit(op0, k, 1e-10, k, 6, precision="fp64", start_ticket=t0)
+    tn = srv.submit(op1, k, 1e-10, k, 6, precision="fp64", start_ticket=t0)
+    warm, near = srv.wait(tw), srv.wait(tn)
+    assert warm["status"] == 0 and near["status"] == 0
+    floor = np.maximum(np.abs(first["eigenvalues"]), 0.1)  # the reference's relative floor
+    assert np.max(np.abs(warm["eigenvalues"] - first["eigenvalues"]) / floor) < 1e-10
+    assert warm["result"].count_eig < first["result"].count_eig
+    ref1 = np.linalg.eigvalsh(A1.to_scipy().toarray())[:k]
+    assert np.max(np.abs(near["eigenvalues"] - ref1) / np.maximum(np.abs(ref1), 0.1)) < 1e-10
+    assert near["result"].count_eig < first["result"].count_eig
+    srv.release(tw)
+    srv.release(tn)
+    srv.release(t0)
+    srv.close()
+
+
+def test_warm_start_rejects_bad_tickets(rcg):
+    print("case: warm start argument checks", flush=True)
+    A = ops.laplacian_2d(20)
+    B = ops.laplacian_2d(21)
+    srv = serving.Server(max_active=1, window_us=0)
+    opa, opb = rcg.CsrOperator(A), rcg.CsrOperator(B)
+    t = srv.submit(opa, 4, 1e-10, 4, 4, precision="fp64")
+    srv.wait(t)
+    with pytest.raises(Exception, match="do not match"):
+        srv.submit(opb, 4, 1e-10, 4, 4, precision="fp64", start_ticket=t)
+    with pytest.raises(Exception, match="unknown start ticket"):
+        srv.submit(opa, 4, 1e-10, 4, 4, precision="fp64", start_ticket=999)
+    srv.release(t)
+    srv.close()
