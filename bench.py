@@ -255,7 +255,9 @@ def serving_bench(args):
     def serve(saturate):
         srv = serving.Server(ctx, hbm_budget=args.hbm_budget << 30, max_active=args.slots,
                              window_us=args.window_us)
+        import resource
         tickets = []
+        ru0 = resource.getrusage(resource.RUSAGE_SELF)
         start = time.perf_counter()
         start_steady = None
         for r, op in zip(reqs, ops_):
@@ -267,6 +269,8 @@ def serving_bench(args):
             tickets.append(srv.submit(op, r.k, r.tol, inc, buf, precision="mixed"))
         outs = [srv.wait(t) for t in tickets]
         makespan = time.perf_counter() - start
+        ru1 = resource.getrusage(resource.RUSAGE_SELF)
+        cpu_s = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
         st = srv.stats()
         fp = srv.filter_profile()
         # latency = completion - arrival, on the library's steady clock
@@ -291,6 +295,7 @@ def serving_bench(args):
                 "filter_batches": st["filter_batches"],
                 "requests_per_batch": round(st["batched_requests"] / max(st["filter_batches"], 1), 2),
                 "hbm_peak_gb": round(st["hbm_peak"] / 2**30, 2), "per_request_mean": phases,
+                "host_cpu_s": round(cpu_s, 2),
                 "shared_filter": {"device_s": round(fp["device_s"], 3), "launches": fp["launches"],
                                   "alg_GBps": round(fp["alg_bytes"] / max(fp["device_s"], 1e-9) / 1e9, 1),
                                   "busy_fraction": round(fp["device_s"] / makespan, 3)}}

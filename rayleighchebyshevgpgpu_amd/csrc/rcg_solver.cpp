@@ -39,6 +39,25 @@ using cplx = std::complex<double>;
 inline double conj_s(double x) { return x; }
 inline cplx conj_s(cplx x) { return std::conj(x); }
 
+// Waits for an event.  hipEventSynchronize busy-waits even for events made
+// with hipEventBlockingSync, and dozens of server threads waiting that way
+// take the host cores the solves need; a polite wait queries the event,
+// yields a few times, then sleeps in short steps.
+inline void wait_event(hipEvent_t e, bool polite) {
+  if (!polite) {
+    RCG_HIP(hipEventSynchronize(e));
+    return;
+  }
+  for (int k = 0;; ++k) {
+    const hipError_t r = hipEventQuery(e);
+    if (r == hipSuccess) return;
+    (void)hipGetLastError();  // hipErrorNotReady is not an error
+    if (r != hipErrorNotReady) RCG_HIP(r);
+    if (k < 8) std::this_thread::yield();
+    else std::this_thread::sleep_for(std::chrono::microseconds(k < 200 ? 20 : 200));
+  }
+}
+
 // ------------------------------------------------------------------ memory
 // Cached device and pinned-host allocations.  hipFree and hipHostFree
 // synchronise the whole device, so a solve that frees its buffers would wait
@@ -459,7 +478,7 @@ class FilterBatcher {
         // requests gather
         const hipEvent_t e = slots_[inflight_.front()].end;
         g.unlock();
-        (void)hipEventSynchronize(e);
+        wait_event(e, true);
         g.lock();
         continue;
       }
@@ -526,7 +545,7 @@ class FilterBatcher {
     const size_t tb = sizeof(rcg::BatchTask) * tasks.size();
     // the slot's previous batch (kSlots batches ago) has finished: at most
     // kDepth are in flight
-    if (sl.used) RCG_HIP(hipEventSynchronize(sl.end));
+    if (sl.used) wait_event(sl.end, true);
     sl.used = true;
     sl.hdesc.ensure(db);
     sl.htask.ensure(tb);
@@ -679,7 +698,7 @@ class Engine {
   void sync() {
     if (ctx_->blocking_ev) {
       RCG_HIP(hipEventRecord(ctx_->blocking_ev, st_));
-      RCG_HIP(hipEventSynchronize(ctx_->blocking_ev));
+      wait_event(ctx_->blocking_ev, true);
     } else {
       RCG_HIP(hipStreamSynchronize(st_));
     }
@@ -893,7 +912,7 @@ class Engine {
     }
     r.done = ev_done_;
     batcher_->submit(&r);
-    RCG_HIP(hipEventSynchronize(ev_done_));
+    wait_event(ev_done_, true);
   }
   rcg::BatchDesc batch_desc(long m, double g1, double g2, int64_t rb, int w) {
     rcg::BatchDesc d{};
@@ -972,7 +991,7 @@ class Engine {
     const size_t nd = 2 * (size_t)rb + (size_t)ld_;
     const size_t bytes = sizeof(double) * nd + sizeof(float) * (size_t)(m + 1) * (size_t)ld_;
     // the pinned table of the previous call is no longer in flight
-    if (coef_ev_) RCG_HIP(hipEventSynchronize(coef_ev_));
+    if (coef_ev_) wait_event(coef_ev_, ctx_->blocking_ev != nullptr);
     else RCG_HIP(hipEventCreateWithFlags(&coef_ev_, hipEventDisableTiming));
     coef_.ensure(bytes);
     hcoef_.ensure(bytes);
@@ -1412,7 +1431,7 @@ class Engine {
       k = std::min(itmax, k0 + 5);
       if (k - k0 == 5) {
         const double* h = hlz() + 24 * ((k0 / 5) & 1);
-        RCG_HIP(hipEventSynchronize(cev[(k0 / 5) & 1]));
+        wait_event(cev[(k0 / 5) & 1], ctx_->blocking_ev != nullptr);
         for (int q = 0; q < 5; ++q) {
           ha[k - 5 + q] = h[q];
           hb[k - 5 + q] = h[8 + q];
@@ -2043,7 +2062,7 @@ void server_worker(rcg_server_s* S, int w) {
         t->evecs = b.release();
       }
       RCG_HIP(hipEventRecord(ctx->blocking_ev, ctx->stream));
-      RCG_HIP(hipEventSynchronize(ctx->blocking_ev));
+      wait_event(ctx->blocking_ev, true);
     } catch (const Error& e) {
       t->status = e.code;
       t->error = e.what();
