@@ -828,12 +828,16 @@ struct MfmaAcc<float> {
 //   kApply: out = p0 A X;  kApplyBeta: out = p0 A X + p1 out;
 //   kChebFirst: out = p2 p1 (p0 A X + X);  kCheb: out = p2 (p1 (p0 A X + X) - Z);
 //   kChebSrc: kCheb + kap[c] R.
-// 64 x 64 output tile per workgroup, 4 waves in 2 x 2, each 32 x 32 (2 x 2
-// MFMA tiles); K in chunks of DBK staged in LDS, the next chunk's loads
-// issued before the current chunk's MFMAs.  Workgroups of one row tile sit
-// on one XCD so the A rows they share come from the same L2.
+// 128 x 64 output tile per workgroup; wave w owns rows [32w, 32w + 32), all
+// 64 columns: 2 x 4 MFMA 16x16x4 tiles, 6 LDS fragment reads per 8 MFMAs.
+// K runs in chunks of BK through two LDS buffers: the next chunk's global
+// loads are issued before the current chunk's MFMAs and stored into the
+// other buffer after them, one barrier per chunk.  The column tiles of a row
+// tile are consecutive logical workgroups, kept on one XCD, so the A rows
+// they share come from the same L2.
 template <typename T>
 struct DenseCfg {
+  static constexpr int BM = 128, BN = 64;
   static constexpr int BK = sizeof(T) == 8 ? 16 : 32;
 };
 template <typename T, int MODE>
@@ -842,11 +846,11 @@ __global__ __launch_bounds__(kThreads) void dense_step_kernel(
     int ld, int ncols, T p0, T p1, T p2, const T* __restrict__ R, const T* __restrict__ kap,
     int ntn) {
   constexpr int BK = DenseCfg<T>::BK;
-  constexpr int BM = 64, BN = 64;
+  constexpr int BM = DenseCfg<T>::BM, BN = DenseCfg<T>::BN;
   constexpr int AE = BM * BK / kThreads;  // A elements per thread per chunk
   constexpr int XE = BK * BN / kThreads;  // X elements per thread per chunk
-  __shared__ T As[BM][BK + 1];
-  __shared__ T Xs[BK][BN + 1];
+  __shared__ T As[2][BM][BK + 1];
+  __shared__ T Xs[2][BK][BN + 1];
   using Acc = typename MfmaAcc<T>::type;
   const int64_t L = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t ntm = (n + BM - 1) / BM;
@@ -855,12 +859,12 @@ __global__ __launch_bounds__(kThreads) void dense_step_kernel(
   const int c0 = (int)(L % ntn) * BN;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  Acc acc[2][2];
+  Acc acc[2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+    for (int b = 0; b < 4; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+  // fixed per-thread load coordinates; the chunk offset moves along K
   T ra[AE], rx[XE];
   auto fetch = [&](int64_t k0) {
 #pragma unroll
@@ -879,43 +883,50 @@ __global__ __launch_bounds__(kThreads) void dense_step_kernel(
       rx[q] = (k < n && c < ncols) ? X[k * ld + c] : T(0);
     }
   };
-  fetch(0);
-  for (int64_t k0 = 0; k0 < n; k0 += BK) {
-    __syncthreads();
+  auto store = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < AE; ++q) {
       const int e = threadIdx.x + q * kThreads;
-      As[e / BK][e % BK] = ra[q];
+      As[buf][e / BK][e % BK] = ra[q];
     }
 #pragma unroll
     for (int q = 0; q < XE; ++q) {
       const int e = threadIdx.x + q * kThreads;
-      Xs[e / BN][e % BN] = rx[q];
+      Xs[buf][e / BN][e % BN] = rx[q];
     }
-    __syncthreads();
-    if (k0 + BK < n) fetch(k0 + BK);
+  };
+  fetch(0);
+  store(0);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t k0 = 0; k0 < n; k0 += BK) {
+    const bool more = k0 + BK < n;
+    if (more) fetch(k0 + BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kk = ks + (lane >> 4);
-      T af[2], bf[2];
+      T af[2], bf[4];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) af[a] = As[32 * wm + 16 * a + (lane & 15)][kk];
+      for (int a = 0; a < 2; ++a) af[a] = As[buf][32 * wave + 16 * a + (lane & 15)][kk];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bf[b] = Xs[kk][32 * wn + 16 * b + (lane & 15)];
+      for (int b = 0; b < 4; ++b) bf[b] = Xs[buf][kk][16 * b + (lane & 15)];
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = mfma16x16x4(af[a], bf[b], acc[a][b]);
+        for (int b = 0; b < 4; ++b) acc[a][b] = mfma16x16x4(af[a], bf[b], acc[a][b]);
     }
+    if (more) store(buf ^ 1);  // the other buffer was last read before the previous barrier
+    __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int64_t i = i0 + 32 * wm + 16 * a + MfmaAcc<T>::row(j, lane);
-        const int c = c0 + 32 * wn + 16 * b + (lane & 15);
+        const int64_t i = i0 + 32 * wave + 16 * a + MfmaAcc<T>::row(j, lane);
+        const int c = c0 + 16 * b + (lane & 15);
         if (i >= n || c >= ncols) continue;
         const int64_t off = i * ld + c;
         const T v = acc[a][b][j];
@@ -1465,8 +1476,8 @@ void launch_dense_step(int64_t n, const T* A, int64_t lda, const T* X, const T* 
                        int ncols, T p0, T p1, T p2, int mode, const T* R, const T* kap,
                        hipStream_t st) {
   if (n <= 0 || ncols <= 0) return;
-  const int ntn = (ncols + 63) / 64;
-  const int64_t tiles = ((n + 63) / 64) * ntn;
+  const int ntn = (ncols + DenseCfg<T>::BN - 1) / DenseCfg<T>::BN;
+  const int64_t tiles = ((n + DenseCfg<T>::BM - 1) / DenseCfg<T>::BM) * ntn;
   const dim3 grid((unsigned)round_up(tiles, kXcds));
 #define RCG_DENSE(M)                                                                           \
   hipLaunchKernelGGL((dense_step_kernel<T, M>), grid, dim3(kThreads), 0, st, n, A, lda, X, Z, out, \

@@ -42,18 +42,18 @@ def main() -> int:
     for c0 in range(0, n, step):
         c1 = min(n, c0 + step)
         G[:, c0:c1] = torch.from_numpy(np.ascontiguousarray(gaussian_columns(n, seed, c0, c1))).to(dev)
-    print(f"[dense_gen] G drawn in {time.time() - t0:.1f}s", flush=True)
+    print(f"[dense_gen] G drawn in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     Q, R = torch.linalg.qr(G)
     del G
     Q = Q * torch.sign(torch.diagonal(R))[None, :]
     del R
     torch.cuda.synchronize()
-    print(f"[dense_gen] QR done at {time.time() - t0:.1f}s", flush=True)
+    print(f"[dense_gen] QR done at {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     lam = torch.linspace(0.0, 1.0, n, dtype=torch.float64, device=dev) ** 2
     A = (Q * lam[None, :]) @ Q.T
     A = 0.5 * (A + A.T)
     np.save(out, A.cpu().numpy())
-    print(f"[dense_gen] A written in {time.time() - t0:.1f}s", flush=True)
+    print(f"[dense_gen] A written in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     return 0
 
 
