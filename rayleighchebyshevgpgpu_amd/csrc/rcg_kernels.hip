@@ -841,7 +841,7 @@ struct DenseCfg {
   static constexpr int BK = sizeof(T) == 8 ? 16 : 32;
 };
 template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads) void dense_step_kernel(
+__global__ __launch_bounds__(kThreads, 3) void dense_step_kernel(
     int64_t n, const T* __restrict__ A, int64_t lda, const T* __restrict__ X, const T* Z, T* out,
     int ld, int ncols, T p0, T p1, T p2, const T* __restrict__ R, const T* __restrict__ kap,
     int ntn) {
@@ -864,36 +864,58 @@ __global__ __launch_bounds__(kThreads) void dense_step_kernel(
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = Acc{0, 0, 0, 0};
-  // fixed per-thread load coordinates; the chunk offset moves along K
+  // Each thread loads AE consecutive A entries of one row and XE consecutive
+  // X entries of one K row, in 16-byte vectors when they are whole and
+  // aligned (the chunk offset moves along K).
+  using VT = V16<T>;
+  constexpr int W = VT::W;
+  constexpr int TPA = BK / AE;  // threads per A row
+  constexpr int TPX = BN / XE;  // threads per X row
+  const int ar = threadIdx.x / TPA, ak = (threadIdx.x % TPA) * AE;
+  const int xk = threadIdx.x / TPX, xc = (threadIdx.x % TPX) * XE;
+  const int64_t ai = i0 + ar;
+  const bool a_vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % W == 0);
+  const bool x_vec = ((reinterpret_cast<uintptr_t>(X) & 15) == 0) && (ld % W == 0) &&
+                     (c0 + xc + XE <= ncols);
   T ra[AE], rx[XE];
   auto fetch = [&](int64_t k0) {
+    const int64_t ka = k0 + ak;
+    if (a_vec && ai < n && ka + AE <= n) {
+      const T* ap = A + ai * lda + ka;
 #pragma unroll
-    for (int q = 0; q < AE; ++q) {
-      const int e = threadIdx.x + q * kThreads;
-      const int r = e / BK, kk = e % BK;
-      const int64_t i = i0 + r, k = k0 + kk;
-      ra[q] = (i < n && k < n) ? A[i * lda + k] : T(0);
+      for (int q = 0; q < AE / W; ++q) {
+        T t[W];
+        VT::ld(ap + q * W, t);
+#pragma unroll
+        for (int w = 0; w < W; ++w) ra[q * W + w] = t[w];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < AE; ++q) ra[q] = (ai < n && ka + q < n) ? A[ai * lda + ka + q] : T(0);
     }
+    const int64_t kx = k0 + xk;
+    if (x_vec && kx < n) {
+      const T* xp = X + kx * ld + c0 + xc;
 #pragma unroll
-    for (int q = 0; q < XE; ++q) {
-      const int e = threadIdx.x + q * kThreads;
-      const int kk = e / BN, cc = e % BN;
-      const int64_t k = k0 + kk;
-      const int c = c0 + cc;
-      rx[q] = (k < n && c < ncols) ? X[k * ld + c] : T(0);
+      for (int q = 0; q < XE / W; ++q) {
+        T t[W];
+        VT::ld(xp + q * W, t);
+#pragma unroll
+        for (int w = 0; w < W; ++w) rx[q * W + w] = t[w];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < XE; ++q) {
+        const int c = c0 + xc + q;
+        rx[q] = (kx < n && c < ncols) ? X[kx * ld + c] : T(0);
+      }
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < AE; ++q) {
-      const int e = threadIdx.x + q * kThreads;
-      As[buf][e / BK][e % BK] = ra[q];
-    }
+    for (int q = 0; q < AE; ++q) As[buf][ar][ak + q] = ra[q];
 #pragma unroll
-    for (int q = 0; q < XE; ++q) {
-      const int e = threadIdx.x + q * kThreads;
-      Xs[buf][e / BN][e % BN] = rx[q];
-    }
+    for (int q = 0; q < XE; ++q) Xs[buf][xk][xc + q] = rx[q];
   };
   fetch(0);
   store(0);
