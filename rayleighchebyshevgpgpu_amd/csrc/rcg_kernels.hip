@@ -38,18 +38,20 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
 #endif
 }
 
-#ifdef RCG_NT_STREAMS
-constexpr bool kNtStreams = true;
-#else
+// Non-temporal single-use streams: 3% faster FP64 and FP32 filter steps at
+// 128^3 (the gathered rows keep more of L2); RCG_TEMPORAL_STREAMS turns them off.
+#ifdef RCG_TEMPORAL_STREAMS
 constexpr bool kNtStreams = false;
+#else
+constexpr bool kNtStreams = true;
 #endif
 
 // 16-byte vector access to a row segment.
 template <typename T>
 struct V16;
 // ld/st go through the caches normally; ld_stream/st_stream mark single-use
-// data (the X_{j-2} read and the X_j write of a step) non-temporal when
-// RCG_NT_STREAMS is set, so L2 keeps the gathered X_{j-1} rows.
+// data (the X_{j-2} read and the X_j write of a step) non-temporal, so L2
+// keeps the gathered X_{j-1} rows.
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 template <>
