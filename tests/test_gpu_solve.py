@@ -234,3 +234,61 @@ def test_cached_buffers_reused_and_trimmed(rcg):
         assert np.max(np.abs(ev - out[0][0]) / out[0][0]) < 1e-12
         assert np.array_equal(ev, out[1][0])
         assert np.array_equal(X, out[1][1])
+
+
+def test_ragged_rows_apply_and_solve(rcg):
+    """Empty rows, one row with thousands of entries (past the LDS staging of
+    a row block) and short rows in one operator: apply against scipy, the
+    lowest eigenpairs (a 10-fold zero among them) against eigvalsh."""
+    print("case: arrow matrix with empty rows, n=2000", flush=True)
+    n = 2000
+    rng = np.random.default_rng(3)
+    empty = np.arange(10, 20)
+    full = np.setdiff1d(np.arange(n), empty)
+    d = 0.5 + 4.0 * rng.random(n)
+    arms = full[full > 0]
+    rows = np.concatenate([full, np.zeros(arms.size, dtype=np.int64), arms])
+    cols = np.concatenate([full, arms, np.zeros(arms.size, dtype=np.int64)])
+    vals = np.concatenate([d[full], np.full(2 * arms.size, 0.01)])
+    A = ops._from_coo(n, rows, cols, vals)
+    assert np.all(np.diff(A.rowptr)[empty] == 0) and np.diff(A.rowptr)[0] == arms.size + 1
+    op = rcg.CsrOperator(A)
+    X = rng.standard_normal((n, 24))
+    bx, by = rcg.Block.from_host(X), rcg.Block(n, 24)
+    op.apply(bx, by, 1.0, 0.0)
+    ref = A.to_scipy() @ X
+    assert np.max(np.abs(by.download() - ref)) <= 1e-13 * np.max(np.abs(ref))
+    rc = rcg.RayleighChebyshevLM()
+    rc.setStopCondition("RESIDUAL_ONLY")
+    ev = []
+    V = rcg.Block(n, 20)
+    assert rc.getMinEigenSystem(12, 1e-10, 12, 8, None, op, rcg.RandomizeOp(0), ev, V) == 12
+    spec = np.linalg.eigvalsh(A.to_scipy().toarray())
+    exact = spec[:12]
+    assert np.max(np.abs(np.array(ev) - exact) / np.maximum(np.abs(exact), 0.1)) < 1e-10
+    Xv = V.download(12)
+    r = np.linalg.norm(A.to_scipy() @ Xv - Xv * np.array(ev), axis=0)
+    assert np.max(r) <= 1e-10 * np.max(np.abs(spec))  # ||Av - lambda v|| <= tol ||A||
+
+
+def test_tiny_operators_and_empty_interval(rcg):
+    """Subspaces clipped to n and the quick return (LM:1005-1036); an
+    interval below the spectrum finds nothing (LM:763-768)."""
+    print("case: n = 6 and 9 chains, interval below the spectrum", flush=True)
+    for n, k, inc, buf in ((6, 6, 4, 4), (6, 3, 4, 4), (9, 2, 2, 2)):
+        A = ops.laplacian_2d(n, 1)
+        exact = np.linalg.eigvalsh(A.to_scipy().toarray())
+        op = rcg.CsrOperator(A)
+        rc = rcg.RayleighChebyshevLM()
+        rc.setStopCondition("RESIDUAL_ONLY")
+        ev = []
+        f = rc.getMinEigenSystem(k, 1e-10, inc, buf, None, op, rcg.RandomizeOp(0), ev, None)
+        assert f >= k, (n, k, f)
+        assert np.max(np.abs(np.array(ev[:k]) - exact[:k]) / np.maximum(exact[:k], 0.1)) < 1e-10
+    A = ops.laplacian_2d(16)
+    op = rcg.CsrOperator(A)
+    rc = rcg.RayleighChebyshevLM()
+    ev = [1.0]
+    assert rc.getMinIntervalEigenSystem(-1.0, 1e-10, 8, 8, 20, None, op, rcg.RandomizeOp(0), ev,
+                                        None) == 0
+    assert ev == []
