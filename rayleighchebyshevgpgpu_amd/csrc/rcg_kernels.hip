@@ -964,6 +964,47 @@ __global__ __launch_bounds__(kThreads, 3) void dense_step_kernel(
       }
 }
 
+// One column of a dense operator (the dense Lanczos matrix-vector products):
+// out = p0 A x (+ p1 out with kApplyBeta).  A wave owns a row: its lanes read
+// 16-byte pieces of the row in order and sum them, then a fixed shuffle tree
+// adds the lanes.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void dense_gemv_kernel(int64_t n,
+                                                              const double* __restrict__ A,
+                                                              int64_t lda,
+                                                              const double* __restrict__ x,
+                                                              double* out, double p0, double p1) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)(kThreads / 64) + (threadIdx.x >> 6);
+  if (i >= n) return;  // uniform across the wave
+  const double* a = A + i * lda;
+  const bool vec = ((reinterpret_cast<uintptr_t>(a) & 15) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(x) & 15) == 0);
+  double s0 = 0.0, s1 = 0.0;
+  int64_t k = 2 * (int64_t)lane;
+  if (vec) {
+    for (; k + 1 < n; k += 128) {
+      const dvec2 av = *reinterpret_cast<const dvec2*>(a + k);
+      const dvec2 xv = *reinterpret_cast<const dvec2*>(x + k);
+      s0 = fma(av.x, xv.x, s0);
+      s1 = fma(av.y, xv.y, s1);
+    }
+    if (k < n) s0 = fma(a[k], x[k], s0);
+  } else {
+    for (; k < n; k += 128) {
+      s0 = fma(a[k], x[k], s0);
+      if (k + 1 < n) s1 = fma(a[k + 1], x[k + 1], s1);
+    }
+  }
+  double s = s0 + s1;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) {
+    if constexpr (MODE == kApplyBeta) out[i] = fma(p0, s, p1 * out[i]);
+    else out[i] = p0 * s;
+  }
+}
+
 // ------------------------------------------------------ column reductions
 // Threads are laid out as (row slot, column): CW columns per pass.
 constexpr int kColRows = 512;  // rows per partial block
@@ -1500,6 +1541,19 @@ void launch_dense_step(int64_t n, const T* A, int64_t lda, const T* X, const T* 
                        int ncols, T p0, T p1, T p2, int mode, const T* R, const T* kap,
                        hipStream_t st) {
   if (n <= 0 || ncols <= 0) return;
+  if constexpr (std::is_same<T, double>::value) {
+    if (ncols == 1 && ld == 1 && (mode == kApply || mode == kApplyBeta)) {
+      const dim3 g1((unsigned)((n + kThreads / 64 - 1) / (kThreads / 64)));
+      if (mode == kApply)
+        hipLaunchKernelGGL(dense_gemv_kernel<kApply>, g1, dim3(kThreads), 0, st, n, A, lda, X,
+                           out, p0, p1);
+      else
+        hipLaunchKernelGGL(dense_gemv_kernel<kApplyBeta>, g1, dim3(kThreads), 0, st, n, A, lda, X,
+                           out, p0, p1);
+      RCG_HIP(hipGetLastError());
+      return;
+    }
+  }
   const int ntn = (ncols + DenseCfg<T>::BN - 1) / DenseCfg<T>::BN;
   const int64_t tiles = ((n + DenseCfg<T>::BM - 1) / DenseCfg<T>::BM) * ntn;
   const dim3 grid((unsigned)round_up(tiles, kXcds));
