@@ -232,6 +232,21 @@ inline cplx conj_v(cplx x) { return std::conj(x); }
 inline double real_v(double x) { return x; }
 inline double real_v(cplx x) { return x.real(); }
 
+// sum_i conj(x_i) y_i with eight partial sums, which the compiler keeps in
+// vector registers (a single running sum would serialise the loop).
+template <class S>
+inline S dotc(const S* x, const S* y, long m) {
+  S p[8] = {S(0.0), S(0.0), S(0.0), S(0.0), S(0.0), S(0.0), S(0.0), S(0.0)};
+  long i = 0;
+  for (; i + 8 <= m; i += 8)
+    for (int u = 0; u < 8; ++u) p[u] += conj_v(x[i + u]) * y[i + u];
+  for (; i < m; ++i) p[i & 7] += conj_v(x[i]) * y[i];
+  return ((p[0] + p[4]) + (p[1] + p[5])) + ((p[2] + p[6]) + (p[3] + p[7]));
+}
+
+// Only the lower triangle of the trailing block is read and updated: the
+// block stays Hermitian, the diagonal, the subdiagonal and every Householder
+// column lie in the lower triangle.
 template <class S>
 inline void tridiagonalize(std::vector<S>& A, long n, std::vector<double>& d,
                            std::vector<double>& e, std::vector<S>& Q) {
@@ -259,25 +274,28 @@ inline void tridiagonalize(std::vector<S>& A, long n, std::vector<double>& d,
     v.assign(m, S(0.0));
     v[0] = x0 - alpha;
     for (long i = 1; i < m; ++i) v[i] = a(k + 1 + i, k);
-    double vv = 0.0;
-    for (long i = 0; i < m; ++i) vv += std::norm(v[i]);
+    const double vv = real_v(dotc(v.data(), v.data(), m));
     const double beta = 2.0 / vv;
     betas[k] = beta;
     sub[k] = alpha;
     // trailing block A22 = A[k+1.., k+1..]: p = beta A22 v, w = p - K v,
-    // K = beta (v^H p) / 2, A22 -= v w^H + w v^H
-    for (long i = 0; i < m; ++i) {
-      S t = S(0.0);
-      for (long j = 0; j < m; ++j) t += a(k + 1 + i, k + 1 + j) * v[j];
-      p[i] = beta * t;
+    // K = beta (v^H p) / 2, A22 -= v w^H + w v^H.  Column j of the lower
+    // triangle gives p[i > j] += A(i, j) v_j (contiguous axpy) and, through
+    // A(j, i) = conj(A(i, j)), p[j] += sum_{i > j} conj(A(i, j)) v_i.
+    std::fill(p.begin(), p.begin() + m, S(0.0));
+    for (long j = 0; j < m; ++j) {
+      const S* col = &a(k + 1, k + 1 + j);
+      const S vj = v[j];
+      for (long i = j + 1; i < m; ++i) p[i] += col[i] * vj;
+      p[j] += real_v(col[j]) * vj + dotc(col + j + 1, v.data() + j + 1, m - j - 1);
     }
-    S vp = S(0.0);
-    for (long i = 0; i < m; ++i) vp += conj_v(v[i]) * p[i];
-    const double K = 0.5 * beta * real_v(vp);
+    for (long i = 0; i < m; ++i) p[i] *= beta;
+    const double K = 0.5 * beta * real_v(dotc(v.data(), p.data(), m));
     for (long i = 0; i < m; ++i) w[i] = p[i] - K * v[i];
     for (long j = 0; j < m; ++j) {
       const S cvj = conj_v(v[j]), cwj = conj_v(w[j]);
-      for (long i = 0; i < m; ++i) a(k + 1 + i, k + 1 + j) -= v[i] * cwj + w[i] * cvj;
+      S* col = &a(k + 1, k + 1 + j);
+      for (long i = j; i < m; ++i) col[i] -= v[i] * cwj + w[i] * cvj;
     }
   }
   for (long k = 0; k < n; ++k) d[k] = real_v(a(k, k));
@@ -290,10 +308,9 @@ inline void tridiagonalize(std::vector<S>& A, long n, std::vector<double>& d,
     const std::vector<S>& v = vs[k];
     const long m = n - k - 1;
     for (long j = k + 1; j < n; ++j) {
-      S t = S(0.0);
-      for (long i = 0; i < m; ++i) t += conj_v(v[i]) * Q[(k + 1 + i) + j * n];
-      t *= betas[k];
-      for (long i = 0; i < m; ++i) Q[(k + 1 + i) + j * n] -= v[i] * t;
+      S* qj = &Q[(k + 1) + j * n];
+      const S t = betas[k] * dotc(v.data(), qj, m);
+      for (long i = 0; i < m; ++i) qj[i] -= v[i] * t;
     }
   }
   // real off-diagonals: T' = D^H T D with D = diag(phases), Q <- Q D
