@@ -517,6 +517,7 @@ def main():
                       "t_lanczos_s": round(last.t_lanczos, 4),
                       "t_filter_steps_device_s": round(last.t_filter_device, 4),
                       "t_ortho_s": round(last.t_ortho, 4), "t_rr_s": round(last.t_rr, 4),
+                      "t_rr_host_eig_s": round(last.t_eig, 4),
                       "lambda_min": float(ev[0]) if f else None,
                       "lambda_k": float(ev[-1]) if f else None},
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),

@@ -244,6 +244,8 @@ typedef struct rcg_solve_result {
                                 (FP64) or CSR(FP32 values) + 4 FP32 block passes */
   int64_t filter_panel;      /* autotuned column-panel width of the filter step */
   int64_t filter_launches;   /* step kernel launches inside t_filter_device */
+  double t_eig;              /* host seconds in the dense eigensolves of the projected
+                                operator (part of t_rr) */
 } rcg_solve_result;
 
 /* RayleighChebyshevLM::getMinEigenSystem (RayleighChebyshevLM.h:693-738;

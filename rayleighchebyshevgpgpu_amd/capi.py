@@ -38,7 +38,7 @@ class SolveResult(C.Structure):
         ("rng_counter", C.c_uint64), ("filter_degree_sum", C.c_int64), ("t_total", C.c_double),
         ("t_lanczos", C.c_double), ("t_filter", C.c_double), ("t_ortho", C.c_double),
         ("t_rr", C.c_double), ("t_filter_device", C.c_double), ("filter_bytes", C.c_double),
-        ("filter_panel", C.c_int64), ("filter_launches", C.c_int64),
+        ("filter_panel", C.c_int64), ("filter_launches", C.c_int64), ("t_eig", C.c_double),
     ]
 
 

@@ -367,10 +367,12 @@ inline void tridiag_ql(std::vector<double>& d, std::vector<double>& e, long n, s
           pp = s * r;
           d[i + 1] = g + pp;
           g = c * r - b;
-          S* zi = &Z[i * n];
-          S* zj = &Z[(i + 1) * n];
-          for (long k = 0; k < n; ++k) {
-            const S fz = zj[k];
+          // real rotation: a complex column is rotated as 2n doubles
+          double* zi = reinterpret_cast<double*>(&Z[i * n]);
+          double* zj = reinterpret_cast<double*>(&Z[(i + 1) * n]);
+          const long len = n * (long)(sizeof(S) / sizeof(double));
+          for (long k = 0; k < len; ++k) {
+            const double fz = zj[k];
             zj[k] = s * zi[k] + c * fz;
             zi[k] = c * zi[k] - s * fz;
           }

@@ -140,9 +140,12 @@ void launch_dense_step(int64_t n, const T* A, int64_t lda, const T* X, const T* 
                        hipStream_t st);
 
 // G (bx x by, column-major, ldg = bx) = X^H Y over n rows; ws holds partials.
+// X == Y (same pointer and ld) computes the upper 64x64 tiles and mirrors
+// them.  lower_only (bx == by): the caller reads only G's lower triangle,
+// whose tiles are the only ones multiplied (FP64).
 template <typename T>
 void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, int by, T* G,
-                 T* ws, size_t ws_elems, hipStream_t st);
+                 T* ws, size_t ws_elems, hipStream_t st, bool lower_only = false);
 template <typename T>
 size_t gram_ws_elems(int64_t n, int bx, int by);
 // Bound of gram_ws_elems over all widths <= bmax.

@@ -613,9 +613,73 @@ __device__ __forceinline__ d4v mfma16x16x4(double a, double b, d4v c) {
 // Partial X^T Y of one 64x64 output tile over a row range.  Wave w takes the
 // 4-row groups i0 + 4w + 16t; lane (kr = l/16, cc = l%16) loads
 // X[row+kr][a0+16p+cc] and Y[row+kr][b0+16q+cc] -- 128-byte row segments --
-// which are exactly the A and B fragments of the 16 MFMA subtiles.  SYM
-// (X == Y, one tile) multiplies only subtiles p <= q and mirrors the rest.
-template <bool SYM>
+// which are exactly the A and B fragments of the 16 MFMA subtiles.  DIAG (a
+// tile on the diagonal of X^T X) multiplies only subtiles p <= q and needs
+// no Y loads.
+template <bool DIAG>
+__device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[4][4], const double* __restrict__ X,
+                                                     int ldx, int bx, const double* __restrict__ Y,
+                                                     int ldy, int by, int a0, int b0, int64_t ib,
+                                                     int64_t i1, int kr, int cc) {
+  // four 16-row groups per pass: all of their loads are issued before the
+  // MFMAs that consume them, and the next pass's loads before this pass's
+  // MFMAs
+  constexpr int U = 4;
+  double xa[U][4], yb[U][4], xn[U][4], yn[U][4];
+  auto load = [&](int64_t jb, double (&xv)[U][4], double (&yv)[U][4]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t row = jb + 16 * u + kr;
+      const bool ok = row < i1;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int c = a0 + 16 * t + cc;
+        xv[u][t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
+      }
+      if constexpr (!DIAG) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int c = b0 + 16 * t + cc;
+          yv[u][t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
+        }
+      }
+    }
+  };
+  if (ib < i1) load(ib, xa, yb);
+  for (; ib < i1; ib += 16 * U) {
+    const bool more = ib + 16 * U < i1;
+    if (more) load(ib + 16 * U, xn, yn);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (DIAG && p > q) continue;
+          acc[p][q] = mfma16x16x4(xa[u][p], DIAG ? xa[u][q] : yb[u][q], acc[p][q]);
+        }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          xa[u][t] = xn[u][t];
+          if constexpr (!DIAG) yb[u][t] = yn[u][t];
+        }
+    }
+  }
+}
+
+// Partial Gram matrices, one bx x by block per row block.  kGramSym (X == Y):
+// only the tiles ta <= tb of the upper triangle run, ntiles = nta (nta+1)/2;
+// each writes its values at (a, b) and mirrored at (b, a).  kGramDiag is the
+// one-tile case (bx <= 64), compiled without the general path so that it
+// keeps two waves per SIMD.  kGramLower (bx == by, X != Y) runs the tiles on
+// and below the diagonal, for callers that read only the lower triangle of
+// X^H Y (the Hermitian projection); the upper tiles get the transposed lower
+// values so that every entry of the result is defined.
+enum { kGramFull = 0, kGramSym = 1, kGramDiag = 2, kGramLower = 3 };
+template <int MODE>
 __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ Y, int ldy,
     int by, double* __restrict__ P, int64_t rows_per_blk, int nta, int ntiles, int64_t nblk) {
@@ -626,8 +690,25 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   if (L >= nblk * ntiles) return;
   const int tile = (int)(L % ntiles);
   const int64_t rblk = L / ntiles;
-  const int a0 = (tile % nta) * kTile;
-  const int b0 = (tile / nta) * kTile;
+  constexpr bool SYM = MODE != kGramFull;
+  int ta, tb;
+  if constexpr (SYM) {
+    tb = 0;
+    while ((tb + 1) * (tb + 2) / 2 <= tile) ++tb;
+    ta = tile - tb * (tb + 1) / 2;
+    if constexpr (MODE == kGramLower) {
+      const int t = ta;
+      ta = tb;
+      tb = t;
+    }
+  } else {
+    ta = tile % nta;
+    tb = tile / nta;
+  }
+  const bool diag = MODE == kGramDiag || (MODE == kGramSym && ta == tb);
+  const bool mirror = SYM && ta != tb;
+  const int a0 = ta * kTile;
+  const int b0 = tb * kTile;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int kr = lane >> 4;
@@ -639,54 +720,12 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
-  // four 16-row groups per pass: all of their loads are issued before the
-  // MFMAs that consume them, and the next pass's loads before this pass's
-  // MFMAs
-  constexpr int U = 4;
-  double xa[U][4], yb[U][4], xn[U][4], yn[U][4];
-  auto load = [&](int64_t ib, double (&xv)[U][4], double (&yv)[U][4]) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t row = ib + 16 * u + kr;
-      const bool ok = row < i1;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int c = a0 + 16 * t + cc;
-        xv[u][t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
-      }
-      if constexpr (!SYM) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int c = b0 + 16 * t + cc;
-          yv[u][t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
-        }
-      }
-    }
-  };
-  int64_t ib = i0 + 4 * wave;
-  if (ib < i1) load(ib, xa, yb);
-  for (; ib < i1; ib += 16 * U) {
-    const bool more = ib + 16 * U < i1;
-    if (more) load(ib + 16 * U, xn, yn);
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (SYM && p > q) continue;
-          acc[p][q] = mfma16x16x4(xa[u][p], SYM ? xa[u][q] : yb[u][q], acc[p][q]);
-        }
-    if (more) {
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          xa[u][t] = xn[u][t];
-          if constexpr (!SYM) yb[u][t] = yn[u][t];
-        }
-    }
-  }
+  if constexpr (MODE == kGramDiag)
+    gram_tile_accumulate<true>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
+  else if (MODE == kGramSym && diag)
+    gram_tile_accumulate<true>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
+  else
+    gram_tile_accumulate<false>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
   // the four waves' tiles are summed in LDS in wave order
   for (int w = 0; w < 4; ++w) {
     if (wave == w) {
@@ -694,7 +733,7 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
       for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          if (SYM && p > q) continue;
+          if (diag && p > q) continue;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const int idx = (16 * p + 4 * j + kr) + kTile * (16 * q + cc);
@@ -711,10 +750,13 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     const int a = a0 + al;
     const int b = b0 + bl;
     if (a >= bx || b >= by) continue;
-    double v;
-    if constexpr (SYM) v = (al / 16 <= bl / 16) ? red[al + kTile * bl] : red[bl + kTile * al];
-    else v = red[al + kTile * bl];
-    Pb[a + (int64_t)b * bx] = v;
+    if (diag) {
+      Pb[a + (int64_t)b * bx] = (al / 16 <= bl / 16) ? red[al + kTile * bl] : red[bl + kTile * al];
+    } else {
+      const double v = red[al + kTile * bl];
+      Pb[a + (int64_t)b * bx] = v;
+      if (mirror) Pb[b + (int64_t)a * bx] = v;
+    }
   }
 }
 
@@ -1637,7 +1679,7 @@ size_t gram_ws_bound(int64_t n, int bmax) {
 
 template <typename T>
 void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, int by, T* G,
-                 T* ws, size_t ws_elems, hipStream_t st) {
+                 T* ws, size_t ws_elems, hipStream_t st, bool lower_only) {
   if (bx <= 0 || by <= 0) return;
   if (n <= 0) {
     RCG_HIP(hipMemsetAsync(G, 0, sizeof(T) * bx * by, st));
@@ -1649,11 +1691,20 @@ void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, in
   const int nta = (bx + kTile - 1) / kTile;
   const int ntiles = nta * ((by + kTile - 1) / kTile);
   if constexpr (std::is_same<T, double>::value) {
-    if (X == Y && bx == by && ldx == ldy && bx <= kTile)
-      hipLaunchKernelGGL(gram_mfma_kernel<true>, dim3((unsigned)round_up(nblk, kXcds)),
+    if (X == Y && bx == by && ldx == ldy && nta == 1) {
+      hipLaunchKernelGGL(gram_mfma_kernel<kGramDiag>, dim3((unsigned)round_up(nblk, kXcds)),
                          dim3(kThreads), 0, st, n, X, ldx, bx, Y, ldy, by, ws, rpb, nta, 1, nblk);
-    else
-      hipLaunchKernelGGL(gram_mfma_kernel<false>,
+    } else if (X == Y && bx == by && ldx == ldy) {
+      const int nsym = nta * (nta + 1) / 2;
+      hipLaunchKernelGGL(gram_mfma_kernel<kGramSym>, dim3((unsigned)round_up(nblk * nsym, kXcds)),
+                         dim3(kThreads), 0, st, n, X, ldx, bx, Y, ldy, by, ws, rpb, nta, nsym, nblk);
+    } else if (lower_only && bx == by && nta > 1) {
+      const int nsym = nta * (nta + 1) / 2;
+      hipLaunchKernelGGL(gram_mfma_kernel<kGramLower>,
+                         dim3((unsigned)round_up(nblk * nsym, kXcds)), dim3(kThreads), 0, st, n, X,
+                         ldx, bx, Y, ldy, by, ws, rpb, nta, nsym, nblk);
+    } else
+      hipLaunchKernelGGL(gram_mfma_kernel<kGramFull>,
                          dim3((unsigned)round_up(nblk * ntiles, kXcds)), dim3(kThreads), 0, st, n,
                          X, ldx, bx, Y, ldy, by, ws, rpb, nta, ntiles, nblk);
   } else {
@@ -1886,7 +1937,7 @@ void launch_lz_step(int64_t n, const int* rowptr, const int* col, const double* 
                                     const CsrSched*, bool);                                       \
   template size_t gram_ws_elems<T>(int64_t, int, int);                                           \
   template void launch_gram<T>(int64_t, const T*, int, int, const T*, int, int, T*, T*, size_t,   \
-                               hipStream_t);                                                      \
+                               hipStream_t, bool);                                                \
   template void launch_tall_gemm<T>(int64_t, const T*, int, int, const T*, int, int, T, T,        \
                                     const T*, T*, int, hipStream_t);                              \
   template void launch_col_sumsq<T>(int64_t, const T*, int, int, double*, double*, size_t,        \

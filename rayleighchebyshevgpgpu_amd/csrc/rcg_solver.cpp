@@ -1126,13 +1126,14 @@ class Engine {
 
   // host[bx x by] = X^H Y (column-major); widths in columns of S.  Complex:
   // the real Gram of the interleaved blocks (2bx x 2by) is combined on the host.
+  // lower_only: only the entries a >= c of the result are meaningful.
   template <class S>
   void gram_to_host(const double* X, int64_t ldx, int64_t bx, const double* Yp, int64_t ldy,
-                    int64_t by, S* host) {
+                    int64_t by, S* host, bool lower_only = false) {
     constexpr int w = std::is_same<S, cplx>::value ? 2 : 1;
     const int64_t rx = w * bx, ry = w * by;
     rcg::launch_gram<double>(n_, X, (int)ldx, (int)rx, Yp, (int)ldy, (int)ry, dmatA(), ws(),
-                             ws_elems_, st_);
+                             ws_elems_, st_, lower_only);
     RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * rx * ry, hipMemcpyDeviceToHost, st_));
     sync();
     const double* g = hmatA();
@@ -1248,7 +1249,7 @@ class Engine {
   void project(int64_t b, std::vector<S>& H) {
     apply(V(), W_, b);
     H.assign(b * b, S(0.0));
-    gram_to_host<S>(V(), ld_, b, W_, ld_, b, H.data());
+    gram_to_host<S>(V(), ld_, b, W_, ld_, b, H.data(), true);
     for (int64_t i = 0; i < b; ++i) {
       H[i + i * b] = std::real(H[i + i * b]);
       for (int64_t j = i + 1; j < b; ++j) H[i + j * b] = conj_s(H[j + i * b]);
@@ -1735,7 +1736,9 @@ class Solver {
         ta = wall();
         eng_.project(subspaceSize, H);
         count_apply_ += subspaceSize;
+        const double te = wall();
         rcg::dense::small_eigensystem(H.data(), subspaceSize, theta, Yh);
+        t_eig_ += wall() - te;
         const long residualCheckCount = (foundSize + subspaceSize < N) ? inc : subspaceSize;
         eng_.rotate_residuals(subspaceSize, Yh, theta, residualCheckCount, subspaceResiduals);
         fTheta.assign(theta.begin(), theta.end());
@@ -1897,6 +1900,7 @@ class Solver {
     R->t_filter = t_filter_;
     R->t_ortho = t_ortho_;
     R->t_rr = t_rr_;
+    R->t_eig = t_eig_;
     R->filter_degree_sum = degree_sum_;
     R->t_filter_device = eng_.step_seconds();
     R->filter_bytes = eng_.step_bytes();
@@ -1922,7 +1926,9 @@ class Solver {
     std::vector<S> H, Z;
     std::vector<double> th, r;
     e2.project(f, H);
+    const double te = wall();
     rcg::dense::small_eigensystem(H.data(), f, th, Z);
+    t_eig_ += wall() - te;
     e2.rotate_residuals(f, Z, th, f, r);
     if (!e2.v_is_first()) e2.copy_cols(e2.V(), ldy, 0, eng_.Y(), ldy, 0, f);
     e2.scale_found(eng_.Y(), ldy, 0, f);
@@ -1951,7 +1957,7 @@ class Solver {
   DevBuf Vb_, Tb_, Wb_, Yb_, Mb_;
   long found_ = 0;
   long count_ortho_ = 0, count_apply_ = 0, count_eig_ = 0, degree_sum_ = 0;
-  double guard_ = 0.0, t_filter_ = 0.0, t_ortho_ = 0.0, t_rr_ = 0.0;
+  double guard_ = 0.0, t_filter_ = 0.0, t_ortho_ = 0.0, t_rr_ = 0.0, t_eig_ = 0.0;
 };
 
 // allow_cx: a complex operator with complex (C128) blocks is accepted too.

@@ -65,7 +65,8 @@ def complex_solve(rcg, A, k, inc, buf, tol, mixed=False, seed=0, stop="RESIDUAL_
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-@pytest.mark.parametrize("L,phi,W,k,inc,buf", [(32, 1 / 8, 1.0, 16, 8, 8), (48, 1 / 16, 2.0, 24, 12, 8)])
+@pytest.mark.parametrize("L,phi,W,k,inc,buf", [(32, 1 / 8, 1.0, 16, 8, 8), (48, 1 / 16, 2.0, 24, 12, 8),
+                                                (48, 1 / 16, 2.0, 40, 24, 16)])
 def test_complex_solve_vs_oracle(rcg, orc, L, phi, W, k, inc, buf, mixed):
     print(f"case: complex solve Hofstadter {L}x{L} phi={phi} W={W} k={k} mixed={mixed}", flush=True)
     tol = 1e-10

@@ -87,7 +87,7 @@ def test_cheb_filter_repetition(rcg, orc):
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("n,cols", [(5000, 40), (20000, 64), (3000, 100)])
+@pytest.mark.parametrize("n,cols", [(5000, 40), (20000, 64), (3000, 100), (6000, 200)])
 def test_orthonormalize(rcg, n, cols):
     print(f"case: CholQR2 {n}x{cols}", flush=True)
     X = np.random.default_rng(n).standard_normal((n, cols))
@@ -160,28 +160,31 @@ def test_residuals_on_untuned_wide_block(rcg, cplx):
     assert np.max(np.abs(res - rref) / rref) < 1e-12
 
 
-def test_project_rotate_residuals(rcg):
-    print("case: projection, rotation and residuals on 12^3 Anderson, b=20", flush=True)
+@pytest.mark.parametrize("b", [20, 150])
+def test_project_rotate_residuals(rcg, b):
+    """b = 150 spans three 64-column tiles: the projection multiplies only the
+    tiles on and below the diagonal and mirrors the lower triangle."""
+    print(f"case: projection, rotation and residuals on 12^3 Anderson, b={b}", flush=True)
     A = anderson3d(12)
     S = A.to_scipy()
     op = rcg.CsrOperator(A)
     rng = np.random.default_rng(4)
-    V, _ = np.linalg.qr(rng.standard_normal((A.n, 20)))
-    bv, bw, bt = rcg.Block.from_host(V), rcg.Block(A.n, 20), rcg.Block(A.n, 20)
-    H = np.zeros((20, 20), order="F")
-    capi.check(capi.lib().rcg_project(op.h, bv.h, bw.h, 20, H.ctypes.data_as(C.c_void_p)))
+    V, _ = np.linalg.qr(rng.standard_normal((A.n, b)))
+    bv, bw, bt = rcg.Block.from_host(V), rcg.Block(A.n, b), rcg.Block(A.n, b)
+    H = np.zeros((b, b), order="F")
+    capi.check(capi.lib().rcg_project(op.h, bv.h, bw.h, b, H.ctypes.data_as(C.c_void_p)))
     Href = V.T @ (S @ V)
     assert np.max(np.abs(H - Href)) < 1e-13 * np.max(np.abs(Href))
     assert np.array_equal(H, H.T)
     w, Z = np.linalg.eigh(H)
     Z = np.asfortranarray(Z)
-    capi.check(capi.lib().rcg_rotate(bv.h, bt.h, 20, Z.ctypes.data_as(C.c_void_p)))
+    capi.check(capi.lib().rcg_rotate(bv.h, bt.h, b, Z.ctypes.data_as(C.c_void_p)))
     Vr = bv.download()
     ref = V @ Z
     ref /= np.linalg.norm(ref, axis=0)
     assert np.max(np.abs(Vr - ref)) < 1e-13
-    res = np.zeros(20)
-    capi.check(capi.lib().rcg_residuals(op.h, bv.h, bw.h, 20, w.ctypes.data_as(C.c_void_p),
+    res = np.zeros(b)
+    capi.check(capi.lib().rcg_residuals(op.h, bv.h, bw.h, b, w.ctypes.data_as(C.c_void_p),
                                         res.ctypes.data_as(C.c_void_p)))
     rref = np.linalg.norm(w * Vr - S @ Vr, axis=0)
     assert np.max(np.abs(res - rref)) < 1e-12
