@@ -846,6 +846,104 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
   }
 }
 
+// out = alpha X C + beta Z for 64 < bx <= 64 KCH (the Gram-sized products of
+// blocks of up to 256 columns).  The workgroup's whole 64-column slice of C
+// (64 KCH x 64) is staged in LDS once; the workgroup then walks 128-row tiles
+// with no further barrier.  Each wave owns 32 rows (two 16-row MFMA groups),
+// so every B fragment read from LDS feeds two MFMAs, and the next 64-wide k
+// chunk of its X rows is loaded while the current one is multiplied.
+template <int KCH>
+__global__ __launch_bounds__(kThreads) void tall_gemm_wide_kernel(
+    int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ C,
+    int ldc, int by, double alpha, double beta, const double* Z, double* out, int ldo,
+    int64_t ntiles, int ct, int64_t nstripe) {
+  constexpr int KP = KCH * kTile;
+  __shared__ double cs[KP][kTile + 1];  // [k][c]
+  const int64_t L = xcd_remap(blockIdx.x, gridDim.x);
+  if (L >= nstripe * ct) return;
+  const int c0 = (int)(L % ct) * kTile;
+  const int64_t stripe = L / ct;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kk = lane >> 4;
+  const int r = lane & 15;
+  const bool vec_ok = ((reinterpret_cast<uintptr_t>(X) & 15) == 0) && (ldx % 2 == 0);
+  for (int e = threadIdx.x; e < KP * kTile; e += kThreads) {
+    const int k = e % KP;
+    const int c = e / KP;
+    cs[k][c] = (k < bx && c0 + c < by) ? C[k + (int64_t)(c0 + c) * ldc] : 0.0;
+  }
+  __syncthreads();
+  // lane (kk, r) of group g holds X[row_g][k0 + 16kk + q], q < 16
+  auto load = [&](int64_t r0, int k0, double (&xv)[2][16]) {
+    const int kb = k0 + 16 * kk;
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int64_t row = r0 + 16 * g + r;
+      if (row < n && kb + 16 <= bx && vec_ok) {
+        const double* xp = X + row * ldx + kb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const dvec2 t = *reinterpret_cast<const dvec2*>(xp + 2 * q);
+          xv[g][2 * q] = t.x;
+          xv[g][2 * q + 1] = t.y;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          xv[g][q] = (row < n && kb + q < bx) ? X[row * ldx + kb + q] : 0.0;
+      }
+    }
+  };
+  for (int64_t tile = stripe; tile < ntiles; tile += nstripe) {
+    const int64_t r0 = tile * (2 * kTile) + wave * 32;
+    d4v acc[2][4];
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[g][t] = d4v{0.0, 0.0, 0.0, 0.0};
+    double xa[2][16], xn[2][16];
+    load(r0, 0, xa);
+#pragma unroll 1
+    for (int ci = 0; ci < KCH; ++ci) {
+      if (ci + 1 < KCH) load(r0, (ci + 1) * kTile, xn);
+#pragma unroll
+      for (int sidx = 0; sidx < 16; ++sidx) {
+        const int k = ci * kTile + 16 * kk + sidx;
+        double b[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) b[t] = cs[k][16 * t + r];
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) acc[g][t] = mfma16x16x4(xa[g][sidx], b[t], acc[g][t]);
+      }
+      if (ci + 1 < KCH) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int q = 0; q < 16; ++q) xa[g][q] = xn[g][q];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t ro = r0 + 16 * g + 4 * j + kk;
+        if (ro >= n) continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int c = c0 + 16 * t + r;
+          if (c >= by) continue;
+          const int64_t off = ro * ldo + c;
+          double v = alpha * acc[g][t][j];
+          if (beta != 0.0) v = fma(beta, Z[off], v);
+          out[off] = v;
+        }
+      }
+  }
+}
+
 // ------------------------------------------------ dense operator (MFMA)
 // v_mfma_f32_16x16x4f32: lane l supplies A[l%16][l/16] and B[l/16][l%16] as
 // for FP64; register j holds D[4(l/16) + j][l%16].
@@ -1726,6 +1824,25 @@ void launch_tall_gemm(int64_t n, const T* X, int ldx, int bx, const T* C, int ld
   const int64_t grid = round_up(nrb, kXcds);
   const unsigned ct = (unsigned)((by + kTile - 1) / kTile);
   if constexpr (std::is_same<T, double>::value) {
+    if (bx > kTile && bx <= 4 * kTile) {
+      // C slice resident in LDS (66.5, 100 or 133 KB) and about 300 registers
+      // per lane: one workgroup per CU, persistent over 128-row tiles
+      const int kch = (bx + kTile - 1) / kTile;
+      const int64_t nrt = (n + 2 * kTile - 1) / (2 * kTile);
+      const int64_t nstripe = std::max<int64_t>(1, std::min<int64_t>(nrt, 256 / (int64_t)ct));
+      const dim3 grid((unsigned)round_up(nstripe * ct, kXcds));
+      if (kch == 2)
+        hipLaunchKernelGGL(tall_gemm_wide_kernel<2>, grid, dim3(kThreads), 0, st, n, X, ldx, bx, C,
+                           ldc, by, alpha, beta, Z, out, ldo, nrt, (int)ct, nstripe);
+      else if (kch == 3)
+        hipLaunchKernelGGL(tall_gemm_wide_kernel<3>, grid, dim3(kThreads), 0, st, n, X, ldx, bx, C,
+                           ldc, by, alpha, beta, Z, out, ldo, nrt, (int)ct, nstripe);
+      else
+        hipLaunchKernelGGL(tall_gemm_wide_kernel<4>, grid, dim3(kThreads), 0, st, n, X, ldx, bx, C,
+                           ldc, by, alpha, beta, Z, out, ldo, nrt, (int)ct, nstripe);
+      RCG_HIP(hipGetLastError());
+      return;
+    }
     // persistent over row tiles: about 4 workgroups per CU of a 256-CU part
     const int64_t nstripe = std::max<int64_t>(1, std::min<int64_t>(nrb, 1024 / (int64_t)ct));
     hipLaunchKernelGGL(tall_gemm_mfma_kernel, dim3((unsigned)round_up(nstripe * ct, kXcds)),

@@ -117,14 +117,15 @@ def test_orthonormalize_ill_conditioned(rcg):
     assert np.max(np.abs(Q.T @ Q - np.eye(b))) < 1e-12
 
 
-def test_deflate(rcg):
-    print("case: deflate 16 cols against 8 found", flush=True)
+@pytest.mark.parametrize("cols,found", [(16, 8), (40, 100)])
+def test_deflate(rcg, cols, found):
+    print(f"case: deflate {cols} cols against {found} found", flush=True)
     rng = np.random.default_rng(2)
     n = 6000
-    Y, _ = np.linalg.qr(rng.standard_normal((n, 8)))
-    X = rng.standard_normal((n, 16))
+    Y, _ = np.linalg.qr(rng.standard_normal((n, found)))
+    X = rng.standard_normal((n, cols))
     bx, by = rcg.Block.from_host(X), rcg.Block.from_host(Y)
-    capi.check(capi.lib().rcg_deflate(bx.h, 16, by.h, 8))
+    capi.check(capi.lib().rcg_deflate(bx.h, cols, by.h, found))
     Z = bx.download()
     assert np.max(np.abs(Y.T @ Z)) < 1e-14
     assert np.allclose(np.linalg.norm(Z, axis=0), 1.0, atol=1e-14)
