@@ -670,19 +670,24 @@ __device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[4][4], const dou
   }
 }
 
-// Partial Gram matrices, one bx x by block per row block.  kGramSym (X == Y):
-// only the tiles ta <= tb of the upper triangle run, ntiles = nta (nta+1)/2;
-// each writes its values at (a, b) and mirrored at (b, a).  kGramDiag is the
-// one-tile case (bx <= 64), compiled without the general path so that it
-// keeps two waves per SIMD.  kGramLower (bx == by, X != Y) runs the tiles on
-// and below the diagonal, for callers that read only the lower triangle of
-// X^H Y (the Hermitian projection); the upper tiles get the transposed lower
-// values so that every entry of the result is defined.
+// Which 64x64 tiles of X^H Y a Gram launch multiplies.  kGramFull: all of
+// them.  kGramSym (X == Y): only the tiles ta <= tb of the upper triangle,
+// ntiles = nta (nta+1)/2, each written at (a, b) and mirrored at (b, a).
+// kGramLower (bx == by, X != Y): the tiles on and below the diagonal, for
+// callers that read only the lower triangle of X^H Y (the Hermitian
+// projection); the upper tiles get the transposed lower values so that every
+// entry of the result is defined.  kGramDiag: the one tile of X^T X when
+// bx <= 64.
 enum { kGramFull = 0, kGramSym = 1, kGramDiag = 2, kGramLower = 3 };
+
+// Partial Gram matrices of one 64x64 tile, one bx x by block per row block:
+// the one-tile launches (kGramFull with bx, by <= 64, or kGramDiag, compiled
+// without the general path so that it keeps two waves per SIMD).
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ Y, int ldy,
     int by, double* __restrict__ P, int64_t rows_per_blk, int nta, int ntiles, int64_t nblk) {
+  static_assert(MODE == kGramFull || MODE == kGramDiag, "one-tile launches only");
   __shared__ double red[kTile * kTile];
   // one 1-D grid: the tiles of a row block are consecutive logical workgroups,
   // which the remap keeps on one XCD, so its rows are read from HBM once
@@ -690,23 +695,9 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   if (L >= nblk * ntiles) return;
   const int tile = (int)(L % ntiles);
   const int64_t rblk = L / ntiles;
-  constexpr bool SYM = MODE != kGramFull;
-  int ta, tb;
-  if constexpr (SYM) {
-    tb = 0;
-    while ((tb + 1) * (tb + 2) / 2 <= tile) ++tb;
-    ta = tile - tb * (tb + 1) / 2;
-    if constexpr (MODE == kGramLower) {
-      const int t = ta;
-      ta = tb;
-      tb = t;
-    }
-  } else {
-    ta = tile % nta;
-    tb = tile / nta;
-  }
-  const bool diag = MODE == kGramDiag || (MODE == kGramSym && ta == tb);
-  const bool mirror = SYM && ta != tb;
+  const int ta = tile % nta;
+  const int tb = tile / nta;
+  constexpr bool diag = MODE == kGramDiag;
   const int a0 = ta * kTile;
   const int b0 = tb * kTile;
   const int lane = threadIdx.x & 63;
@@ -720,12 +711,7 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
-  if constexpr (MODE == kGramDiag)
-    gram_tile_accumulate<true>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
-  else if (MODE == kGramSym && diag)
-    gram_tile_accumulate<true>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
-  else
-    gram_tile_accumulate<false>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
+  gram_tile_accumulate<diag>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
   // the four waves' tiles are summed in LDS in wave order
   for (int w = 0; w < 4; ++w) {
     if (wave == w) {
@@ -750,14 +736,143 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     const int a = a0 + al;
     const int b = b0 + bl;
     if (a >= bx || b >= by) continue;
-    if (diag) {
+    if constexpr (diag)
       Pb[a + (int64_t)b * bx] = (al / 16 <= bl / 16) ? red[al + kTile * bl] : red[bl + kTile * al];
-    } else {
-      const double v = red[al + kTile * bl];
-      Pb[a + (int64_t)b * bx] = v;
-      if (mirror) Pb[b + (int64_t)a * bx] = v;
-    }
+    else
+      Pb[a + (int64_t)b * bx] = red[al + kTile * bl];
   }
+}
+
+// Gram partials for blocks wider than one tile (bx or by > 64), through LDS.
+// Each workgroup owns one 64x64 output tile (enumerated as in gram_mfma_kernel)
+// and one row block; 32-row chunks of X[:, a0:a0+64] and Y[:, b0:b0+64] are
+// loaded with 16-byte loads one chunk ahead and staged in double-buffered LDS
+// (one barrier per chunk).  Wave w owns the 32x32 quadrant (w & 1, w >> 1) of
+// the tile, so no cross-wave reduction is needed; in a diagonal tile of X^T X
+// the quadrant below the diagonal is the mirror of the one above and its wave
+// idles.  Row blocks are sized so that all workgroups are resident at once.
+constexpr int kGramKr = 32;
+template <int MODE>
+__global__ __launch_bounds__(kThreads, 2) void gram_lds_kernel(
+    int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ Y, int ldy,
+    int by, double* __restrict__ P, int64_t rows_per_blk, int nta, int ntiles, int64_t nblk) {
+  __shared__ double xs[2][kGramKr][kTile + 1];
+  __shared__ double ys[2][kGramKr][kTile + 1];
+  const int64_t L = xcd_remap(blockIdx.x, gridDim.x);
+  if (L >= nblk * ntiles) return;
+  const int tile = (int)(L % ntiles);
+  const int64_t rblk = L / ntiles;
+  int ta, tb;
+  if constexpr (MODE != kGramFull) {
+    tb = 0;
+    while ((tb + 1) * (tb + 2) / 2 <= tile) ++tb;
+    ta = tile - tb * (tb + 1) / 2;
+    if constexpr (MODE == kGramLower) {
+      const int t = ta;
+      ta = tb;
+      tb = t;
+    }
+  } else {
+    ta = tile % nta;
+    tb = tile / nta;
+  }
+  const bool diag = MODE == kGramSym && ta == tb;
+  const int a0 = ta * kTile;
+  const int b0 = tb * kTile;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kr = lane >> 4;
+  const int cc = lane & 15;
+  const int qa = wave & 1;
+  const int qb = wave >> 1;
+  const bool idle = diag && qa == 1 && qb == 0;
+  const int64_t i0 = rblk * rows_per_blk;
+  const int64_t i1 = min(n, i0 + rows_per_blk);
+  const bool vx = ((reinterpret_cast<uintptr_t>(X) & 15) == 0) && (ldx % 2 == 0);
+  const bool vy = ((reinterpret_cast<uintptr_t>(Y) & 15) == 0) && (ldy % 2 == 0);
+  // thread t moves pairs e = t + 256 m, m < 4: chunk row e / 32, columns
+  // 2 (e % 32) and 2 (e % 32) + 1 of the tile's 64
+  auto ld2 = [&](const double* B, int ldb, int bw, bool vec, int64_t row, int col) -> dvec2 {
+    if (row >= i1) return dvec2{0.0, 0.0};
+    const double* ptr = B + row * ldb + col;
+    if (vec && col + 1 < bw) return *reinterpret_cast<const dvec2*>(ptr);
+    return dvec2{col < bw ? ptr[0] : 0.0, col + 1 < bw ? ptr[1] : 0.0};
+  };
+  dvec2 px[4], py[4];
+  auto fetch = [&](int64_t r0) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int e = threadIdx.x + kThreads * m;
+      const int64_t row = r0 + e / 32;
+      const int c = 2 * (e % 32);
+      px[m] = ld2(X, ldx, bx, vx, row, a0 + c);
+      if (!diag) py[m] = ld2(Y, ldy, by, vy, row, b0 + c);
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int e = threadIdx.x + kThreads * m;
+      const int rr = e / 32;
+      const int c = 2 * (e % 32);
+      xs[buf][rr][c] = px[m].x;
+      xs[buf][rr][c + 1] = px[m].y;
+      if (!diag) {
+        ys[buf][rr][c] = py[m].x;
+        ys[buf][rr][c + 1] = py[m].y;
+      }
+    }
+  };
+  d4v acc[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
+  const bool dquad = diag && qa == qb;  // quadrant on the diagonal: subtiles p <= q
+  int buf = 0;
+  if (i0 < i1) fetch(i0);
+  for (int64_t r0 = i0; r0 < i1; r0 += kGramKr) {
+    stash(buf);
+    __syncthreads();
+    if (r0 + kGramKr < i1) fetch(r0 + kGramKr);
+    if (!idle) {
+      const double(*yb)[kTile + 1] = diag ? xs[buf] : ys[buf];
+#pragma unroll
+      for (int k4 = 0; k4 < kGramKr; k4 += 4) {
+        const int kk = k4 + kr;
+        double af[2], bf[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) af[p] = xs[buf][kk][32 * qa + 16 * p + cc];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) bf[q] = yb[kk][32 * qb + 16 * q + cc];
+        acc[0][0] = mfma16x16x4(af[0], bf[0], acc[0][0]);
+        acc[0][1] = mfma16x16x4(af[0], bf[1], acc[0][1]);
+        if (!dquad) acc[1][0] = mfma16x16x4(af[1], bf[0], acc[1][0]);
+        acc[1][1] = mfma16x16x4(af[1], bf[1], acc[1][1]);
+      }
+    }
+    buf ^= 1;
+  }
+  if (idle) return;
+  double* Pb = P + rblk * (int64_t)bx * by;
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (dquad && p > q) continue;
+      const int as = a0 + 32 * qa + 16 * p;
+      const int bs = b0 + 32 * qb + 16 * q;
+      const bool mirror = (MODE == kGramSym && as != bs) || (MODE == kGramLower && ta != tb);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int a = as + 4 * j + kr;
+        const int b = bs + cc;
+        if (a >= bx || b >= by) continue;
+        const double v = acc[p][q][j];
+        Pb[a + (int64_t)b * bx] = v;
+        if (mirror) Pb[b + (int64_t)a * bx] = v;
+      }
+    }
 }
 
 // out = alpha X C + beta Z (X: n x bx, C: bx x by column-major).  A wave owns
@@ -768,6 +883,7 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
 // j, so 16 lanes store 16 consecutive columns of one row.  Workgroups loop
 // over row tiles; C is staged once when bx <= 64.  blockIdx.y selects 64
 // output columns.
+template <bool BETA>
 __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ C,
     int ldc, int by, double alpha, double beta, const double* Z, double* out, int ldo,
@@ -829,6 +945,20 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
         for (int t = 0; t < 4; ++t) acc[t] = mfma16x16x4(xa[sidx], cs[k][16 * t + r], acc[t]);
       }
     }
+    // Z may be `out` itself: all of the lane's Z values are read before its
+    // first store, so the loads are issued together
+    double zv[4][4];
+    if constexpr (BETA) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t ro = tile * kTile + wave * 16 + 4 * j + kk;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int c = c0 + 16 * t + r;
+          zv[j][t] = (ro < n && c < by) ? Z[ro * ldo + c] : 0.0;
+        }
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int64_t ro = tile * kTile + wave * 16 + 4 * j + kk;
@@ -837,10 +967,9 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
       for (int t = 0; t < 4; ++t) {
         const int c = c0 + 16 * t + r;
         if (c >= by) continue;
-        const int64_t off = ro * ldo + c;
         double v = alpha * acc[t][j];
-        if (beta != 0.0) v = fma(beta, Z[off], v);
-        out[off] = v;
+        if constexpr (BETA) v = fma(beta, zv[j][t], v);
+        out[ro * ldo + c] = v;
       }
     }
   }
@@ -903,10 +1032,27 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_wide_kernel(
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[g][t] = d4v{0.0, 0.0, 0.0, 0.0};
     double xa[2][16], xn[2][16];
+    // beta Z of the epilogue, read at the start of the last chunk: Z may be
+    // `out` itself, so loads issued between the stores would each wait for
+    // the one before
+    double zv[2][4][4];
     load(r0, 0, xa);
 #pragma unroll 1
     for (int ci = 0; ci < KCH; ++ci) {
       if (ci + 1 < KCH) load(r0, (ci + 1) * kTile, xn);
+      if (ci + 1 == KCH && beta != 0.0) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int64_t ro = r0 + 16 * g + 4 * j + kk;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int c = c0 + 16 * t + r;
+              zv[g][j][t] = (ro < n && c < by) ? Z[ro * ldo + c] : 0.0;
+            }
+          }
+      }
 #pragma unroll
       for (int sidx = 0; sidx < 16; ++sidx) {
         const int k = ci * kTile + 16 * kk + sidx;
@@ -935,10 +1081,9 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_wide_kernel(
         for (int t = 0; t < 4; ++t) {
           const int c = c0 + 16 * t + r;
           if (c >= by) continue;
-          const int64_t off = ro * ldo + c;
           double v = alpha * acc[g][t][j];
-          if (beta != 0.0) v = fma(beta, Z[off], v);
-          out[off] = v;
+          if (beta != 0.0) v = fma(beta, zv[g][j][t], v);
+          out[ro * ldo + c] = v;
         }
       }
   }
@@ -1784,23 +1929,36 @@ void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, in
     return;
   }
   int64_t rpb;
-  const int64_t nblk = gram_blocks<T>(n, bx, by, &rpb);
+  int64_t nblk = gram_blocks<T>(n, bx, by, &rpb);
   if ((size_t)nblk * bx * by > ws_elems) throw Error(3, "gram workspace too small");
   const int nta = (bx + kTile - 1) / kTile;
-  const int ntiles = nta * ((by + kTile - 1) / kTile);
+  const int ntb = (by + kTile - 1) / kTile;
+  const int ntiles = nta * ntb;
   if constexpr (std::is_same<T, double>::value) {
-    if (X == Y && bx == by && ldx == ldy && nta == 1) {
+    const bool sym = X == Y && bx == by && ldx == ldy;
+    if (ntiles > 1) {
+      // LDS-staged tiles; the launched tiles times the row blocks fill two
+      // workgroups per CU of a 256-CU part once
+      const bool low = !sym && lower_only && bx == by;
+      const int nl = (sym || low) ? nta * (nta + 1) / 2 : ntiles;
+      const int64_t target = std::max<int64_t>(1, 512 / nl);
+      rpb = round_up((n + target - 1) / target, kRb);
+      rpb = std::max<int64_t>(rpb, round_up(4 * (int64_t)std::max(bx, by), kRb));
+      nblk = (n + rpb - 1) / rpb;
+      if ((size_t)nblk * bx * by > ws_elems) throw Error(3, "gram workspace too small");
+      const dim3 grid((unsigned)round_up(nblk * nl, kXcds));
+      if (sym)
+        hipLaunchKernelGGL(gram_lds_kernel<kGramSym>, grid, dim3(kThreads), 0, st, n, X, ldx, bx,
+                           Y, ldy, by, ws, rpb, nta, nl, nblk);
+      else if (low)
+        hipLaunchKernelGGL(gram_lds_kernel<kGramLower>, grid, dim3(kThreads), 0, st, n, X, ldx, bx,
+                           Y, ldy, by, ws, rpb, nta, nl, nblk);
+      else
+        hipLaunchKernelGGL(gram_lds_kernel<kGramFull>, grid, dim3(kThreads), 0, st, n, X, ldx, bx,
+                           Y, ldy, by, ws, rpb, nta, nl, nblk);
+    } else if (sym) {
       hipLaunchKernelGGL(gram_mfma_kernel<kGramDiag>, dim3((unsigned)round_up(nblk, kXcds)),
                          dim3(kThreads), 0, st, n, X, ldx, bx, Y, ldy, by, ws, rpb, nta, 1, nblk);
-    } else if (X == Y && bx == by && ldx == ldy) {
-      const int nsym = nta * (nta + 1) / 2;
-      hipLaunchKernelGGL(gram_mfma_kernel<kGramSym>, dim3((unsigned)round_up(nblk * nsym, kXcds)),
-                         dim3(kThreads), 0, st, n, X, ldx, bx, Y, ldy, by, ws, rpb, nta, nsym, nblk);
-    } else if (lower_only && bx == by && nta > 1) {
-      const int nsym = nta * (nta + 1) / 2;
-      hipLaunchKernelGGL(gram_mfma_kernel<kGramLower>,
-                         dim3((unsigned)round_up(nblk * nsym, kXcds)), dim3(kThreads), 0, st, n, X,
-                         ldx, bx, Y, ldy, by, ws, rpb, nta, nsym, nblk);
     } else
       hipLaunchKernelGGL(gram_mfma_kernel<kGramFull>,
                          dim3((unsigned)round_up(nblk * ntiles, kXcds)), dim3(kThreads), 0, st, n,
@@ -1843,11 +2001,18 @@ void launch_tall_gemm(int64_t n, const T* X, int ldx, int bx, const T* C, int ld
       RCG_HIP(hipGetLastError());
       return;
     }
-    // persistent over row tiles: about 4 workgroups per CU of a 256-CU part
-    const int64_t nstripe = std::max<int64_t>(1, std::min<int64_t>(nrb, 1024 / (int64_t)ct));
-    hipLaunchKernelGGL(tall_gemm_mfma_kernel, dim3((unsigned)round_up(nstripe * ct, kXcds)),
-                       dim3(kThreads), 0, st, n, X, ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo,
-                       nrb, (int)ct, nstripe);
+    // persistent over row tiles: as many workgroups as fit at once on a
+    // 256-CU part (four per CU, three with the registers that hold beta Z)
+    const int64_t wgs = beta != 0.0 ? 768 : 1024;
+    const int64_t nstripe = std::max<int64_t>(1, std::min<int64_t>(nrb, wgs / (int64_t)ct));
+    if (beta != 0.0)
+      hipLaunchKernelGGL(tall_gemm_mfma_kernel<true>, dim3((unsigned)round_up(nstripe * ct, kXcds)),
+                         dim3(kThreads), 0, st, n, X, ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo,
+                         nrb, (int)ct, nstripe);
+    else
+      hipLaunchKernelGGL(tall_gemm_mfma_kernel<false>, dim3((unsigned)round_up(nstripe * ct, kXcds)),
+                         dim3(kThreads), 0, st, n, X, ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo,
+                         nrb, (int)ct, nstripe);
   } else {
     hipLaunchKernelGGL(tall_gemm_kernel<T>, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
                        ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);
