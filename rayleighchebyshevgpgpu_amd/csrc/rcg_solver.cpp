@@ -711,6 +711,7 @@ class Engine {
   //   [2bb^2+bb, +bb)    vector: uploaded eigenvalues
   //   [2bb^2+2bb, +bb)   vector: downloaded sums
   double* dmatA() { return ctx_->small.as<double>(); }
+  double* dmatB() { return ctx_->small.as<double>() + bb_ * bb_; }
   double* dsums() { return ctx_->small.as<double>() + 2 * bb_ * bb_; }
   double* dtheta() { return dsums() + bb_; }
   double* hmatA() { return ctx_->pinned.as<double>(); }
@@ -1209,6 +1210,7 @@ class Engine {
   // orthogonalize (RayleighChebyshevLM.h:1782-1837): CholQR2, plus a third
   // pass when the first needed a shift.
   void orthonormalize(int64_t b) {
+    fold_b_ = 0;
     if (b <= 0) return;
     if (cx_) return orthonormalize_t<cplx>(b);
     orthonormalize_t<double>(b);
@@ -1247,6 +1249,7 @@ class Engine {
   // for j >= i, mirrored (conjugated).
   template <class S>
   void project(int64_t b, std::vector<S>& H) {
+    fold_b_ = 0;
     apply(V(), W_, b);
     H.assign(b * b, S(0.0));
     gram_to_host<S>(V(), ld_, b, W_, ld_, b, H.data(), true);
@@ -1256,16 +1259,122 @@ class Engine {
     }
   }
 
+  // Complex Gram of interleaved blocks from the real Gram g of the 2bx x 2by
+  // real columns (column-major, leading dimension 2bx).
+  template <class S>
+  static void combine_gram(const double* g, int64_t bx, int64_t by, S* host) {
+    if constexpr (std::is_same<S, double>::value) {
+      std::memcpy(host, g, sizeof(double) * bx * by);
+    } else {
+      const int64_t rx = 2 * bx;
+      for (int64_t c = 0; c < by; ++c)
+        for (int64_t a = 0; a < bx; ++a) {
+          const double g00 = g[2 * a + 2 * c * rx], g10 = g[2 * a + 1 + 2 * c * rx];
+          const double g01 = g[2 * a + (2 * c + 1) * rx], g11 = g[2 * a + 1 + (2 * c + 1) * rx];
+          host[a + c * bx] = cplx(g00 + g11, g01 - g10);
+        }
+    }
+  }
+
+  // orthogonalize + formVtAV (RayleighChebyshevLM.h:1782-1837, 1928-1969) with
+  // CholQR2's second pass folded into the projection, for blocks of at most 64
+  // real columns (the host b^3 products then cost less than the device block
+  // product they replace).  After one CholQR pass V1: W = A V1; G = V1^H V1 and
+  // H1 = V1^H W come back with one host wait; G = R^H R, and
+  // H = R^{-H} H1 R^{-1} is the projection on V1 R^{-1}, the basis the second
+  // pass would have formed.  R^{-1} is kept and multiplied into the Ritz
+  // rotation (rotate_residuals), so the block is rotated once.  Wider blocks,
+  // a first pass that needed a shift, or a G that does not factor take the
+  // plain passes.
+  template <class S>
+  void ortho_project(int64_t b, std::vector<S>& H) {
+    fold_b_ = 0;
+    if (b <= 0) return;
+    if (cw_ * b > kFoldCols) {
+      orthonormalize_t<S>(b);
+      project<S>(b, H);
+      return;
+    }
+    if (!cholqr_pass<S>(b, true)) {
+      cholqr_pass<S>(b, true);
+      cholqr_pass<S>(b, false);
+      project<S>(b, H);
+      return;
+    }
+    apply(V(), W_, b);
+    const int64_t rb = cw_ * b;
+    rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, V(), (int)ld_, (int)rb, dmatA(), ws(),
+                             ws_elems_, st_);
+    rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, W_, (int)ld_, (int)rb, dmatB(), ws(),
+                             ws_elems_, st_, true);
+    RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * rb * rb, hipMemcpyDeviceToHost, st_));
+    RCG_HIP(hipMemcpyAsync(hmatB(), dmatB(), sizeof(double) * rb * rb, hipMemcpyDeviceToHost, st_));
+    sync();
+    std::vector<S> G(b * b), H1(b * b);
+    combine_gram<S>(hmatA(), b, b, G.data());
+    combine_gram<S>(hmatB(), b, b, H1.data());
+    std::vector<S> R, Ri;
+    if (!rcg::dense::cholesky_upper(G.data(), b, R)) {
+      // V1 is left as it is: the plain second (and third) pass, then project
+      if (!cholqr_pass<S>(b, true)) cholqr_pass<S>(b, false);
+      project<S>(b, H);
+      return;
+    }
+    rcg::dense::upper_inverse(R, b, Ri);
+    for (int64_t i = 0; i < b; ++i) {
+      H1[i + i * b] = std::real(H1[i + i * b]);
+      for (int64_t j = i + 1; j < b; ++j) H1[i + j * b] = conj_s(H1[j + i * b]);
+    }
+    // T = H1 Ri, H = Ri^H T (Ri upper triangular)
+    std::vector<S> T(b * b, S(0.0));
+    for (int64_t j = 0; j < b; ++j)
+      for (int64_t k = 0; k <= j; ++k) {
+        const S r = Ri[k + j * b];
+        for (int64_t i = 0; i < b; ++i) T[i + j * b] += H1[i + k * b] * r;
+      }
+    H.assign(b * b, S(0.0));
+    for (int64_t j = 0; j < b; ++j)
+      for (int64_t i = 0; i < b; ++i) {
+        S t = S(0.0);
+        for (int64_t k = 0; k <= i; ++k) t += conj_s(Ri[k + i * b]) * T[k + j * b];
+        H[i + j * b] = t;
+      }
+    for (int64_t i = 0; i < b; ++i) {
+      H[i + i * b] = std::real(H[i + i * b]);
+      for (int64_t j = i + 1; j < b; ++j) {
+        const S m = 0.5 * (H[i + j * b] + conj_s(H[j + i * b]));
+        H[i + j * b] = m;
+        H[j + i * b] = conj_s(m);
+      }
+    }
+    fold<S>().swap(Ri);
+    fold_b_ = b;
+  }
+
   // createEigenVectorsAndResiduals (RayleighChebyshevLM.h:1984-2087):
   // V <- V Y; W = A V; r_c = ||theta_c V_c - W_c|| / ||V_c||, c < count.  With V
   // orthonormal and Y orthogonal the columns of V Y have unit norm to
   // round-off; the residuals are divided by the computed norms instead of
   // rescaling the block (the filter is linear and CholQR normalises), and the
   // columns handed out as eigenvectors are scaled at lock time.
+  // After ortho_project the block is V1 and the rotation is R^{-1} Y.
   template <class S>
   void rotate_residuals(int64_t b, const std::vector<S>& Yh, const std::vector<double>& th,
                         int64_t count, std::vector<double>& res) {
-    right_multiply<S>(b, Yh.data());
+    if (fold_b_ == b && b > 0) {
+      const std::vector<S>& Ri = fold<S>();
+      std::vector<S> C(b * b, S(0.0));
+      for (int64_t j = 0; j < b; ++j)
+        for (int64_t k = 0; k < b; ++k) {
+          const S y = Yh[k + j * b];
+          for (int64_t i = 0; i <= k; ++i) C[i + j * b] += Ri[i + k * b] * y;
+        }
+      fold_b_ = 0;
+      right_multiply<S>(b, C.data());
+    } else {
+      fold_b_ = 0;
+      right_multiply<S>(b, Yh.data());
+    }
     residuals_normalized(count, th.data(), res);
   }
 
@@ -1509,6 +1618,16 @@ class Engine {
   float* E_[2] = {nullptr, nullptr};  // FP32 correction blocks (mixed filter)
   float* R32_ = nullptr;              // FP32 residual block (mixed filter)
   std::vector<double> last_norms_;    // column norms of V at the last residuals
+  // the second CholQR factor's inverse kept by ortho_project for the rotation
+  static constexpr int64_t kFoldCols = 64;
+  int64_t fold_b_ = 0;
+  std::vector<double> fold_r_;
+  std::vector<cplx> fold_c_;
+  template <class S>
+  std::vector<S>& fold() {
+    if constexpr (std::is_same<S, double>::value) return fold_r_;
+    else return fold_c_;
+  }
   DevBuf coef_;                       // mixed filter coefficient table
   hipEvent_t coef_ev_ = nullptr;      // its upload from hcoef_ has finished
   HostBuf hcoef_;
@@ -1729,12 +1848,12 @@ class Solver {
 
         ta = wall();
         eng_.deflate(subspaceSize, foundSize);
-        eng_.orthonormalize(subspaceSize);
         count_ortho_ += 1;
         t_ortho_ += wall() - ta;
 
+        // orthogonalize and project (the CholQR passes count as t_rr)
         ta = wall();
-        eng_.project(subspaceSize, H);
+        eng_.ortho_project(subspaceSize, H);
         count_apply_ += subspaceSize;
         const double te = wall();
         rcg::dense::small_eigensystem(H.data(), subspaceSize, theta, Yh);
