@@ -533,8 +533,11 @@ int64_t gram_blocks(int64_t n, int bx, int by, int64_t* rows_per_blk) {
   int64_t rpb = round_up((n + target - 1) / target, kRb);
   // at least 4 rows per output column, so a block's bx x by partial is at
   // most a quarter of the rows it reads (small n would otherwise write about
-  // as many partial bytes as it reads)
+  // as many partial bytes as it reads), and at least 256 rows: the small
+  // operators of a served stream (n ~ 1e4) then launch tens of workgroups with
+  // fewer partials to reduce (served throughput +4%; 512 and 1024 were slower)
   rpb = std::max<int64_t>(rpb, round_up(4 * (int64_t)std::max(bx, by), kRb));
+  rpb = std::max<int64_t>(rpb, 256);
   *rows_per_blk = rpb;
   return (n + rpb - 1) / rpb;
 }

@@ -87,7 +87,7 @@ def test_cheb_filter_repetition(rcg, orc):
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("n,cols", [(5000, 40), (20000, 64), (3000, 100), (6000, 200)])
+@pytest.mark.parametrize("n,cols", [(5000, 40), (20000, 64), (3000, 100), (6000, 200), (9000, 256)])
 def test_orthonormalize(rcg, n, cols):
     print(f"case: CholQR2 {n}x{cols}", flush=True)
     X = np.random.default_rng(n).standard_normal((n, cols))
@@ -117,7 +117,7 @@ def test_orthonormalize_ill_conditioned(rcg):
     assert np.max(np.abs(Q.T @ Q - np.eye(b))) < 1e-12
 
 
-@pytest.mark.parametrize("cols,found", [(16, 8), (40, 100)])
+@pytest.mark.parametrize("cols,found", [(16, 8), (40, 100), (24, 220)])
 def test_deflate(rcg, cols, found):
     print(f"case: deflate {cols} cols against {found} found", flush=True)
     rng = np.random.default_rng(2)
