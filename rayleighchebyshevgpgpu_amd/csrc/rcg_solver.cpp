@@ -711,7 +711,6 @@ class Engine {
   //   [2bb^2+bb, +bb)    vector: uploaded eigenvalues
   //   [2bb^2+2bb, +bb)   vector: downloaded sums
   double* dmatA() { return ctx_->small.as<double>(); }
-  double* dmatB() { return ctx_->small.as<double>() + bb_ * bb_; }
   double* dsums() { return ctx_->small.as<double>() + 2 * bb_ * bb_; }
   double* dtheta() { return dsums() + bb_; }
   double* hmatA() { return ctx_->pinned.as<double>(); }
@@ -1305,14 +1304,15 @@ class Engine {
     const int64_t rb = cw_ * b;
     rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, V(), (int)ld_, (int)rb, dmatA(), ws(),
                              ws_elems_, st_);
-    rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, W_, (int)ld_, (int)rb, dmatB(), ws(),
-                             ws_elems_, st_, true);
-    RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * rb * rb, hipMemcpyDeviceToHost, st_));
-    RCG_HIP(hipMemcpyAsync(hmatB(), dmatB(), sizeof(double) * rb * rb, hipMemcpyDeviceToHost, st_));
+    // H1 right after G, so that one copy brings both back
+    rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, W_, (int)ld_, (int)rb, dmatA() + rb * rb,
+                             ws(), ws_elems_, st_, true);
+    RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * 2 * rb * rb, hipMemcpyDeviceToHost,
+                           st_));
     sync();
     std::vector<S> G(b * b), H1(b * b);
     combine_gram<S>(hmatA(), b, b, G.data());
-    combine_gram<S>(hmatB(), b, b, H1.data());
+    combine_gram<S>(hmatA() + rb * rb, b, b, H1.data());
     std::vector<S> R, Ri;
     if (!rcg::dense::cholesky_upper(G.data(), b, R)) {
       // V1 is left as it is: the plain second (and third) pass, then project
