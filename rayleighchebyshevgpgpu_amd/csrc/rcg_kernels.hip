@@ -46,6 +46,15 @@ constexpr bool kNtStreams = false;
 constexpr bool kNtStreams = true;
 #endif
 
+
+// A single-use scalar (the CSR arrays, read once per step), non-temporal
+// with the streams above (FP64 step at 128^3: 944 -> 943 us).
+template <typename T>
+__device__ __forceinline__ T ld_once(const T* p) {
+  if constexpr (kNtStreams) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // 16-byte vector access to a row segment.
 template <typename T>
 struct V16;
@@ -140,19 +149,19 @@ __device__ __forceinline__ void csr_rows(
   const int g = threadIdx.x % G;
   const int r = threadIdx.x / G;
   const int nr = (int)(i1 - i0);
-  for (int t = threadIdx.x; t <= nr; t += kThreads) s_rp[t] = rowptr[i0 + t];
+  for (int t = threadIdx.x; t <= nr; t += kThreads) s_rp[t] = ld_once(rowptr + i0 + t);
   __syncthreads();
   const int base = s_rp[0];
   const int nz = s_rp[nr] - base;
   const bool staged = nz <= kStageNz;
   if (staged)
     for (int t = threadIdx.x; t < nz; t += kThreads) {
-      s_col[t] = col[base + t];
+      s_col[t] = ld_once(col + base + t);
       if constexpr (CX) {
-        s_val[2 * t] = val[2 * (int64_t)(base + t)];
-        s_val[2 * t + 1] = val[2 * (int64_t)(base + t) + 1];
+        s_val[2 * t] = ld_once(val + 2 * (int64_t)(base + t));
+        s_val[2 * t + 1] = ld_once(val + 2 * (int64_t)(base + t) + 1);
       } else {
-        s_val[t] = val[base + t];
+        s_val[t] = ld_once(val + base + t);
       }
     }
   __syncthreads();
