@@ -375,9 +375,7 @@ __global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __
 #ifndef RCG_CSR_RPI
 #define RCG_CSR_RPI 1
 #endif
-#ifndef RCG_CSR_GDIV
-#define RCG_CSR_GDIV 1
-#endif
+
 
 template <typename T, int MODE, bool CX, int G, int NV>
 void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
@@ -418,15 +416,23 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
   if (ld % 4 != 0) throw Error(1, "block leading dimension must be a multiple of 4");
 #define RCG_CSR_CASE(LIM, GG, NN) \
   if (ncv <= LIM) return csr_launch_g<T, MODE, CX, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st, sc, R, kap);
+  // (16-byte vectors per row, lanes per row G, vectors per lane NV).  From 8
+  // vectors on, each lane carries two or more vectors: half the lanes per row
+  // of one vector each, twice the rows per wave (FP64 step at 128^3, b = 64:
+  // 942 -> 867 us; 1024^2 complex, b = 96: 1351 -> 1343 us; four vectors per
+  // lane was slower).  A column's sum runs over the row's nonzeros in the same
+  // order for any (G, NV), so results do not depend on this table; the
+  // batched step keeps one vector per lane (measured faster for the small
+  // served operators).
   RCG_CSR_CASE(1, 1, 1)
   RCG_CSR_CASE(2, 2, 1)
   RCG_CSR_CASE(4, 4, 1)
-  RCG_CSR_CASE(8, 8, 1)
-  RCG_CSR_CASE(16, 16 / RCG_CSR_GDIV, RCG_CSR_GDIV)
-  RCG_CSR_CASE(32, 32 / RCG_CSR_GDIV, RCG_CSR_GDIV)
-  RCG_CSR_CASE(64, 64 / RCG_CSR_GDIV, RCG_CSR_GDIV)
-  RCG_CSR_CASE(128, 64, 2)
-  RCG_CSR_CASE(256, 64, 4)
+  RCG_CSR_CASE(8, 4, 2)
+  RCG_CSR_CASE(16, 8, 2)
+  RCG_CSR_CASE(32, 16, 2)
+  RCG_CSR_CASE(64, 32, 2)
+  RCG_CSR_CASE(128, 32, 4)
+  RCG_CSR_CASE(256, 32, 8)
 #undef RCG_CSR_CASE
   throw Error(6, "CSR block kernels support at most 256 16-byte column vectors per row");
 }
