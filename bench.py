@@ -477,9 +477,9 @@ def main():
                     if precision == "mixed" else
                     "csr_block_kernel<double,...,kCheb,complex> (fused complex128 Chebyshev step)")
         else:
-            kern = ("csr_block_kernel<float,16,1,kChebSrc> (FP32 correction step)"
+            kern = ("csr_block_kernel<float,8,2,kChebSrc> (FP32 correction step)"
                     if precision == "mixed" else
-                    "csr_block_kernel<double,32,1,kCheb> (fused FP64 Chebyshev step)")
+                    "csr_block_kernel<double,16,2,kCheb> (fused FP64 Chebyshev step)")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profiles(precision), "kernel": kern,
