@@ -421,9 +421,10 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
   // of one vector each, twice the rows per wave (FP64 step at 128^3, b = 64:
   // 942 -> 867 us; 1024^2 complex, b = 96: 1351 -> 1343 us; four vectors per
   // lane was slower).  A column's sum runs over the row's nonzeros in the same
-  // order for any (G, NV), so results do not depend on this table; the
-  // batched step keeps one vector per lane (measured faster for the small
-  // served operators).
+  // order for any (G, NV), so results do not depend on this table.  The
+  // batched step keeps one vector per lane: the same change there gave a
+  // slower served stream (1177 against 1192 eigenpairs/s) and, built into this
+  // file, a 3% slower single-operator FP64 step.
   RCG_CSR_CASE(1, 1, 1)
   RCG_CSR_CASE(2, 2, 1)
   RCG_CSR_CASE(4, 4, 1)
