@@ -17,8 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librcg.so")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["rcg_kernels.hip", "rcg_solver.cpp", "rcg_io.cpp"]
-HEADERS = ["rcg_internal.hpp", "rcg_cpoly.hpp", "rcg_dense.hpp"]
+SOURCES = ["rcg_kernels.hip", "rcg_batch.hip", "rcg_solver.cpp", "rcg_io.cpp"]
+HEADERS = ["rcg_internal.hpp", "rcg_csr_rows.hpp", "rcg_cpoly.hpp", "rcg_dense.hpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [

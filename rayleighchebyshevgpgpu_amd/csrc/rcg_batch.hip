@@ -1,0 +1,97 @@
+// rcg_batch.hip -- the batched filter step of the serving loop: step j of
+// the Chebyshev filters of many operators in one launch.  A translation unit
+// of its own, so that its instantiations of csr_rows cannot change the code
+// generated for the single-operator step in rcg_kernels.hip.
+#include "rcg_csr_rows.hpp"
+
+namespace rcg {
+namespace {
+
+// ---------------------------------------------------- batched filter step
+// Step j of the Chebyshev filters of many operators at once: task t is row
+// block tasks[t].blk of request tasks[t].req.  Requests hold their own CSR
+// arrays, blocks and coefficients; a request's blocks buf[0], buf[1] are the
+// recurrence's ping-pong pair (X_{j-1} in buf[(j-1)%2], X_{j-2} and X_j in
+// buf[j%2]; the first step reads buf[0] and writes buf[1]).  SRC: the FP32
+// correction steps (j >= 2 only) with the source block R and the kap table
+// (row j at kap + j*ld).  Workgroups stride over the tasks.
+template <typename T, int G, int NV, bool SRC, bool CX>
+__global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __restrict__ descs,
+                                                             const BatchTask* __restrict__ tasks,
+                                                             int ntasks, int j) {
+  __shared__ int s_rp[kMaxRpb + 1];
+  __shared__ int s_col[kStageNz];
+  __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
+  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    const BatchTask tk = tasks[t];
+    const BatchDesc& d = descs[tk.req];
+    const int64_t i0 = (int64_t)tk.blk * d.rows_per_blk;
+    const int64_t i1 = min(d.n, i0 + d.rows_per_blk);
+    const T* val = static_cast<const T*>(d.val);
+    T* b0 = static_cast<T*>(d.buf[0]);
+    T* b1 = static_cast<T*>(d.buf[1]);
+    const T g1 = T(d.g1), g2 = T(d.g2);
+    if constexpr (SRC) {
+      T* xin = (j % 2 == 0) ? b1 : b0;
+      T* xz = (j % 2 == 0) ? b0 : b1;
+      csr_rows<T, G, NV, kChebSrc, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, g1,
+                                          g2, T(1), i0, i1, static_cast<const T*>(d.R),
+                                          static_cast<const T*>(d.kap) + (int64_t)j * d.ld, s_rp,
+                                          s_col, s_val);
+    } else if (j == 1) {
+      csr_rows<T, G, NV, kChebFirst, 1, CX>(d.n, d.rowptr, d.col, val, b0, nullptr, b1, d.ld,
+                                            d.ncv, g1, g2, T(d.m == 1 ? 0.5 : 1.0), i0, i1,
+                                            nullptr, nullptr, s_rp, s_col, s_val);
+    } else {
+      T* xin = (j % 2 == 0) ? b1 : b0;
+      T* xz = (j % 2 == 0) ? b0 : b1;
+      const T sc = (j == d.m) ? T(1.0 / double(d.m + 1)) : T(1);
+      csr_rows<T, G, NV, kCheb, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, g1, g2,
+                                       sc, i0, i1, nullptr, nullptr, s_rp, s_col, s_val);
+    }
+    __syncthreads();  // the LDS stages are reused by the next task
+  }
+}
+
+}  // namespace
+
+template <typename T, bool SRC, bool CX>
+void cheb_batch_t(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, int cls,
+                  hipStream_t st) {
+  const unsigned grid = (unsigned)std::min(ntasks, 8192);
+#define RCG_BATCH_CASE(C, GG, NN)                                                                 \
+  case C:                                                                                         \
+    hipLaunchKernelGGL((csr_batch_kernel<T, GG, NN, SRC, CX>), dim3(grid), dim3(kThreads), 0, st, \
+                       descs, tasks, ntasks, j);                                                  \
+    break;
+  switch (cls) {
+    RCG_BATCH_CASE(0, 1, 1)
+    RCG_BATCH_CASE(1, 2, 1)
+    RCG_BATCH_CASE(2, 4, 1)
+    RCG_BATCH_CASE(3, 8, 1)
+    RCG_BATCH_CASE(4, 16, 1)
+    RCG_BATCH_CASE(5, 32, 1)
+    RCG_BATCH_CASE(6, 64, 1)
+    RCG_BATCH_CASE(7, 64, 2)
+    RCG_BATCH_CASE(8, 64, 4)
+    default: throw Error(1, "bad CSR launch class");
+  }
+#undef RCG_BATCH_CASE
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_cheb_batch(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, bool f32,
+                       bool src, bool cx, int cls, hipStream_t st) {
+  if (ntasks <= 0) return;
+  if (f32) {
+    if (!src) throw Error(1, "FP32 batched steps are the correction steps");
+    if (cx) cheb_batch_t<float, true, true>(descs, tasks, ntasks, j, cls, st);
+    else cheb_batch_t<float, true, false>(descs, tasks, ntasks, j, cls, st);
+  } else {
+    if (src) throw Error(1, "FP64 batched steps are the plain recurrence");
+    if (cx) cheb_batch_t<double, false, true>(descs, tasks, ntasks, j, cls, st);
+    else cheb_batch_t<double, false, false>(descs, tasks, ntasks, j, cls, st);
+  }
+}
+
+}  // namespace rcg

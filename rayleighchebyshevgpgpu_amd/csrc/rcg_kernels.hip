@@ -18,295 +18,10 @@
 #include <cmath>
 #include <type_traits>
 
-#include "rcg_internal.hpp"
+#include "rcg_csr_rows.hpp"
 
 namespace rcg {
 namespace {
-
-constexpr int kXcds = 8;
-constexpr int kThreads = 256;
-
-// Logical block for physical block `bid` of `nb` (nb a multiple of 8).  The
-// hardware deals workgroups round-robin to the 8 XCDs; this mapping gives
-// XCD x the logical blocks [x*nb/8, (x+1)*nb/8), i.e. contiguous rows.
-__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
-#ifdef RCG_NO_XCD_REMAP
-  return bid;
-#else
-  const int64_t per = nb / kXcds;
-  return (bid % kXcds) * per + bid / kXcds;
-#endif
-}
-
-// Non-temporal single-use streams: 3% faster FP64 and FP32 filter steps at
-// 128^3 (the gathered rows keep more of L2); RCG_TEMPORAL_STREAMS turns them off.
-#ifdef RCG_TEMPORAL_STREAMS
-constexpr bool kNtStreams = false;
-#else
-constexpr bool kNtStreams = true;
-#endif
-
-
-// A single-use scalar (the CSR arrays, read once per step), non-temporal
-// with the streams above (FP64 step at 128^3: 944 -> 943 us).
-template <typename T>
-__device__ __forceinline__ T ld_once(const T* p) {
-  if constexpr (kNtStreams) return __builtin_nontemporal_load(p);
-  else return *p;
-}
-
-// 16-byte vector access to a row segment.
-template <typename T>
-struct V16;
-// ld/st go through the caches normally; ld_stream/st_stream mark single-use
-// data (the X_{j-2} read and the X_j write of a step) non-temporal, so L2
-// keeps the gathered X_{j-1} rows.
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-typedef float fvec4 __attribute__((ext_vector_type(4)));
-template <>
-struct V16<double> {
-  static constexpr int W = 2;
-  __device__ static void ld(const double* p, double (&o)[2]) {
-    const dvec2 t = *reinterpret_cast<const dvec2*>(p);
-    o[0] = t.x;
-    o[1] = t.y;
-  }
-  __device__ static void st(double* p, const double (&o)[2]) {
-    *reinterpret_cast<dvec2*>(p) = dvec2{o[0], o[1]};
-  }
-  __device__ static void ld_stream(const double* p, double (&o)[2]) {
-    if constexpr (kNtStreams) {
-      const dvec2 t = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p));
-      o[0] = t.x;
-      o[1] = t.y;
-    } else {
-      ld(p, o);
-    }
-  }
-  __device__ static void st_stream(double* p, const double (&o)[2]) {
-    if constexpr (kNtStreams) __builtin_nontemporal_store(dvec2{o[0], o[1]}, reinterpret_cast<dvec2*>(p));
-    else st(p, o);
-  }
-};
-template <>
-struct V16<float> {
-  static constexpr int W = 4;
-  __device__ static void ld(const float* p, float (&o)[4]) {
-    const fvec4 t = *reinterpret_cast<const fvec4*>(p);
-    o[0] = t.x;
-    o[1] = t.y;
-    o[2] = t.z;
-    o[3] = t.w;
-  }
-  __device__ static void st(float* p, const float (&o)[4]) {
-    *reinterpret_cast<fvec4*>(p) = fvec4{o[0], o[1], o[2], o[3]};
-  }
-  __device__ static void ld_stream(const float* p, float (&o)[4]) {
-    if constexpr (kNtStreams) {
-      const fvec4 t = __builtin_nontemporal_load(reinterpret_cast<const fvec4*>(p));
-      o[0] = t.x;
-      o[1] = t.y;
-      o[2] = t.z;
-      o[3] = t.w;
-    } else {
-      ld(p, o);
-    }
-  }
-  __device__ static void st_stream(float* p, const float (&o)[4]) {
-    if constexpr (kNtStreams) __builtin_nontemporal_store(fvec4{o[0], o[1], o[2], o[3]}, reinterpret_cast<fvec4*>(p));
-    else st(p, o);
-  }
-};
-
-// ------------------------------------------------------------ CSR x block
-// kChebSrc: kCheb plus a per-column multiple of a source block,
-//   out = p2 * (p1*(p0*A*X + X) - Z + kap[c] * R),
-// the correction recurrence of the mixed-precision filter.
-enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3, kChebSrc = 4 };
-
-// Rows per workgroup block (cap) and nonzeros staged in LDS per block.
-constexpr int kMaxRpb = 512;
-constexpr int kStageNz = 1024;
-
-// G lanes own one row; each lane handles NV 16-byte column vectors at
-// cv = g + k*G.  256/G rows are in flight per workgroup.  The block's row
-// pointers (and, when they fit, its column indices and values) are staged in
-// LDS by one coalesced load, and a row's own-row operands are loaded before
-// its gathers, so that computing a row costs a single trip to memory.
-// CX: complex operator and block (interleaved re, im): each 16-byte vector
-// holds W/2 complex entries and the gather multiplies complex values.
-// Rows [i0, i1) of one step (the body shared by the single-operator and the
-// batched kernels); s_rp / s_col / s_val are the workgroup's LDS stages.
-template <typename T, int G, int NV, int MODE, int RPI, bool CX>
-__device__ __forceinline__ void csr_rows(
-    int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
-    const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
-    T p0, T p1, T p2, int64_t i0, int64_t i1, const T* __restrict__ R,
-    const T* __restrict__ kap, int* s_rp, int* s_col, T* s_val) {
-  using VT = V16<T>;
-  constexpr int W = VT::W;
-  constexpr int RS = kThreads / G;
-  const int g = threadIdx.x % G;
-  const int r = threadIdx.x / G;
-  const int nr = (int)(i1 - i0);
-  for (int t = threadIdx.x; t <= nr; t += kThreads) s_rp[t] = ld_once(rowptr + i0 + t);
-  __syncthreads();
-  const int base = s_rp[0];
-  const int nz = s_rp[nr] - base;
-  const bool staged = nz <= kStageNz;
-  if (staged)
-    for (int t = threadIdx.x; t < nz; t += kThreads) {
-      s_col[t] = ld_once(col + base + t);
-      if constexpr (CX) {
-        s_val[2 * t] = ld_once(val + 2 * (int64_t)(base + t));
-        s_val[2 * t + 1] = ld_once(val + 2 * (int64_t)(base + t) + 1);
-      } else {
-        s_val[t] = ld_once(val + base + t);
-      }
-    }
-  __syncthreads();
-  // RPI rows per lane iteration: rows li and li + RS, ... so that each wave
-  // has RPI rows' gathers in flight at once.
-  for (int li0 = r; li0 < nr; li0 += RS * RPI) {
-    int pb[RPI], pe[RPI];
-    bool live_row[RPI];
-#pragma unroll
-    for (int q = 0; q < RPI; ++q) {
-      const int li = li0 + q * RS;
-      live_row[q] = li < nr;
-      pb[q] = live_row[q] ? s_rp[li] : 0;
-      pe[q] = live_row[q] ? s_rp[li + 1] : 0;
-    }
-    // own-row operands first: they do not depend on the gathers
-    T xs[RPI][NV][W], zs[RPI][NV][W];
-    T rs[MODE == kChebSrc ? RPI : 1][NV][W];
-#pragma unroll
-    for (int q = 0; q < RPI; ++q) {
-      if (!live_row[q]) continue;
-      const int64_t rowoff = (i0 + li0 + q * RS) * (int64_t)ld;
-#pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        const int cv = g + k * G;
-        if (cv < ncv) {
-          if constexpr (MODE == kChebFirst || MODE == kCheb || MODE == kChebSrc)
-            VT::ld(X + rowoff + cv * W, xs[q][k]);
-          if constexpr (MODE == kCheb || MODE == kChebSrc) VT::ld_stream(Z + rowoff + cv * W, zs[q][k]);
-          if constexpr (MODE == kChebSrc) VT::ld_stream(R + rowoff + cv * W, rs[q][k]);
-          if constexpr (MODE == kApplyBeta) VT::ld(out + rowoff + cv * W, zs[q][k]);
-        }
-      }
-    }
-    T acc[RPI][NV][W];
-#pragma unroll
-    for (int q = 0; q < RPI; ++q)
-#pragma unroll
-      for (int k = 0; k < NV; ++k)
-#pragma unroll
-        for (int w = 0; w < W; ++w) acc[q][k][w] = T(0);
-    int maxlen = 0;
-#pragma unroll
-    for (int q = 0; q < RPI; ++q) maxlen = max(maxlen, pe[q] - pb[q]);
-    // chunks of 4 nonzeros per row: every gather of a chunk (over the RPI
-    // rows) is issued before its FMAs
-    for (int c = 0; c < maxlen; c += 4) {
-      int jj[RPI][4];
-      T aa[RPI][4], ai[RPI][4];
-#pragma unroll
-      for (int q = 0; q < RPI; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int p = pb[q] + c + e;
-          const bool live = p < pe[q];
-          if constexpr (CX) {
-            if (staged) {
-              jj[q][e] = live ? s_col[p - base] : 0;
-              aa[q][e] = live ? s_val[2 * (p - base)] : T(0);
-              ai[q][e] = live ? s_val[2 * (p - base) + 1] : T(0);
-            } else {
-              jj[q][e] = live ? col[p] : 0;
-              aa[q][e] = live ? val[2 * (int64_t)p] : T(0);
-              ai[q][e] = live ? val[2 * (int64_t)p + 1] : T(0);
-            }
-          } else {
-            if (staged) {
-              jj[q][e] = live ? s_col[p - base] : 0;
-              aa[q][e] = live ? s_val[p - base] : T(0);
-            } else {
-              jj[q][e] = live ? col[p] : 0;
-              aa[q][e] = live ? val[p] : T(0);
-            }
-          }
-        }
-      T u[RPI][4][NV][W];
-#pragma unroll
-      for (int q = 0; q < RPI; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (pb[q] + c + e < pe[q]) {
-            const T* xr = X + (int64_t)jj[q][e] * ld;
-#pragma unroll
-            for (int k = 0; k < NV; ++k) {
-              const int cv = g + k * G;
-              if (cv < ncv) VT::ld(xr + cv * W, u[q][e][k]);
-            }
-          }
-        }
-#pragma unroll
-      for (int q = 0; q < RPI; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (pb[q] + c + e < pe[q]) {
-#pragma unroll
-            for (int k = 0; k < NV; ++k) {
-              if constexpr (CX) {
-#pragma unroll
-                for (int w = 0; w < W; w += 2) {
-                  acc[q][k][w] = fma(aa[q][e], u[q][e][k][w], acc[q][k][w]);
-                  acc[q][k][w] = fma(-ai[q][e], u[q][e][k][w + 1], acc[q][k][w]);
-                  acc[q][k][w + 1] = fma(aa[q][e], u[q][e][k][w + 1], acc[q][k][w + 1]);
-                  acc[q][k][w + 1] = fma(ai[q][e], u[q][e][k][w], acc[q][k][w + 1]);
-                }
-              } else {
-#pragma unroll
-                for (int w = 0; w < W; ++w) acc[q][k][w] = fma(aa[q][e], u[q][e][k][w], acc[q][k][w]);
-              }
-            }
-          }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < RPI; ++q) {
-      if (!live_row[q]) continue;
-      const int64_t rowoff = (i0 + li0 + q * RS) * (int64_t)ld;
-#pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        const int cv = g + k * G;
-        if (cv >= ncv) continue;
-        T o[W];
-        if constexpr (MODE == kApply) {
-#pragma unroll
-          for (int w = 0; w < W; ++w) o[w] = p0 * acc[q][k][w];
-        } else if constexpr (MODE == kApplyBeta) {
-#pragma unroll
-          for (int w = 0; w < W; ++w) o[w] = fma(p0, acc[q][k][w], p1 * zs[q][k][w]);
-        } else if constexpr (MODE == kChebFirst) {
-#pragma unroll
-          for (int w = 0; w < W; ++w) o[w] = p2 * (p1 * fma(p0, acc[q][k][w], xs[q][k][w]));
-        } else if constexpr (MODE == kChebSrc) {
-#pragma unroll
-          for (int w = 0; w < W; ++w)
-            o[w] = p2 * fma(kap[cv * W + w], rs[q][k][w],
-                            fma(p1, fma(p0, acc[q][k][w], xs[q][k][w]), -zs[q][k][w]));
-        } else {
-#pragma unroll
-          for (int w = 0; w < W; ++w)
-            o[w] = p2 * fma(p1, fma(p0, acc[q][k][w], xs[q][k][w]), -zs[q][k][w]);
-        }
-        VT::st_stream(out + rowoff + cv * W, o);
-      }
-    }
-  }
-}
 
 template <typename T, int G, int NV, int MODE, int RPI, bool CX>
 __global__ __launch_bounds__(kThreads) void csr_block_kernel(
@@ -324,52 +39,6 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
   const int64_t i1 = min(n, i0 + rows_per_blk);
   csr_rows<T, G, NV, MODE, RPI, CX>(n, rowptr, col, val, X, Z, out, ld, ncv, p0, p1, p2, i0, i1, R,
                                     kap, s_rp, s_col, s_val);
-}
-
-// ---------------------------------------------------- batched filter step
-// Step j of the Chebyshev filters of many operators at once: task t is row
-// block tasks[t].blk of request tasks[t].req.  Requests hold their own CSR
-// arrays, blocks and coefficients; a request's blocks buf[0], buf[1] are the
-// recurrence's ping-pong pair (X_{j-1} in buf[(j-1)%2], X_{j-2} and X_j in
-// buf[j%2]; the first step reads buf[0] and writes buf[1]).  SRC: the FP32
-// correction steps (j >= 2 only) with the source block R and the kap table
-// (row j at kap + j*ld).  Workgroups stride over the tasks.
-template <typename T, int G, int NV, bool SRC, bool CX>
-__global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __restrict__ descs,
-                                                             const BatchTask* __restrict__ tasks,
-                                                             int ntasks, int j) {
-  __shared__ int s_rp[kMaxRpb + 1];
-  __shared__ int s_col[kStageNz];
-  __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
-  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
-    const BatchTask tk = tasks[t];
-    const BatchDesc& d = descs[tk.req];
-    const int64_t i0 = (int64_t)tk.blk * d.rows_per_blk;
-    const int64_t i1 = min(d.n, i0 + d.rows_per_blk);
-    const T* val = static_cast<const T*>(d.val);
-    T* b0 = static_cast<T*>(d.buf[0]);
-    T* b1 = static_cast<T*>(d.buf[1]);
-    const T g1 = T(d.g1), g2 = T(d.g2);
-    if constexpr (SRC) {
-      T* xin = (j % 2 == 0) ? b1 : b0;
-      T* xz = (j % 2 == 0) ? b0 : b1;
-      csr_rows<T, G, NV, kChebSrc, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, g1,
-                                          g2, T(1), i0, i1, static_cast<const T*>(d.R),
-                                          static_cast<const T*>(d.kap) + (int64_t)j * d.ld, s_rp,
-                                          s_col, s_val);
-    } else if (j == 1) {
-      csr_rows<T, G, NV, kChebFirst, 1, CX>(d.n, d.rowptr, d.col, val, b0, nullptr, b1, d.ld,
-                                            d.ncv, g1, g2, T(d.m == 1 ? 0.5 : 1.0), i0, i1,
-                                            nullptr, nullptr, s_rp, s_col, s_val);
-    } else {
-      T* xin = (j % 2 == 0) ? b1 : b0;
-      T* xz = (j % 2 == 0) ? b0 : b1;
-      const T sc = (j == d.m) ? T(1.0 / double(d.m + 1)) : T(1);
-      csr_rows<T, G, NV, kCheb, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, g1, g2,
-                                       sc, i0, i1, nullptr, nullptr, s_rp, s_col, s_val);
-    }
-    __syncthreads();  // the LDS stages are reused by the next task
-  }
 }
 
 #ifndef RCG_CSR_RPI
@@ -422,9 +91,9 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
   // 942 -> 867 us; 1024^2 complex, b = 96: 1351 -> 1343 us; four vectors per
   // lane was slower).  A column's sum runs over the row's nonzeros in the same
   // order for any (G, NV), so results do not depend on this table.  The
-  // batched step keeps one vector per lane: the same change there gave a
-  // slower served stream (1177 against 1192 eigenpairs/s) and, built into this
-  // file, a 3% slower single-operator FP64 step.
+  // batched step (rcg_batch.hip) keeps one vector per lane: the same change
+  // there served 1168-1182 against 1181-1186 eigenpairs/s (and, while both
+  // kernels shared this file, made the single-operator FP64 step 3% slower).
   RCG_CSR_CASE(1, 1, 1)
   RCG_CSR_CASE(2, 2, 1)
   RCG_CSR_CASE(4, 4, 1)
@@ -1889,45 +1558,6 @@ int csr_class(int ncv) {
   for (int c = 0; c < 9; ++c)
     if (ncv <= lim[c]) return c;
   throw Error(6, "CSR block kernels support at most 256 16-byte column vectors per row");
-}
-
-template <typename T, bool SRC, bool CX>
-void cheb_batch_t(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, int cls,
-                  hipStream_t st) {
-  const unsigned grid = (unsigned)std::min(ntasks, 8192);
-#define RCG_BATCH_CASE(C, GG, NN)                                                                 \
-  case C:                                                                                         \
-    hipLaunchKernelGGL((csr_batch_kernel<T, GG, NN, SRC, CX>), dim3(grid), dim3(kThreads), 0, st, \
-                       descs, tasks, ntasks, j);                                                  \
-    break;
-  switch (cls) {
-    RCG_BATCH_CASE(0, 1, 1)
-    RCG_BATCH_CASE(1, 2, 1)
-    RCG_BATCH_CASE(2, 4, 1)
-    RCG_BATCH_CASE(3, 8, 1)
-    RCG_BATCH_CASE(4, 16, 1)
-    RCG_BATCH_CASE(5, 32, 1)
-    RCG_BATCH_CASE(6, 64, 1)
-    RCG_BATCH_CASE(7, 64, 2)
-    RCG_BATCH_CASE(8, 64, 4)
-    default: throw Error(1, "bad CSR launch class");
-  }
-#undef RCG_BATCH_CASE
-  RCG_HIP(hipGetLastError());
-}
-
-void launch_cheb_batch(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, bool f32,
-                       bool src, bool cx, int cls, hipStream_t st) {
-  if (ntasks <= 0) return;
-  if (f32) {
-    if (!src) throw Error(1, "FP32 batched steps are the correction steps");
-    if (cx) cheb_batch_t<float, true, true>(descs, tasks, ntasks, j, cls, st);
-    else cheb_batch_t<float, true, false>(descs, tasks, ntasks, j, cls, st);
-  } else {
-    if (src) throw Error(1, "FP64 batched steps are the plain recurrence");
-    if (cx) cheb_batch_t<double, false, true>(descs, tasks, ntasks, j, cls, st);
-    else cheb_batch_t<double, false, false>(descs, tasks, ntasks, j, cls, st);
-  }
 }
 
 template <typename T>
