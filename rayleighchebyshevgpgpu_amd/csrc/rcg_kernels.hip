@@ -88,8 +88,10 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
   // (16-byte vectors per row, lanes per row G, vectors per lane NV).  From 8
   // vectors on, each lane carries two or more vectors: half the lanes per row
   // of one vector each, twice the rows per wave (FP64 step at 128^3, b = 64:
-  // 942 -> 867 us; 1024^2 complex, b = 96: 1351 -> 1343 us; four vectors per
-  // lane was slower).  A column's sum runs over the row's nonzeros in the same
+  // 942 -> 867 us; 1024^2 complex, b = 96: 1351 -> 1343 us).  Four vectors
+  // per lane from 33 vectors on: the complex64 step at b = 96 (48 vectors),
+  // 925 -> 709 us; four per lane at 16-32 vectors and eight at 65-128 were
+  // slower.  A column's sum runs over the row's nonzeros in the same
   // order for any (G, NV), so results do not depend on this table.  The
   // batched step (rcg_batch.hip) keeps one vector per lane: the same change
   // there served 1168-1182 against 1181-1186 eigenpairs/s (and, while both
@@ -100,7 +102,7 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
   RCG_CSR_CASE(8, 4, 2)
   RCG_CSR_CASE(16, 8, 2)
   RCG_CSR_CASE(32, 16, 2)
-  RCG_CSR_CASE(64, 32, 2)
+  RCG_CSR_CASE(64, 16, 4)
   RCG_CSR_CASE(128, 32, 4)
   RCG_CSR_CASE(256, 32, 8)
 #undef RCG_CSR_CASE
