@@ -108,9 +108,11 @@ int rcg_block_info(rcg_block b, int64_t* n, int64_t* cols, int64_t* ld, int* dty
  * (mem = RCG_MEM_HOST) or referenced (RCG_MEM_DEVICE: the caller keeps them
  * alive).  val_dtype: RCG_F64, or RCG_C128 for a Hermitian operator (values
  * as interleaved re, im pairs, the layout of double _Complex); the FP32 /
- * complex64 copy for mixed-precision filtering is made on demand.  Complex
- * operators work with rcg_operator_apply, rcg_residuals, rcg_lanczos_bounds
- * and rcg_solve_lowest; the other piecewise entry points are real only. */
+ * complex64 copy for mixed-precision filtering is made on demand.  A complex
+ * operator takes C128 blocks in every entry point that has an operator; the
+ * block-only entry points (rcg_orthonormalize, rcg_deflate, rcg_rotate) take
+ * FP64 or C128 blocks, like the reference's D- and Z- library calls
+ * (RayleighChebyshevLMCuda.h:2030-2047, 2139-2161). */
 int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32_t* rowptr,
                             const int32_t* col, const void* val, int val_dtype, int mem,
                             rcg_operator* out);
@@ -151,7 +153,8 @@ int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alph
  * and LanczosCpolyOperatorLM::apply (LanczosCpolyOperatorLM.h:192-304):
  * X <- P_m(A')^rep X with A' = (A + shift)/rhoB, rhoB = (lambda_max + shift)
  * / UpperX*(m), over the first `cols` columns; tmp is a block of X's shape
- * used as the second recurrence buffer. */
+ * used as the second recurrence buffer.  Columns past `cols` are not
+ * written.  C128 blocks with a C128 operator (complex columns). */
 int rcg_cheb_filter(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols, int64_t degree,
                     int64_t repetition, double lambda_max, double shift);
 /* No counterpart in the reference (its filter is FP64 only): the same filter
@@ -174,11 +177,14 @@ int rcg_orthonormalize(rcg_block X, rcg_block tmp, int64_t cols);
  * (classical Gram-Schmidt applied twice). */
 int rcg_deflate(rcg_block X, int64_t cols, rcg_block Y, int64_t found);
 /* formVtAV (RayleighChebyshevLM.h:1928-1969, RayleighChebyshevLMCuda.h
- * :2115-2163): W = A V, H = V^H W (cols x cols, column-major, host). */
+ * :2115-2163): W = A V, H = V^H W (cols x cols, column-major, host).  A C128
+ * operator takes C128 blocks and H is complex: cols x cols interleaved
+ * (re, im) pairs, 2 cols^2 doubles (the cublasZgemm at LMC:2139-2161). */
 int rcg_project(rcg_operator op, rcg_block V, rcg_block W, int64_t cols, double* H_host);
 /* V*Y rotation of createEigenVectorsAndResiduals
  * (RayleighChebyshevLMCuda.h:2200-2232): V <- normalize(V Y), Y host
- * column-major cols x cols; tmp as above. */
+ * column-major cols x cols; tmp as above.  C128 blocks take a complex Y
+ * (interleaved (re, im) pairs). */
 int rcg_rotate(rcg_block V, rcg_block tmp, int64_t cols, const double* Y_host);
 /* computeVtVeigensystem (RayleighChebyshevLM.h:478-569,
  * RayleighChebyshevLMCuda.h:570-642): ascending eigenvalues and column
@@ -265,7 +271,9 @@ int rcg_solve_lowest(rcg_operator op, const rcg_solve_params* params, rcg_solve_
  * continuous-batching solve server on one device.  Requests queue in arrival
  * order and are admitted when one of `max_active` worker slots is free and
  * the request's device footprint fits `hbm_budget` bytes next to the running
- * solves.  Running solves share their Chebyshev filter launches (a batcher
+ * solves and the resident eigenvectors of finished tickets (counted until
+ * rcg_server_release); a request that does not fit runs once no other solve
+ * is running.  Running solves share their Chebyshev filter launches (a batcher
  * gathers the sweeps of all running solves for up to `window_us`
  * microseconds); everything else runs on each worker's own stream.  A served
  * solve gives the same result as the same request solved alone.  The caller

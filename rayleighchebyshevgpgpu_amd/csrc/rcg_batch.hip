@@ -34,19 +34,19 @@ __global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __
     if constexpr (SRC) {
       T* xin = (j % 2 == 0) ? b1 : b0;
       T* xz = (j % 2 == 0) ? b0 : b1;
-      csr_rows<T, G, NV, kChebSrc, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, g1,
+      csr_rows<T, G, NV, kChebSrc, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, d.tail, g1,
                                           g2, T(1), i0, i1, static_cast<const T*>(d.R),
                                           static_cast<const T*>(d.kap) + (int64_t)j * d.ld, s_rp,
                                           s_col, s_val);
     } else if (j == 1) {
       csr_rows<T, G, NV, kChebFirst, 1, CX>(d.n, d.rowptr, d.col, val, b0, nullptr, b1, d.ld,
-                                            d.ncv, g1, g2, T(d.m == 1 ? 0.5 : 1.0), i0, i1,
+                                            d.ncv, d.tail, g1, g2, T(d.m == 1 ? 0.5 : 1.0), i0, i1,
                                             nullptr, nullptr, s_rp, s_col, s_val);
     } else {
       T* xin = (j % 2 == 0) ? b1 : b0;
       T* xz = (j % 2 == 0) ? b0 : b1;
       const T sc = (j == d.m) ? T(1.0 / double(d.m + 1)) : T(1);
-      csr_rows<T, G, NV, kCheb, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, g1, g2,
+      csr_rows<T, G, NV, kCheb, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, d.tail, g1, g2,
                                        sc, i0, i1, nullptr, nullptr, s_rp, s_col, s_val);
     }
     __syncthreads();  // the LDS stages are reused by the next task

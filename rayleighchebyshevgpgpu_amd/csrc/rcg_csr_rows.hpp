@@ -129,13 +129,15 @@ constexpr int kStageNz = 1024;
 // its gathers, so that computing a row costs a single trip to memory.
 // CX: complex operator and block (interleaved re, im): each 16-byte vector
 // holds W/2 complex entries and the gather multiplies complex values.
+// tail (1..W): entries of the last vector (cv = ncv - 1) that are columns of
+// the call; the rest of that vector is read but never written.
 // Rows [i0, i1) of one step (the body shared by the single-operator and the
 // batched kernels); s_rp / s_col / s_val are the workgroup's LDS stages.
 template <typename T, int G, int NV, int MODE, int RPI, bool CX>
 __device__ __forceinline__ void csr_rows(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
-    T p0, T p1, T p2, int64_t i0, int64_t i1, const T* __restrict__ R,
+    int tail, T p0, T p1, T p2, int64_t i0, int64_t i1, const T* __restrict__ R,
     const T* __restrict__ kap, int* s_rp, int* s_col, T* s_val) {
   using VT = V16<T>;
   constexpr int W = VT::W;
@@ -296,7 +298,15 @@ __device__ __forceinline__ void csr_rows(
           for (int w = 0; w < W; ++w)
             o[w] = p2 * fma(p1, fma(p0, acc[q][k][w], xs[q][k][w]), -zs[q][k][w]);
         }
-        VT::st_stream(out + rowoff + cv * W, o);
+        if (tail < W && cv == ncv - 1) {
+          // the last vector holds columns past the caller's count: only the
+          // first `tail` entries are stored
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            if (w < tail) out[rowoff + cv * W + w] = o[w];
+        } else {
+          VT::st_stream(out + rowoff + cv * W, o);
+        }
       }
     }
   }

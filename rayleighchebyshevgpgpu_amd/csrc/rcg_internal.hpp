@@ -71,6 +71,7 @@ struct BatchDesc {
   int64_t n;
   int ld;           // real elements per row
   int ncv;          // 16-byte column vectors per row
+  int tail;         // columns of the call in the last vector (1..elements per vector)
   int m;            // filter degree
   int rows_per_blk;
   double g1, g2;

@@ -27,7 +27,7 @@ template <typename T, int G, int NV, int MODE, int RPI, bool CX>
 __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
-    T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk, const int* __restrict__ order,
+    int tail, T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk, const int* __restrict__ order,
     int remap, const T* __restrict__ R, const T* __restrict__ kap) {
   __shared__ int s_rp[kMaxRpb + 1];
   __shared__ int s_col[kStageNz];
@@ -37,8 +37,8 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
   const int64_t lb = order ? (int64_t)order[lq] : lq;
   const int64_t i0 = lb * rows_per_blk;
   const int64_t i1 = min(n, i0 + rows_per_blk);
-  csr_rows<T, G, NV, MODE, RPI, CX>(n, rowptr, col, val, X, Z, out, ld, ncv, p0, p1, p2, i0, i1, R,
-                                    kap, s_rp, s_col, s_val);
+  csr_rows<T, G, NV, MODE, RPI, CX>(n, rowptr, col, val, X, Z, out, ld, ncv, tail, p0, p1, p2, i0,
+                                    i1, R, kap, s_rp, s_col, s_val);
 }
 
 #ifndef RCG_CSR_RPI
@@ -48,8 +48,8 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
 
 template <typename T, int MODE, bool CX, int G, int NV>
 void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
-                  T* out, int ld, int ncv, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
-                  const T* R, const T* kap) {
+                  T* out, int ld, int ncv, int tail, T p0, T p1, T p2, hipStream_t st,
+                  const CsrSched* sc, const T* R, const T* kap) {
   constexpr int RS = kThreads / G;
   int64_t rows_per_blk, nblk;
   const int* order = nullptr;
@@ -71,7 +71,7 @@ void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T*
   const int remap = sc ? sc->remap : 1;
   const int64_t grid = round_up(nblk, kXcds);
   hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE, RCG_CSR_RPI, CX>), dim3((unsigned)grid), dim3(kThreads), 0,
-                     st, n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, rows_per_blk, nblk, order,
+                     st, n, rp, ci, va, X, Z, out, ld, ncv, tail, p0, p1, p2, rows_per_blk, nblk, order,
                      remap, R, kap);
   RCG_HIP(hipGetLastError());
 }
@@ -82,9 +82,10 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
                    const T* R, const T* kap) {
   constexpr int W = V16<T>::W;
   const int ncv = (ncols + W - 1) / W;
+  const int tail = ncols - (ncv - 1) * W;
   if (ld % 4 != 0) throw Error(1, "block leading dimension must be a multiple of 4");
 #define RCG_CSR_CASE(LIM, GG, NN) \
-  if (ncv <= LIM) return csr_launch_g<T, MODE, CX, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, p0, p1, p2, st, sc, R, kap);
+  if (ncv <= LIM) return csr_launch_g<T, MODE, CX, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, tail, p0, p1, p2, st, sc, R, kap);
   // (16-byte vectors per row, lanes per row G, vectors per lane NV).  From 8
   // vectors on, each lane carries two or more vectors: half the lanes per row
   // of one vector each, twice the rows per wave (FP64 step at 128^3, b = 64:

@@ -653,9 +653,12 @@ class Engine {
   // Complex operators (dtype C128): every block column is a complex column
   // stored as two interleaved real columns; ld and the attached buffers are
   // in real elements, and widths passed to the methods count complex columns.
-  Engine(rcg_context_s* ctx, rcg_operator_s* op, int64_t n) : ctx_(ctx), op_(op), n_(n) {
+  // cx_blocks: complex blocks with no operator (orthonormalisation, deflation,
+  // rotation of C128 blocks); with an operator its dtype decides.
+  Engine(rcg_context_s* ctx, rcg_operator_s* op, int64_t n, bool cx_blocks = false)
+      : ctx_(ctx), op_(op), n_(n) {
     st_ = ctx->stream;
-    cx_ = op && op->dtype == RCG_C128;
+    cx_ = op ? op->dtype == RCG_C128 : cx_blocks;
     cw_ = cx_ ? 2 : 1;
     coef_.use_stream(st_);
     hcoef_.use_stream(st_);
@@ -815,8 +818,13 @@ class Engine {
       if (v > 0) t.panel = v;
     }
     if (const char* env = std::getenv("RCG_FILTER_SCHED")) {
+      int pick = -1;
       for (int sidx = 0; sidx < (int)op_->scheds.size(); ++sidx)
-        if (op_->scheds[sidx]->name == env) t.sched = sidx;
+        if (op_->scheds[sidx]->name == env) pick = sidx;
+      if (pick < 0)
+        throw Error(RCG_ERR_INVALID, std::string("RCG_FILTER_SCHED names no schedule of this "
+                                                 "operator: ") + env);
+      t.sched = pick;
     }
     return t;
   }
@@ -921,6 +929,7 @@ class Engine {
     d.n = n_;
     d.ld = (int)ld_;
     d.ncv = (int)((rb + w - 1) / w);
+    d.tail = (int)(rb - (int64_t)(d.ncv - 1) * w);
     d.m = (int)m;
     // 64-row tasks: the served stream's operators (n ~ 1e4) spread over twice
     // the workgroups of 128-row tasks (served 1186 -> 1236-1258 eigenpairs/s;
@@ -2097,6 +2106,7 @@ struct Ticket {
   rcg_operator_s* op = nullptr;
   rcg_solve_params p{};
   int64_t footprint = 0;
+  int64_t resident = 0;  // bytes of evecs, held against the budget until release
   int state = 0;  // 0 queued, 1 running, 2 done
   double t_submit = 0, t_admit = 0, t_done = 0;
   rcg_solve_result res{};
@@ -2124,7 +2134,10 @@ int64_t solve_footprint(const rcg_operator_s* op, const rcg_solve_params& p) {
 
 struct rcg_server_s {
   rcg_context_s* ctx = nullptr;
+  // used: footprints of the running solves plus the resident eigenvectors of
+  // finished, unreleased tickets
   int64_t budget = 0, used = 0, peak = 0;
+  int running = 0;
   int max_active = 1;
   std::mutex mu;
   std::condition_variable cv;
@@ -2150,13 +2163,16 @@ void server_worker(rcg_server_s* S, int w) {
     {
       std::unique_lock<std::mutex> g(S->mu);
       S->cv.wait(g, [&] {
+        // a request that does not fit next to the others runs once no other
+        // solve is running (resident vectors alone cannot hold it back forever)
         return S->stop || (!S->queue.empty() && (S->used + S->queue.front()->footprint <= S->budget ||
-                                                 S->used == 0));
+                                                 S->running == 0));
       });
       if (S->stop) return;
       t = S->queue.front();
       S->queue.pop_front();
       S->used += t->footprint;
+      ++S->running;
       S->peak = std::max(S->peak, S->used);
       t->state = 1;
       t->t_admit = now_s();
@@ -2206,6 +2222,12 @@ void server_worker(rcg_server_s* S, int w) {
         t->start = nullptr;
       }
       S->used -= t->footprint;
+      --S->running;
+      if (t->evecs) {
+        t->resident = (int64_t)t->evecs->mem.bytes;
+        S->used += t->resident;
+        S->peak = std::max(S->peak, S->used);
+      }
       t->state = 2;
       t->t_done = now_s();
       ++S->completed;
@@ -2231,23 +2253,33 @@ void check_pair(rcg_operator op, rcg_block a, rcg_block b, int64_t cols, bool al
 // leading dimension of a block in real elements
 int64_t real_ld(const rcg_block_s* b) { return (b->dtype == RCG_C128 ? 2 : 1) * b->ld; }
 
-// Engine over caller blocks for the piecewise entry points.
+// Engine over caller blocks for the piecewise entry points (C128 blocks run
+// as their interleaved real columns).
 struct Piece {
   Engine eng;
   Piece(rcg_block V, rcg_block T, rcg_operator_s* op, int64_t kmax = 4)
-      : eng(V->ctx, op, V->n) {
-    eng.reserve(V->ld, std::max<int64_t>(kmax, 4));
+      : eng(V->ctx, op, V->n, V->dtype == RCG_C128) {
+    eng.reserve(real_ld(V), std::max<int64_t>(kmax, 4));
     eng.attach(static_cast<double*>(V->d()), T ? static_cast<double*>(T->d()) : nullptr, nullptr,
-               V->ld, nullptr, 0);
+               real_ld(V), nullptr, 0);
   }
   // make sure the result sits in the caller's V block
   void settle(rcg_block V, int64_t cols) {
     if (!eng.v_is_first())
-      rcg::launch_copy_cols<double>(V->n, eng.V(), (int)V->ld, 0, static_cast<double*>(V->d()),
-                                    (int)V->ld, 0, (int)cols, eng.stream());
+      rcg::launch_copy_cols<double>(V->n, eng.V(), (int)real_ld(V), 0, static_cast<double*>(V->d()),
+                                    (int)real_ld(V), 0, (int)(eng.cw() * cols), eng.stream());
     eng.sync();
   }
 };
+
+// X and tmp of one shape and dtype (FP64, or C128 for complex blocks).
+void check_block_pair(rcg_block X, rcg_block tmp, int64_t cols) {
+  need(X && tmp && X != tmp, "X and tmp must be distinct blocks");
+  need(X->n == tmp->n && X->ld == tmp->ld && X->ctx == tmp->ctx, "X and tmp differ in shape");
+  need(cols >= 1 && cols <= X->cols && cols <= tmp->cols, "bad column count");
+  need((X->dtype == RCG_F64 || X->dtype == RCG_C128) && tmp->dtype == X->dtype,
+       "FP64 or C128 blocks of one dtype expected");
+}
 
 }  // namespace
 
@@ -2542,7 +2574,7 @@ int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alph
 int rcg_cheb_filter(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols, int64_t degree,
                     int64_t repetition, double lambda_max, double shift) {
   return guarded([&] {
-    check_pair(op, X, tmp, cols);
+    check_pair(op, X, tmp, cols, true);
     need(X != tmp, "X and tmp must differ");
     need(degree >= 1 && repetition >= 1, "degree and repetition must be >= 1");
     DeviceGuard g(op->ctx->device);
@@ -2556,21 +2588,22 @@ int rcg_cheb_filter_mixed(rcg_operator op, rcg_block X, rcg_block tmp, int64_t c
                           int64_t degree, int64_t repetition, double lambda_max, double shift,
                           const double* theta, const int32_t* ritz) {
   return guarded([&] {
-    check_pair(op, X, tmp, cols);
+    check_pair(op, X, tmp, cols, true);
     need(X != tmp, "X and tmp must differ");
     need(degree >= 1 && repetition >= 1, "degree and repetition must be >= 1");
     need(!ritz || theta, "Ritz flags need Ritz values");
     DeviceGuard g(op->ctx->device);
     Piece pc(X, tmp, op);
     // tmp doubles as the FP64 A*X block of the correction start
+    const int64_t rld = real_ld(X);
     pc.eng.attach(static_cast<double*>(X->d()), static_cast<double*>(tmp->d()),
-                  static_cast<double*>(tmp->d()), X->ld, nullptr, 0);
-    const size_t fb = sizeof(float) * (size_t)X->n * (size_t)X->ld;
+                  static_cast<double*>(tmp->d()), rld, nullptr, 0);
+    const size_t fb = sizeof(float) * (size_t)X->n * (size_t)rld;
     DevBuf mb(pc.eng.stream());
     mb.ensure(3 * fb);
     RCG_HIP(hipMemsetAsync(mb.p, 0, 3 * fb, pc.eng.stream()));
     float* m0 = mb.as<float>();
-    pc.eng.attach_mixed(m0, m0 + (size_t)X->n * X->ld, m0 + 2 * (size_t)X->n * X->ld);
+    pc.eng.attach_mixed(m0, m0 + (size_t)X->n * rld, m0 + 2 * (size_t)X->n * rld);
     std::vector<double> th(cols, 0.0);
     std::vector<char> rz(cols, 0);
     for (int64_t c = 0; c < cols; ++c) {
@@ -2584,10 +2617,7 @@ int rcg_cheb_filter_mixed(rcg_operator op, rcg_block X, rcg_block tmp, int64_t c
 
 int rcg_orthonormalize(rcg_block X, rcg_block tmp, int64_t cols) {
   return guarded([&] {
-    need(X && tmp && X != tmp, "X and tmp must be distinct blocks");
-    need(X->n == tmp->n && X->ld == tmp->ld && X->ctx == tmp->ctx, "X and tmp differ in shape");
-    need(cols >= 1 && cols <= X->cols, "bad column count");
-    need(X->dtype == RCG_F64 && tmp->dtype == RCG_F64, "FP64 blocks expected");
+    check_block_pair(X, tmp, cols);
     DeviceGuard g(X->ctx->device);
     Piece pc(X, tmp, nullptr);
     pc.eng.orthonormalize(cols);
@@ -2599,12 +2629,14 @@ int rcg_deflate(rcg_block X, int64_t cols, rcg_block Y, int64_t found) {
   return guarded([&] {
     need(X && Y && X != Y, "bad arguments");
     need(X->n == Y->n && X->ctx == Y->ctx, "X and Y differ in rows or context");
-    need(X->dtype == RCG_F64 && Y->dtype == RCG_F64, "FP64 blocks expected");
+    need((X->dtype == RCG_F64 || X->dtype == RCG_C128) && Y->dtype == X->dtype,
+         "FP64 or C128 blocks of one dtype expected");
     need(cols >= 1 && cols <= X->cols && found >= 0 && found <= Y->cols, "bad column counts");
     DeviceGuard g(X->ctx->device);
-    Engine e(X->ctx, nullptr, X->n);
-    e.reserve(std::max(X->ld, Y->ld), std::max(X->ld, Y->ld));
-    e.attach(static_cast<double*>(X->d()), nullptr, nullptr, X->ld, static_cast<double*>(Y->d()), Y->ld);
+    Engine e(X->ctx, nullptr, X->n, X->dtype == RCG_C128);
+    e.reserve(std::max(real_ld(X), real_ld(Y)), std::max(real_ld(X), real_ld(Y)));
+    e.attach(static_cast<double*>(X->d()), nullptr, nullptr, real_ld(X),
+             static_cast<double*>(Y->d()), real_ld(Y));
     e.deflate(cols, found);
     e.normalize(cols);
     e.sync();
@@ -2613,26 +2645,33 @@ int rcg_deflate(rcg_block X, int64_t cols, rcg_block Y, int64_t found) {
 
 int rcg_project(rcg_operator op, rcg_block V, rcg_block W, int64_t cols, double* H_host) {
   return guarded([&] {
-    check_pair(op, V, W, cols);
+    check_pair(op, V, W, cols, true);
     need(V != W && H_host && cols >= 1, "bad arguments");
     DeviceGuard g(op->ctx->device);
     Engine e(op->ctx, op, op->n);
-    e.reserve(V->ld, 4);
-    e.attach(static_cast<double*>(V->d()), nullptr, static_cast<double*>(W->d()), V->ld, nullptr, 0);
-    std::vector<double> H;
-    e.project(cols, H);
-    std::memcpy(H_host, H.data(), sizeof(double) * cols * cols);
+    e.reserve(real_ld(V), 4);
+    e.attach(static_cast<double*>(V->d()), nullptr, static_cast<double*>(W->d()), real_ld(V),
+             nullptr, 0);
+    if (e.cx()) {
+      std::vector<cplx> H;
+      e.project(cols, H);
+      std::memcpy(H_host, H.data(), sizeof(cplx) * cols * cols);
+    } else {
+      std::vector<double> H;
+      e.project(cols, H);
+      std::memcpy(H_host, H.data(), sizeof(double) * cols * cols);
+    }
   });
 }
 
 int rcg_rotate(rcg_block V, rcg_block tmp, int64_t cols, const double* Y_host) {
   return guarded([&] {
-    need(V && tmp && V != tmp && Y_host, "bad arguments");
-    need(V->n == tmp->n && V->ld == tmp->ld && V->ctx == tmp->ctx, "shape mismatch");
-    need(cols >= 1 && cols <= V->cols, "bad column count");
+    need(Y_host != nullptr, "bad arguments");
+    check_block_pair(V, tmp, cols);
     DeviceGuard g(V->ctx->device);
     Piece pc(V, tmp, nullptr);
-    pc.eng.right_multiply(cols, Y_host);
+    if (pc.eng.cx()) pc.eng.right_multiply(cols, reinterpret_cast<const cplx*>(Y_host));
+    else pc.eng.right_multiply(cols, Y_host);
     pc.eng.normalize(cols);
     pc.settle(V, cols);
   });
@@ -2856,7 +2895,9 @@ int rcg_server_release(rcg_server s, int64_t ticket) {
       need(it->second->pins == 0, "the ticket's vectors are the start of a queued or running solve");
       t = std::move(it->second);
       s->tickets.erase(it);
+      s->used -= t->resident;
     }
+    s->cv.notify_all();  // the freed bytes may admit the queue head
     if (t->evecs) delete t->evecs;
   });
 }
