@@ -7,6 +7,8 @@
 namespace rcg {
 namespace {
 
+static_assert(kBatchMaxRows <= kMaxRpb, "batched row blocks must fit the LDS row-pointer stage");
+
 // ---------------------------------------------------- batched filter step
 // Step j of the Chebyshev filters of many operators at once: task t is row
 // block tasks[t].blk of request tasks[t].req.  Requests hold their own CSR

@@ -76,6 +76,9 @@ struct BatchDesc {
   int rows_per_blk;
   double g1, g2;
 };
+// Largest row block of a batched task (the kernel stages its row pointers
+// in LDS).
+constexpr int kBatchMaxRows = 512;
 struct BatchTask {
   int req;
   int blk;

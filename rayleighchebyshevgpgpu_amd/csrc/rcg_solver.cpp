@@ -442,7 +442,12 @@ class FilterBatcher {
     pending_.push_back(r);
     cv_.notify_all();
     cv_done_.wait(g, [&] { return r->enqueued || !error_.empty(); });
-    if (!r->enqueued) throw Error(RCG_ERR_HIP, error_);
+    if (!r->enqueued) {
+      // r lives in the caller's frame, which the throw ends: it must not
+      // stay queued (the loop never runs a batch once error_ is set)
+      pending_.erase(std::remove(pending_.begin(), pending_.end(), r), pending_.end());
+      throw Error(RCG_ERR_HIP, error_);
+    }
   }
   int64_t batches() const { return batches_; }
   int64_t requests() const { return requests_; }
@@ -487,6 +492,13 @@ class FilterBatcher {
         // or the window closes
         const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(window_us_);
         cv_.wait_until(g, until, [&] { return stop_ || (int)pending_.size() >= members_; });
+      }
+      if (!error_.empty()) {
+        // a failed launch leaves the stream's state unknown: no further
+        // batches; the waiting solves see error_ and throw
+        pending_.clear();
+        cv_done_.notify_all();
+        continue;
       }
       std::vector<FilterReq*> batch;
       batch.swap(pending_);
@@ -537,6 +549,18 @@ class FilterBatcher {
     for (size_t q = 0; q < batch.size(); ++q) {
       first_task[q] = (int)tasks.size();
       const rcg::BatchDesc& d = batch[q]->desc;
+      // what the batched kernel assumes of each request: row blocks within
+      // the LDS row-pointer stage, the launch class of its width, a degree
+      // and start step the sweep covers, both recurrence buffers
+      const FilterReq* r = batch[q];
+      if (d.n <= 0 || d.rows_per_blk <= 0 || d.rows_per_blk > rcg::kBatchMaxRows ||
+          d.ncv <= 0 || d.tail < 1 || d.ld < 4 * d.ncv / (r->f32 ? 1 : 2) || d.m < 1 ||
+          r->jstart < 1 || (r->f32 && r->jstart < 2) || r->cls != rcg::csr_class(d.ncv) ||
+          !d.rowptr || !d.col || !d.val || !d.buf[0] || !d.buf[1] ||
+          (r->src && (!d.R || !d.kap)))
+        throw Error(RCG_ERR_INVALID, "malformed batched filter request");
+      if ((int64_t)tasks.size() + (d.n + d.rows_per_blk - 1) / d.rows_per_blk > INT32_MAX)
+        throw Error(RCG_ERR_INVALID, "too many row blocks in one filter batch");
       const int nb = (int)((d.n + d.rows_per_blk - 1) / d.rows_per_blk);
       for (int b = 0; b < nb; ++b) tasks.push_back({(int)q, b});
     }

@@ -97,17 +97,59 @@ def test_hbm_budget_respected(rcg):
     # a budget of 6 MB admits a few of these at a time
     srv = serving.Server(max_active=8, hbm_budget=6 << 20, window_us=200)
     ops_, tickets = submit_all(srv, rcg, reqs, "mixed")
+    outs = []
     for t in tickets:
         out = srv.wait(t)
         assert out["status"] == 0, out["error"]
         assert out["t_submit"] <= out["t_admit"] <= out["t_done"]
+        outs.append(out)
     st = srv.stats()
-    print(f"server stats: {st}")
+    print(f"server stats, all done, none released: {st}")
+    # finished, unreleased tickets keep their eigenvectors on the device and
+    # count against the budget: at least n x ld(found) FP64 each
+    resident = sum(r.n * serving._block_ld(o["found"]) * 8 for r, o in zip(reqs, outs))
+    assert st["hbm_in_use"] >= resident and st["queued"] == 0
     fps = [serving.solve_footprint(r.n, r.k, r.k, max(4, r.k // 2)) for r in reqs]
-    # a request larger than the whole budget is admitted only when nothing else runs
-    assert st["hbm_peak"] <= max(6 << 20, max(fps))
+    # a request larger than the whole budget is admitted only when no other
+    # solve runs, next to at most the resident vectors of finished tickets
+    assert st["hbm_peak"] <= max(6 << 20, max(fps) + st["hbm_in_use"])
     assert st["hbm_peak"] >= max(fps)
+    for t in tickets:
+        srv.release(t)
+    st = srv.stats()
     assert st["hbm_in_use"] == 0 and st["queued"] == 0
+    srv.close()
+
+
+def test_resident_vectors_hold_back_admission(rcg):
+    """Unreleased results count against the budget until their release; a
+    request that does not fit next to them is still admitted once no other
+    solve runs, so resident vectors cannot hold the queue back for good."""
+    print("case: resident vectors of unreleased tickets against a small budget", flush=True)
+    A = ops.laplacian_2d(40)
+    k = 8
+    fp = serving.solve_footprint(A.n, k, k, 4)
+    op = rcg.CsrOperator(A)
+    # room for one running solve plus a little: two unreleased results and a
+    # running solve do not fit together
+    budget = fp + 4096
+    srv = serving.Server(max_active=4, hbm_budget=budget, window_us=0)
+    t1 = srv.submit(op, k, 1e-10, k, 4, precision="fp64")
+    r1 = srv.wait(t1)
+    assert r1["status"] == 0, r1["error"]
+    held = srv.stats()["hbm_in_use"]
+    assert held >= A.n * serving._block_ld(k) * 8
+    # nothing runs, so the next request is admitted next to the resident block
+    t2 = srv.submit(op, k, 1e-10, k, 4, precision="fp64")
+    r2 = srv.wait(t2)
+    assert r2["status"] == 0, r2["error"]
+    st = srv.stats()
+    assert st["hbm_peak"] <= fp + 2 * held
+    assert st["hbm_in_use"] >= 2 * A.n * serving._block_ld(k) * 8
+    srv.release(t1)
+    srv.release(t2)
+    assert srv.stats()["hbm_in_use"] == 0
+    assert np.array_equal(r1["eigenvalues"], r2["eigenvalues"])
     srv.close()
 
 
