@@ -134,6 +134,33 @@ def cpu_baseline(A, gpu_counts, max_inner=1):
     }
 
 
+def cpu_model_check(grid=32):
+    """The cpu_baseline model against a whole timed oracle solve on a smaller
+    operator of the same family ({grid}^3 periodic Anderson, k 64, inc 32,
+    buf 32): the model's per-phase unit costs from a bounded sample (Lanczos +
+    one inner iteration per outer pass) times the whole solve's own counts,
+    next to the solve's measured wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    A = build_operator(grid)
+    t0 = time.time()
+    full = orc.solve(A, K, TOL, INC, BUF, seed=0, stop="RESIDUAL_ONLY")
+    wall_full = time.time() - t0
+    f = full["timing"]
+    t1 = time.time()
+    smp = orc.solve(A, K, TOL, INC, BUF, seed=0, stop="RESIDUAL_ONLY", max_inner=1)
+    wall_sample = time.time() - t1
+    t = smp["timing"]
+    est = (t["lanczos"] + t["filter"] / max(t["degree_sum"], 1) * f["degree_sum"]
+           + t["ortho"] / max(t["n_ortho"], 1) * f["n_ortho"]
+           + (t["deflate"] / max(t["n_deflate"], 1) + t["rr"] / max(t["n_rr"], 1)) * f["n_rr"])
+    return {"operator": f"{grid}^3 periodic Laplacian + Anderson U[-2,2] (seed 1), k={K} "
+                        f"inc={INC} buf={BUF}",
+            "measured_solve_s": round(wall_full, 2), "model_estimate_s": round(est, 2),
+            "model_over_measured": round(est / wall_full, 3), "sample_s": round(wall_sample, 2),
+            "found": int(full["found"]), "filter_degrees": int(f["degree_sum"])}
+
+
 def cpu_baseline_dense(A, gpu_counts, b):
     """The dense workload on the host cores: each filter degree is one n x n by
     n x b product, timed with numpy's BLAS on the full block (two products),
@@ -233,18 +260,20 @@ def serving_cpu_baseline(reqs, inc_buf, budget_s=15.0):
                        f"eigenpairs in {wall:.1f} s (operator construction included)")}
 
 
-def serving_bench(args):
+def serving_run(args, cpu=True, cpu_budget_s=15.0):
     """Batched serving: the request stream through rcg_server (mixed precision,
     continuous batching) against the plain FP64 path solving requests one by
-    one.  Operators are built and uploaded before the timed region."""
-    from rayleighchebyshevgpgpu_amd import capi, serving, solver
+    one.  Operators are uploaded before the timed region, one sparsity pattern
+    per grid shape with each request's own values.  Returns the measurements."""
+    from rayleighchebyshevgpgpu_amd import serving, solver
     reqs = serving.serving_requests(args.requests, rate=args.rate, seed=4)
     ctx = solver.Context(0)
     t0 = time.time()
     ops_ = []
+    cache = serving.OperatorCache(ctx)
     for i, r in enumerate(reqs):
-        ops_.append(solver.CsrOperator(r.operator(), ctx))
-        if i % 512 == 511:
+        ops_.append(cache.operator(r))
+        if i % 1024 == 1023:
             log(f"operators built: {i + 1}/{len(reqs)} ({time.time() - t0:.0f}s)")
     log(f"{len(reqs)} operators on the device in {time.time() - t0:.1f}s "
         f"(total n {sum(r.n for r in reqs)})")
@@ -259,7 +288,6 @@ def serving_bench(args):
         tickets = []
         ru0 = resource.getrusage(resource.RUSAGE_SELF)
         start = time.perf_counter()
-        start_steady = None
         for r, op in zip(reqs, ops_):
             if not saturate:
                 delay = r.arrival - (time.perf_counter() - start)
@@ -322,38 +350,23 @@ def serving_bench(args):
     ctx.synchronize()
     plain_s = time.perf_counter() - tp
     plain = found_plain / plain_s
+    plain_rec = {"requests": m, "eigenpairs": found_plain, "seconds": round(plain_s, 3),
+                 "value": round(plain, 2)}
     log(f"plain FP64: {m} requests in {plain_s:.3f} s, {plain:.1f} eigenpairs/s")
     if args.plain_only:
-        print(json.dumps({"plain_fp64": {"requests": m, "eigenpairs": found_plain,
-                                         "seconds": round(plain_s, 3), "value": round(plain, 2)}}),
-              flush=True)
-        return
+        return {"plain_fp64": plain_rec}
     best = saturated or offered
     sf = best["shared_filter"]
-    cpu = None if args.no_cpu else serving_cpu_baseline(reqs, inc_buf)
-    line = {
-        "metric": METRIC,
-        "value": best["value"],
-        "unit": "eigenpairs/s",
-        "n_gpus": 1,
-        "steps": 1,
-        "warmup": 0,
+    return {
+        "value": best["value"], "unit": "eigenpairs/s",
         "ms_per_step": round(best["makespan_s"] * 1e3, 1),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 filter steps (correction form), f64 elsewhere",
-        "data": "synthetic",
-        "config": {"workload": (f"batched serving: {len(reqs)} requests, Poisson arrivals at "
-                                f"{args.rate:g}/s (seed 4), 2-D/3-D Dirichlet Laplacians n log-"
-                                "uniform in [1e3, 2.56e5] + U[-W/2,W/2], W in [0,8], k in "
-                                "{8,16,32}, tol 1e-10, mixed precision"),
-                   "slots": args.slots, "window_us": args.window_us,
-                   "hbm_budget_gb": args.hbm_budget},
+        "workload": (f"batched serving: {len(reqs)} requests, Poisson arrivals at {args.rate:g}/s "
+                     "(seed 4), 2-D/3-D Dirichlet Laplacians n log-uniform in [1e3, 2.56e5] + "
+                     "U[-W/2,W/2], W in [0,8], k in {8,16,32}, tol 1e-10, mixed precision"),
+        "slots": args.slots, "window_us": args.window_us, "hbm_budget_gb": args.hbm_budget,
         "p50_ms": offered["p50_ms"], "p99_ms": offered["p99_ms"],
         "offered_load": offered, "saturated": saturated,
-        "plain_fp64": {"requests": m, "eigenpairs": found_plain, "seconds": round(plain_s, 3),
-                       "value": round(plain, 2)},
+        "plain_fp64": plain_rec,
         "speedup_vs_plain_fp64": round(best["value"] / plain, 2) if plain > 0 else None,
         # the shared filter launches: algorithmic bytes over their device time
         "roofline": {"bound": "hbm", "achieved": sf["alg_GBps"], "peak": HBM_PEAK_GBS,
@@ -361,8 +374,34 @@ def serving_bench(args):
                      "traffic": None,
                      "kernel": "csr_batch_kernel<float,...,SRC> (batched FP32 correction steps)",
                      "launches": sf["launches"], "device_s": sf["device_s"]},
-        "cpu_baseline": cpu,
+        "cpu_baseline": (serving_cpu_baseline(reqs, inc_buf, cpu_budget_s) if cpu else None),
     }
+
+
+def serving_bench(args):
+    """--workload serving: the serving measurements as the bench line."""
+    r = serving_run(args, cpu=not args.no_cpu)
+    if args.plain_only:
+        print(json.dumps(r), flush=True)
+        return
+    line = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 filter steps (correction form), f64 elsewhere",
+        "data": "synthetic",
+        "config": {"workload": r["workload"], "slots": r["slots"], "window_us": r["window_us"],
+                   "hbm_budget_gb": r["hbm_budget_gb"]},
+    }
+    line.update({k: r[k] for k in ("p50_ms", "p99_ms", "offered_load", "saturated", "plain_fp64",
+                                   "speedup_vs_plain_fp64", "roofline", "cpu_baseline")})
     print(json.dumps(line), flush=True)
 
 
@@ -388,6 +427,8 @@ def main():
     ap.add_argument("--plain-requests", type=int, default=256,
                     help="serving: requests solved one by one on the plain FP64 path")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solves")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="anderson128: skip the serving workload reported under 'serving'")
     ap.add_argument("--kernel-only", type=int, default=0, metavar="M",
                     help="profiling mode: run M Chebyshev steps of the filter on the 64-column "
                          "block and one 2 GiB device copy (the HBM calibration kernel), no solve")
@@ -587,6 +628,19 @@ def main():
             line["cpu_baseline"] = cpu_baseline_dense(Ad, counts, INC + BUF)
         else:
             line["cpu_baseline"] = cpu_baseline(A, counts, args.cpu_iters)
+            if args.workload == "anderson128":
+                log("cpu baseline model check on 32^3 ...")
+                chk = cpu_model_check(32)
+                line["cpu_baseline"]["model_check"] = chk
+                line["cpu_baseline"]["sample"] += (
+                    f"; the same model on a whole {chk['operator']} oracle solve estimates "
+                    f"{chk['model_estimate_s']} s against {chk['measured_solve_s']} s measured "
+                    f"(x{chk['model_over_measured']})")
+    if args.workload == "anderson128" and not args.no_serving:
+        # the batched serving workload beside the single-operator solve: the
+        # mixed-precision served stream against the plain FP64 path
+        log("serving workload ...")
+        line["serving"] = serving_run(args, cpu=not args.no_cpu, cpu_budget_s=10.0)
     print(json.dumps(line), flush=True)
 
 

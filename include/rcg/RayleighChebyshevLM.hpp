@@ -180,6 +180,77 @@ class CsrOperator {
   long n_ = 0;
 };
 
+// The device block a caller's operator sees inside its apply: the
+// AmatrixClassLMCuda accessors such an apply uses (getDataPointer_d,
+// getRowSize, getColSize, getDimension; AmatrixClassLMCuda.h:130-349) over a
+// column-major m x n device block with leading dimension getLeadingDimension()
+// (= m), and the HIP stream (as void*) its work must go to.  A view of a
+// library buffer: valid during the apply only.
+template <typename T = double>
+class DeviceMatrix {
+ public:
+  DeviceMatrix(void* data, long m, long n, long ld, void* stream)
+      : d_(static_cast<T*>(data)), m_(m), n_(n), ld_(ld), stream_(stream) {}
+  T* getDataPointer_d() const { return d_; }
+  long getRowSize() const { return m_; }
+  long getColSize() const { return n_; }
+  long getLeadingDimension() const { return ld_; }
+  size_t getDimension() const { return (size_t)m_ * (size_t)n_; }
+  void* stream() const { return stream_; }
+
+ private:
+  T* d_;
+  long m_, n_, ld_;
+  void* stream_;
+};
+
+// A caller's operator as code, the reference's contract
+// Otype::apply(Atype& in, Atype& out, alpha, beta): out = alpha A in + beta out
+// (LanczosCpolyOperatorLMCuda.h:236,245, RayleighChebyshevLMCuda.h:2135,2235).
+// UserOp provides apply(DeviceMatrix<T>& in, DeviceMatrix<T>& out, double
+// alpha, double beta) that enqueues its work on in.stream() (or finishes it
+// before returning); T is double or std::complex<double> (Hermitian).  An
+// exception from it fails the solve with std::runtime_error.  The user
+// operator must outlive this adapter.
+template <class UserOp, typename T = double>
+class CallbackOperator {
+ public:
+  CallbackOperator(Context& ctx, long n, UserOp& op) : ctx_(&ctx), op_(&op), n_(n) {
+    static_assert(std::is_same<T, double>::value || std::is_same<T, std::complex<double>>::value,
+                  "a caller's operator is real or complex double");
+    check(rcg_operator_callback_create(ctx.handle(), n,
+                                       std::is_same<T, double>::value ? RCG_F64 : RCG_C128,
+                                       &CallbackOperator::trampoline, this, &h_));
+  }
+  ~CallbackOperator() { rcg_operator_destroy(h_); }
+  CallbackOperator(const CallbackOperator&) = delete;
+  CallbackOperator& operator=(const CallbackOperator&) = delete;
+  template <class Atype>
+  void apply(Atype& in, Atype& out, double alpha, double beta) {
+    check(rcg_operator_apply(h_, in.handle(), out.handle(), alpha, beta, in.getColSize()));
+  }
+  rcg_operator handle() const { return h_; }
+  long getDimension() const { return n_; }
+  Context& context() const { return *ctx_; }
+
+ private:
+  static int trampoline(void* user, const void* in, void* out, int64_t n, int64_t cols, int64_t ld,
+                        double alpha, double beta, void* stream) {
+    try {
+      DeviceMatrix<T> a(const_cast<void*>(in), (long)n, (long)cols, (long)ld, stream);
+      DeviceMatrix<T> b(out, (long)n, (long)cols, (long)ld, stream);
+      static_cast<CallbackOperator*>(user)->op_->apply(a, b, alpha, beta);
+      return 0;
+    } catch (...) {
+      return 1;
+    }
+  }
+  Context* ctx_;
+  UserOp* op_;
+  rcg_operator h_ = nullptr;
+  long n_ = 0;
+};
+
 // A dense real symmetric operator (column-major n x n, leading dimension lda).
 class DenseOperator {
  public:

@@ -116,12 +116,38 @@ int rcg_block_info(rcg_block b, int64_t* n, int64_t* cols, int64_t* ld, int* dty
 int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32_t* rowptr,
                             const int32_t* col, const void* val, int val_dtype, int mem,
                             rcg_operator* out);
+/* No counterpart in the reference: an operator with the sparsity pattern of
+ * the CSR operator `base` (its row pointers and columns, device-resident,
+ * shared, not copied) and values of its own (nnz values of base's dtype,
+ * copied from the host or referenced on the device), e.g. the members of a
+ * parameter sweep or of a served stream over one grid.  base must outlive the
+ * new operator.  The grid hint is inherited. */
+int rcg_operator_csr_share(rcg_operator base, const void* val, int mem, rcg_operator* out);
 /* The user operator as a dense real symmetric n x n matrix (column-major
  * with leading dimension lda; symmetric, so also row-major).  Copied to the
  * device (RCG_MEM_HOST) or referenced (RCG_MEM_DEVICE).  Its filter steps are
  * MFMA block products (FP64, or FP32 for the mixed-precision filter). */
 int rcg_operator_dense_create(rcg_context ctx, int64_t n, const double* A, int64_t lda, int mem,
                               rcg_operator* out);
+/* The reference's user operator as code: its contract is
+ * Otype::apply(Atype& in, Atype& out, alpha, beta), out = alpha*A*in +
+ * beta*out (LanczosCpolyOperatorLMCuda.h:236,245, RayleighChebyshevLMCuda.h
+ * :2135,2235).  apply(user, in, out, n, cols, ld, alpha, beta, hip_stream)
+ * gets n x cols column-major device blocks (the AmatrixClassLMCuda layout,
+ * leading dimension ld; FP64, or interleaved (re, im) pairs for RCG_C128)
+ * and must enqueue its work on hip_stream (the context's stream) or finish it
+ * before returning; it returns 0, anything else fails the calling entry point
+ * with RCG_ERR_INVALID.  in and out are library buffers valid for the call
+ * only; apply must not keep them or call back into this library with them.
+ * The operator must be Hermitian.  Its filter runs unfused: per degree one
+ * copy of the block into the column-major layout, the caller's product and
+ * one pass that forms the Chebyshev step from it.  Mixed-precision filtering
+ * (it needs FP32 operator values) and the server take CSR and dense
+ * operators only (RCG_ERR_UNSUPPORTED). */
+typedef int (*rcg_apply_fn)(void* user, const void* in, void* out, int64_t n, int64_t cols,
+                            int64_t ld, double alpha, double beta, void* hip_stream);
+int rcg_operator_callback_create(rcg_context ctx, int64_t n, int dtype, rcg_apply_fn apply,
+                                 void* user, rcg_operator* out);
 /* Operator ingest.  The reference's
  * CSR index width is the includer's RAY_INT (RayleighChebyshevLMCuda.h:409ff):
  * this takes 64-bit row pointers and columns and narrows them to the int32
@@ -243,7 +269,14 @@ typedef struct rcg_solve_result {
   int64_t trace_len;
   uint64_t rng_counter;
   int64_t filter_degree_sum; /* Chebyshev step launches (sum of starDegree * rep) */
-  double t_total, t_lanczos, t_filter, t_ortho, t_rr; /* host wall seconds per phase */
+  double t_total;            /* host wall seconds of the solve */
+  double t_lanczos, t_filter, t_ortho, t_rr; /* seconds of the solve's stream per phase,
+                                from hipEvents recorded where each phase begins (so a
+                                phase includes the stream's idle time while the host
+                                works inside it); t_ortho: start vectors, CholQR and
+                                deflation; t_rr: the projection with its CholQR passes,
+                                the small eigensolve, rotation, residuals and the final
+                                refinement */
   double t_filter_device;    /* seconds between hipEvents around the Chebyshev step
                                 launches (FP64 steps, or the FP32 correction steps) */
   double filter_bytes;       /* algorithmic bytes of those steps: CSR + 3 block passes

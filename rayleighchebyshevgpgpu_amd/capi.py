@@ -42,6 +42,12 @@ class SolveResult(C.Structure):
     ]
 
 
+# int apply(void* user, const void* in, void* out, int64_t n, int64_t cols,
+#           int64_t ld, double alpha, double beta, void* hip_stream)
+APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
+                       C.c_int64, C.c_double, C.c_double, C.c_void_p)
+
+
 class RcgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"rcg status {code}: {msg}")
@@ -84,6 +90,8 @@ def lib() -> C.CDLL:
         "rcg_matrix_market_csr": ([C.c_char_p, vp, vp, vp], i32),
         "rcg_operator_from_matrix_market": ([vp, C.c_char_p, P(vp)], i32),
         "rcg_operator_set_grid": ([vp, i64, i64, i64], i32),
+        "rcg_operator_csr_share": ([vp, vp, i32, P(vp)], i32),
+        "rcg_operator_callback_create": ([vp, i64, i32, APPLY_FN, vp, P(vp)], i32),
         "rcg_operator_apply": ([vp, vp, vp, d, d, i64], i32),
         "rcg_cheb_filter": ([vp, vp, vp, i64, i64, i64, d, d], i32),
         "rcg_cheb_filter_mixed": ([vp, vp, vp, i64, i64, i64, d, d, vp, vp], i32),

@@ -121,6 +121,11 @@ enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3, kChebSrc =
 // Rows per workgroup block (cap) and nonzeros staged in LDS per block.
 constexpr int kMaxRpb = 512;
 constexpr int kStageNz = 1024;
+// Nonzeros of a row whose gathers are issued together before their FMAs.
+#ifndef RCG_CSR_CHUNK
+#define RCG_CSR_CHUNK 4
+#endif
+constexpr int kChunk = RCG_CSR_CHUNK;
 
 // G lanes own one row; each lane handles NV 16-byte column vectors at
 // cv = g + k*G.  256/G rows are in flight per workgroup.  The block's row
@@ -202,15 +207,15 @@ __device__ __forceinline__ void csr_rows(
     int maxlen = 0;
 #pragma unroll
     for (int q = 0; q < RPI; ++q) maxlen = max(maxlen, pe[q] - pb[q]);
-    // chunks of 4 nonzeros per row: every gather of a chunk (over the RPI
-    // rows) is issued before its FMAs
-    for (int c = 0; c < maxlen; c += 4) {
-      int jj[RPI][4];
-      T aa[RPI][4], ai[RPI][4];
+    // chunks of kChunk nonzeros per row: every gather of a chunk (over the
+    // RPI rows) is issued before its FMAs
+    for (int c = 0; c < maxlen; c += kChunk) {
+      int jj[RPI][kChunk];
+      T aa[RPI][kChunk], ai[RPI][kChunk];
 #pragma unroll
       for (int q = 0; q < RPI; ++q)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < kChunk; ++e) {
           const int p = pb[q] + c + e;
           const bool live = p < pe[q];
           if constexpr (CX) {
@@ -233,11 +238,11 @@ __device__ __forceinline__ void csr_rows(
             }
           }
         }
-      T u[RPI][4][NV][W];
+      T u[RPI][kChunk][NV][W];
 #pragma unroll
       for (int q = 0; q < RPI; ++q)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < kChunk; ++e) {
           if (pb[q] + c + e < pe[q]) {
             const T* xr = X + (int64_t)jj[q][e] * ld;
 #pragma unroll
@@ -250,7 +255,7 @@ __device__ __forceinline__ void csr_rows(
 #pragma unroll
       for (int q = 0; q < RPI; ++q)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < kChunk; ++e) {
           if (pb[q] + c + e < pe[q]) {
 #pragma unroll
             for (int k = 0; k < NV; ++k) {

@@ -177,6 +177,21 @@ void launch_resid_norm_sumsq(int64_t n, const T* V, const T* W, int ld, int nc,
                              const double* theta, double* sums, double* ws, size_t ws_elems,
                              hipStream_t st);
 
+// Caller operators (column-major blocks) around the row-interleaved solver
+// blocks: `cols` entries per row (cw = 2: complex entries of two doubles),
+// the rows layout with ldr_real doubles per row, the column-major one with
+// ldc entries per column.  to_cols: rows -> column-major, else the reverse.
+void launch_layout(int64_t n, int cols, int cw, double* rows, int64_t ldr_real, double* colmaj,
+                   int64_t ldc, bool to_cols, hipStream_t st);
+// out = p2 (p1 (p0 AX + X) - Z) (rows layout; AX column-major, Z optional
+// and may alias out): a Chebyshev step around a caller operator's product.
+void launch_cheb_combine(int64_t n, int cols, int cw, const double* AX, int64_t ldc,
+                         const double* X, const double* Z, double* out, int64_t ldr_real,
+                         double p0, double p1, double p2, hipStream_t st);
+// alpha_k = Re sum conj(w) v, alpha_im_k = Im of it (interleaved complex n-vectors)
+void launch_lz_dot_c(int64_t n, const double* w, const double* v, double* alpha_k,
+                     double* alpha_im_k, double* ws, size_t ws_elems, hipStream_t st);
+
 // X[i,c] *= 1/sqrt(sums[c]), c < nc.
 template <typename T>
 void launch_normalize_cols(int64_t n, T* X, int ld, int nc, const double* sums, hipStream_t st);

@@ -44,6 +44,14 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
 #ifndef RCG_CSR_RPI
 #define RCG_CSR_RPI 1
 #endif
+// lanes per row and vectors per lane for rows of 17-32 16-byte vectors
+// (b = 64 FP64, b = 128 FP32)
+#ifndef RCG_CSR32_G
+#define RCG_CSR32_G 16
+#endif
+#ifndef RCG_CSR32_NV
+#define RCG_CSR32_NV 2
+#endif
 
 
 template <typename T, int MODE, bool CX, int G, int NV>
@@ -102,7 +110,7 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
   RCG_CSR_CASE(4, 4, 1)
   RCG_CSR_CASE(8, 4, 2)
   RCG_CSR_CASE(16, 8, 2)
-  RCG_CSR_CASE(32, 16, 2)
+  RCG_CSR_CASE(32, RCG_CSR32_G, RCG_CSR32_NV)
   RCG_CSR_CASE(64, 16, 4)
   RCG_CSR_CASE(128, 32, 4)
   RCG_CSR_CASE(256, 32, 8)
@@ -1871,6 +1879,176 @@ void launch_lz_step(int64_t n, const int* rowptr, const int* col, const double* 
 }
 
 // ------------------------------------------------------- instantiations
+// ------------------------------------------ layouts for caller operators
+// A caller-supplied operator works on column-major blocks (element (i, c) at
+// C[c*ldc + i], the reference's AmatrixClassLMCuda layout); the solver's
+// blocks are row-interleaved (element (i, c) at X[i*ldx + c]).  E is one
+// block entry: double, or double2 for a complex entry (re, im).  32 x 32
+// tiles through LDS keep both sides coalesced.
+template <typename E>
+struct EntryOps;
+template <>
+struct EntryOps<double> {
+  __device__ static double comb(double p0, double p1, double p2, double ax, double x, double z) {
+    return p2 * fma(p1, fma(p0, ax, x), -z);
+  }
+  __device__ static double zero() { return 0.0; }
+};
+template <>
+struct EntryOps<double2> {
+  __device__ static double2 comb(double p0, double p1, double p2, double2 ax, double2 x,
+                                 double2 z) {
+    return make_double2(p2 * fma(p1, fma(p0, ax.x, x.x), -z.x), p2 * fma(p1, fma(p0, ax.y, x.y), -z.y));
+  }
+  __device__ static double2 zero() { return make_double2(0.0, 0.0); }
+};
+
+constexpr int kLt = 32;  // layout tile
+
+// TO_COLS: rows layout -> column-major; else the reverse.
+template <typename E, bool TO_COLS>
+__global__ __launch_bounds__(kThreads) void layout_kernel(int64_t n, int cols, E* R, int64_t ldr,
+                                                          E* Cm, int64_t ldc) {
+  __shared__ E tile[kLt][kLt + 1];
+  const int64_t i0 = (int64_t)blockIdx.x * kLt;
+  const int c0 = blockIdx.y * kLt;
+  const int tx = threadIdx.x % kLt, ty = threadIdx.x / kLt;
+  for (int r = ty; r < kLt; r += kThreads / kLt) {
+    if (TO_COLS) {  // tile[row][col] from the rows layout
+      const int64_t i = i0 + r;
+      const int c = c0 + tx;
+      if (i < n && c < cols) tile[r][tx] = R[i * ldr + c];
+    } else {  // tile[col][row] from the column-major layout
+      const int c = c0 + r;
+      const int64_t i = i0 + tx;
+      if (i < n && c < cols) tile[r][tx] = Cm[(int64_t)c * ldc + i];
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < kLt; r += kThreads / kLt) {
+    if (TO_COLS) {
+      const int c = c0 + r;
+      const int64_t i = i0 + tx;
+      if (i < n && c < cols) Cm[(int64_t)c * ldc + i] = tile[tx][r];
+    } else {
+      const int64_t i = i0 + r;
+      const int c = c0 + tx;
+      if (i < n && c < cols) R[i * ldr + c] = tile[tx][r];
+    }
+  }
+}
+
+// out = p2 (p1 (p0 AX + X) - Z) in the rows layout, AX column-major (Z may
+// be null: no Z term; out may alias Z): the Chebyshev step around a caller
+// operator's product, the same expression as the fused CSR step's epilogue.
+template <typename E>
+__global__ __launch_bounds__(kThreads) void cheb_combine_kernel(int64_t n, int cols,
+                                                                const E* __restrict__ AX,
+                                                                int64_t ldc, const E* X,
+                                                                const E* Z, E* out, int64_t ldr,
+                                                                double p0, double p1, double p2) {
+  __shared__ E tile[kLt][kLt + 1];
+  const int64_t i0 = (int64_t)blockIdx.x * kLt;
+  const int c0 = blockIdx.y * kLt;
+  const int tx = threadIdx.x % kLt, ty = threadIdx.x / kLt;
+  for (int r = ty; r < kLt; r += kThreads / kLt) {
+    const int c = c0 + r;
+    const int64_t i = i0 + tx;
+    if (i < n && c < cols) tile[r][tx] = AX[(int64_t)c * ldc + i];
+  }
+  __syncthreads();
+  for (int r = ty; r < kLt; r += kThreads / kLt) {
+    const int64_t i = i0 + r;
+    const int c = c0 + tx;
+    if (i < n && c < cols) {
+      const int64_t o = i * ldr + c;
+      const E z = Z ? Z[o] : EntryOps<E>::zero();
+      out[o] = EntryOps<E>::comb(p0, p1, p2, tile[tx][r], X[o], z);
+    }
+  }
+}
+
+void launch_layout(int64_t n, int cols, int cw, double* rows, int64_t ldr_real, double* colmaj,
+                   int64_t ldc, bool to_cols, hipStream_t st) {
+  if (n <= 0 || cols <= 0) return;
+  if (cw != 1 && cw != 2) throw Error(1, "entries are real or complex");
+  if (ldr_real % cw != 0) throw Error(1, "row stride is not a whole number of entries");
+  const dim3 grid((unsigned)((n + kLt - 1) / kLt), (unsigned)((cols + kLt - 1) / kLt));
+  const int64_t ldr = ldr_real / cw;
+  if (cw == 1) {
+    if (to_cols)
+      hipLaunchKernelGGL((layout_kernel<double, true>), grid, dim3(kThreads), 0, st, n, cols, rows,
+                         ldr, colmaj, ldc);
+    else
+      hipLaunchKernelGGL((layout_kernel<double, false>), grid, dim3(kThreads), 0, st, n, cols, rows,
+                         ldr, colmaj, ldc);
+  } else {
+    double2* r2 = reinterpret_cast<double2*>(rows);
+    double2* c2 = reinterpret_cast<double2*>(colmaj);
+    if (to_cols)
+      hipLaunchKernelGGL((layout_kernel<double2, true>), grid, dim3(kThreads), 0, st, n, cols, r2,
+                         ldr, c2, ldc);
+    else
+      hipLaunchKernelGGL((layout_kernel<double2, false>), grid, dim3(kThreads), 0, st, n, cols, r2,
+                         ldr, c2, ldc);
+  }
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_cheb_combine(int64_t n, int cols, int cw, const double* AX, int64_t ldc,
+                         const double* X, const double* Z, double* out, int64_t ldr_real,
+                         double p0, double p1, double p2, hipStream_t st) {
+  if (n <= 0 || cols <= 0) return;
+  if (cw != 1 && cw != 2) throw Error(1, "entries are real or complex");
+  if (ldr_real % cw != 0) throw Error(1, "row stride is not a whole number of entries");
+  const dim3 grid((unsigned)((n + kLt - 1) / kLt), (unsigned)((cols + kLt - 1) / kLt));
+  const int64_t ldr = ldr_real / cw;
+  if (cw == 1) {
+    hipLaunchKernelGGL(cheb_combine_kernel<double>, grid, dim3(kThreads), 0, st, n, cols, AX, ldc,
+                       X, Z, out, ldr, p0, p1, p2);
+  } else {
+    hipLaunchKernelGGL(cheb_combine_kernel<double2>, grid, dim3(kThreads), 0, st, n, cols,
+                       reinterpret_cast<const double2*>(AX), ldc,
+                       reinterpret_cast<const double2*>(X), reinterpret_cast<const double2*>(Z),
+                       reinterpret_cast<double2*>(out), ldr, p0, p1, p2);
+  }
+  RCG_HIP(hipGetLastError());
+}
+
+// partial sums of Re and Im of sum conj(w) v over interleaved complex vectors
+__global__ __launch_bounds__(kThreads) void lz_dot_c_kernel(int64_t n, const double* w,
+                                                            const double* v, double* P,
+                                                            int64_t nblk, int64_t rpb) {
+  __shared__ double red[kThreads];
+  const int64_t i0 = blockIdx.x * rpb;
+  const int64_t i1 = min(n, i0 + rpb);
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const double wr = w[2 * i], wi = w[2 * i + 1], vr = v[2 * i], vi = v[2 * i + 1];
+    sr = fma(wr, vr, fma(wi, vi, sr));
+    si = fma(wr, vi, fma(-wi, vr, si));
+  }
+  sr = block_sum(sr, red);
+  si = block_sum(si, red);
+  if (threadIdx.x == 0) {
+    P[blockIdx.x] = sr;
+    P[nblk + blockIdx.x] = si;
+  }
+}
+
+void launch_lz_dot_c(int64_t n, const double* w, const double* v, double* alpha_k,
+                     double* alpha_im_k, double* ws, size_t ws_elems, hipStream_t st) {
+  const int64_t rpb = lz_rows(n);
+  const int64_t nblk = (n + rpb - 1) / rpb;
+  if ((size_t)(2 * nblk) > ws_elems) throw Error(3, "Lanczos workspace too small");
+  hipLaunchKernelGGL(lz_dot_c_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, st, n, w, v, ws,
+                     nblk, rpb);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws, nblk, alpha_k, 0);
+  hipLaunchKernelGGL(lz_finalize_kernel, dim3(1), dim3(kThreads), 0, st, ws + nblk, nblk,
+                     alpha_im_k, 0);
+  RCG_HIP(hipGetLastError());
+}
+
 #define RCG_INST(T)                                                                              \
   template void launch_csr_apply<T>(int64_t, const int*, const int*, const T*, const T*, T*, int, \
                                     int, T, T, hipStream_t, const CsrSched*, bool);               \

@@ -300,11 +300,16 @@ struct rcg_operator_s {
   DevBuf val32;                    // FP32 copy of val for the mixed-precision filter
   // dense symmetric operator (kind 1): n x n, row-major = column-major
   std::mutex lazy_mu;              // guards the lazily built members below and above
-  int kind = 0;                    // 0 CSR, 1 dense
+  int kind = 0;                    // 0 CSR, 1 dense, 2 caller's apply
   const double* dense = nullptr;
   int64_t lda = 0;
   DevBuf own_dense;
+  // caller's operator (kind 2): out = alpha A in + beta out on column-major blocks
+  rcg_apply_fn apply_fn = nullptr;
+  void* user = nullptr;
   bool is_dense() const { return kind == 1; }
+  bool is_csr() const { return kind == 0; }
+  bool is_callback() const { return kind == 2; }
 };
 
 namespace {
@@ -686,6 +691,8 @@ class Engine {
     cw_ = cx_ ? 2 : 1;
     coef_.use_stream(st_);
     hcoef_.use_stream(st_);
+    cb_in_.use_stream(st_);
+    cb_out_.use_stream(st_);
   }
   bool cx() const { return cx_; }
   int cw() const { return cw_; }
@@ -869,6 +876,11 @@ class Engine {
   }
   // W[:, :b] = A V[:, :b] (b complex columns when complex).
   void apply(const double* X, double* out, int64_t b) {
+    if (op_->is_callback()) {
+      const double* ax = cb_product(X, b);
+      rcg::launch_layout(n_, (int)b, cw_, out, ld_, const_cast<double*>(ax), n_, false, st_);
+      return;
+    }
     if (op_->is_dense()) {
       rcg::launch_dense_step<double>(n_, op_->dense, op_->lda, X, nullptr, out, (int)ld_, (int)b,
                                      1.0, 0.0, 0.0, rcg::kStepApply, nullptr, nullptr, st_);
@@ -891,6 +903,7 @@ class Engine {
   // X1 = g2(g1 A X0 + X0), Xj = g2(g1 A X_{j-1} + X_{j-1}) - X_{j-2}, out = Xm/(m+1).
   void filter(int64_t b, long m, long rep, double lmax, double shift) {
     if (op_->is_dense()) return filter_dense(b, m, rep, lmax, shift);
+    if (op_->is_callback()) return filter_callback(b, m, rep, lmax, shift);
     if (batcher_) return filter_batched(b, m, rep, lmax, shift);
     const auto c = rcg::cpoly::coefficients(m, lmax, shift);
     const int64_t rb = cw_ * b;  // real columns
@@ -1003,6 +1016,62 @@ class Engine {
     step_end();
   }
 
+  // The caller's operator (LanczosCpolyOperatorLMCuda.h:236,245, its
+  // Otype::apply(in, out, alpha, beta)): its product on the column-major
+  // copy of the block, then the rest of the step in one pass back to the
+  // block layout.  One layout change, the caller's apply and one combine per
+  // degree; the step's bytes depend on the caller's product (not counted).
+  void filter_callback(int64_t b, long m, long rep, double lmax, double shift) {
+    const auto c = rcg::cpoly::coefficients(m, lmax, shift);
+    last_panel_ = (int)b;
+    last_sched_ = "caller";
+    step_begin(rep * m, 0.0);
+    for (long r = 0; r < rep; ++r) {
+      double* X0 = V();
+      double* X1 = T();
+      const double s1 = (m == 1) ? 0.5 : 1.0;
+      rcg::launch_cheb_combine(n_, (int)b, cw_, cb_product(X0, b), n_, X0, nullptr, X1, ld_, c.g1,
+                               c.g2, s1, st_);
+      for (long j = 2; j <= m; ++j) {
+        const double s = (j == m) ? 1.0 / double(m + 1) : 1.0;
+        rcg::launch_cheb_combine(n_, (int)b, cw_, cb_product(X1, b), n_, X1, X0, X0, ld_, c.g1,
+                                 c.g2, s, st_);
+        std::swap(X0, X1);
+      }
+      if (X1 != V()) swapVT();
+    }
+    step_end();
+  }
+  // A X over b columns of X (rows layout, ld_) through the caller's apply:
+  // the column-major product (n x b, ld n), valid until the next call.
+  const double* cb_product(const double* X, int64_t b) {
+    const size_t bytes = sizeof(double) * (size_t)cw_ * (size_t)n_ * (size_t)b;
+    cb_in_.ensure(bytes);
+    cb_out_.ensure(bytes);
+    rcg::launch_layout(n_, (int)b, cw_, const_cast<double*>(X), ld_, cb_in_.as<double>(), n_, true,
+                       st_);
+    call_user(cb_in_.p, cb_out_.p, b, n_, 1.0, 0.0);
+    return cb_out_.as<double>();
+  }
+  // out = alpha A X + beta out over b columns (rows layout, ld_) through the
+  // caller's apply (rcg_operator_apply on a caller's operator)
+  void cb_apply_ab(const double* X, double* out, int64_t b, double alpha, double beta) {
+    const size_t bytes = sizeof(double) * (size_t)cw_ * (size_t)n_ * (size_t)b;
+    cb_in_.ensure(bytes);
+    cb_out_.ensure(bytes);
+    rcg::launch_layout(n_, (int)b, cw_, const_cast<double*>(X), ld_, cb_in_.as<double>(), n_, true,
+                       st_);
+    if (beta != 0.0)
+      rcg::launch_layout(n_, (int)b, cw_, out, ld_, cb_out_.as<double>(), n_, true, st_);
+    call_user(cb_in_.p, cb_out_.p, b, n_, alpha, beta);
+    rcg::launch_layout(n_, (int)b, cw_, out, ld_, cb_out_.as<double>(), n_, false, st_);
+  }
+  void call_user(const void* in, void* out, int64_t cols, int64_t ld, double alpha, double beta) {
+    const int rc = op_->apply_fn(op_->user, in, out, n_, cols, ld, alpha, beta, (void*)st_);
+    if (rc != 0)
+      throw Error(RCG_ERR_INVALID, "the operator's apply callback returned " + std::to_string(rc));
+  }
+
   // Mixed-precision filter (the correction recurrence of
   // launch_cheb_step_src): V <- P_m(A')^rep V over b columns with FP32 steps
   // and FP32 CSR values.  theta[c] / ritz[c]: the Ritz value of column c and
@@ -1011,6 +1080,9 @@ class Engine {
   // columns run as their two real columns with the same theta.
   void filter_mixed(int64_t b, long m, long rep, double lmax, double shift,
                     const std::vector<double>& theta_c, const std::vector<char>& ritz_c) {
+    if (op_->is_callback())
+      throw Error(RCG_ERR_UNSUPPORTED, "mixed-precision filtering needs the operator's values "
+                                       "(a CSR or dense operator), not a caller's apply");
     if (m < 2) return filter(b, m, rep, lmax, shift);
     if (!E_[0]) throw Error(RCG_ERR_INVALID, "mixed-precision buffers are not attached");
     const int64_t rb = cw_ * b;
@@ -1497,7 +1569,7 @@ class Engine {
     // scalings and the real-alpha axpy act on them as real vectors
     const int64_t nr = cw_ * n_;
     // real CSR operators run the fused step on a ping-pong pair (v, w), (v2, w2)
-    const bool fused = !cx_ && !op_->is_dense();
+    const bool fused = !cx_ && op_->is_csr();
     DevBuf vecs(st_), ab(st_);
     vecs.ensure(sizeof(double) * (size_t)nr * (fused ? 4 : 2));
     ab.ensure(sizeof(double) * (size_t)(3 * itmax + 8));
@@ -1527,6 +1599,13 @@ class Engine {
         rcg::launch_dense_step<double>(n_, op_->dense, op_->lda, w, nullptr, v, 1, 1, 1.0, 1.0,
                                        0.0, rcg::kStepApplyBeta, nullptr, nullptr, st_);
         rcg::launch_lz_dot(n_, w, v, alpha + k, ws(), ws_elems_, st_);
+      } else if (op_->is_callback()) {
+        // v = A w + v: an n-vector is a one-column block in either layout
+        call_user(w, v, 1, n_, 1.0, 1.0);
+        if (cx_)
+          rcg::launch_lz_dot_c(n_, w, v, alpha + k, alpha_im + k, ws(), ws_elems_, st_);
+        else
+          rcg::launch_lz_dot(n_, w, v, alpha + k, ws(), ws_elems_, st_);
       } else {
         rcg::launch_lz_apply_dot_c(n_, op_->rowptr, op_->col, op_->val, w, v, alpha + k,
                                    alpha_im + k, ws(), ws_elems_, st_);
@@ -1665,6 +1744,7 @@ class Engine {
     else return fold_c_;
   }
   DevBuf coef_;                       // mixed filter coefficient table
+  DevBuf cb_in_, cb_out_;             // column-major blocks of a caller's operator
   hipEvent_t coef_ev_ = nullptr;      // its upload from hcoef_ has finished
   HostBuf hcoef_;
   hipEvent_t fev_[2] = {nullptr, nullptr};
@@ -1680,11 +1760,75 @@ struct TraceRow {
   double v[10];
 };
 
+// Stream time per phase of a solve.  An event is recorded on the solve's
+// stream where a phase begins; the time between consecutive events goes to
+// the phase that began at the first of them, so it includes the stream's
+// idle time while the host works inside the phase (the small eigensolve
+// counts to Rayleigh-Ritz, a served solve's wait for its shared filter batch
+// to the filter).  Finished intervals are folded whenever a new phase
+// begins; the solve's host waits keep the list of pending events short.
+class PhaseClock {
+ public:
+  enum { kLanczos = 0, kFilter, kOrtho, kRR, kOther, kCount };
+  explicit PhaseClock(hipStream_t st) : st_(st) {}
+  PhaseClock(const PhaseClock&) = delete;
+  PhaseClock& operator=(const PhaseClock&) = delete;
+  ~PhaseClock() {
+    for (const Mark& m : marks_) (void)hipEventDestroy(m.ev);
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  }
+  void mark(int phase) {
+    fold(false);
+    hipEvent_t e;
+    if (!pool_.empty()) {
+      e = pool_.back();
+      pool_.pop_back();
+    } else {
+      RCG_HIP(hipEventCreate(&e));
+    }
+    RCG_HIP(hipEventRecord(e, st_));
+    marks_.push_back({e, phase});
+  }
+  // closes the last phase and folds every interval (waits for the stream)
+  void finish() {
+    mark(kOther);
+    if (!marks_.empty()) wait_event(marks_.back().ev, true);
+    fold(true);
+  }
+  double seconds(int phase) const { return sec_[phase]; }
+
+ private:
+  struct Mark {
+    hipEvent_t ev;
+    int phase;
+  };
+  void fold(bool done) {
+    while (marks_.size() >= 2) {
+      if (!done) {
+        const hipError_t r = hipEventQuery(marks_[1].ev);
+        if (r != hipSuccess) {
+          (void)hipGetLastError();
+          break;
+        }
+      }
+      float ms = 0.f;
+      RCG_HIP(hipEventElapsedTime(&ms, marks_[0].ev, marks_[1].ev));
+      sec_[marks_[0].phase] += 1.0e-3 * ms;
+      pool_.push_back(marks_[0].ev);
+      marks_.pop_front();
+    }
+  }
+  hipStream_t st_;
+  std::deque<Mark> marks_;
+  std::vector<hipEvent_t> pool_;
+  double sec_[kCount] = {};
+};
+
 class Solver {
  public:
   Solver(rcg_operator_s* op, const rcg_solve_params& p, rcg_context_s* ctx = nullptr,
          FilterBatcher* batcher = nullptr)
-      : op_(op), p_(p), eng_(ctx ? ctx : op->ctx, op, op->n) {
+      : op_(op), p_(p), eng_(ctx ? ctx : op->ctx, op, op->n), clock_(eng_.stream()) {
     eng_.set_batcher(batcher);
     for (DevBuf* b : {&Vb_, &Tb_, &Wb_, &Yb_, &Mb_}) b->use_stream(eng_.stream());
   }
@@ -1758,7 +1902,7 @@ class Solver {
 
     // spectral bounds (getMinEigenSystem, LM:712-738; interval, LM:745-782)
     double minEigValue, maxEigValue, lambdaMax;
-    const double tl0 = wall();
+    clock_.mark(PhaseClock::kLanczos);
     if (p_.use_bounds) {  // caller's bounds, no Lanczos (RayleighChebyshev.h:709-723, 812-821)
       minEigValue = p_.min_eig;
       maxEigValue = p_.max_eig;
@@ -1774,7 +1918,8 @@ class Solver {
       R->max_est = mx;
       if (p_.interval && p_.lambda_max < mn) {
         R->found = 0;
-        R->t_lanczos = wall() - tl0;
+        clock_.finish();
+        R->t_lanczos = clock_.seconds(PhaseClock::kLanczos);
         R->t_total = wall() - t0;
         return 0;
       }
@@ -1783,7 +1928,6 @@ class Solver {
       maxEigValue = mx;
       lambdaMax = p_.interval ? p_.lambda_max : mx;
     }
-    R->t_lanczos = wall() - tl0;
     const bool intervalFlag = p_.interval != 0;
 
     long foundSize = 0;
@@ -1797,6 +1941,7 @@ class Solver {
     std::vector<S> H, Yh;
 
     // start vectors (LM:948-985)
+    clock_.mark(PhaseClock::kOrtho);
     if (start_ && start_cols_ > 0) {
       const long have = std::min<long>((long)start_cols_, subspaceSize);
       eng_.copy_cols(start_, cw * start_ld_, 0, eng_.V(), ld, 0, have);
@@ -1815,6 +1960,9 @@ class Solver {
       eigValues = theta;
       found_ = subspaceSize;
       R->found = subspaceSize;
+      clock_.finish();
+      R->t_lanczos = clock_.seconds(PhaseClock::kLanczos);
+      R->t_ortho = clock_.seconds(PhaseClock::kOrtho);
       R->t_total = wall() - t0;
       return subspaceSize;
     }
@@ -1832,10 +1980,9 @@ class Solver {
         eng_.randomize(buf, subspaceSize - buf, p_.seed, &counter_);
         for (long k = buf; k < subspaceSize; ++k) fRitz[k] = 0;
       }
-      double ta = wall();
+      clock_.mark(PhaseClock::kOrtho);
       eng_.orthonormalize(subspaceSize);
       count_ortho_ += 2;
-      t_ortho_ += wall() - ta;
 
       double lambdaStar = maxEigValue;
       eigDiffRatio = 1.0;
@@ -1870,7 +2017,7 @@ class Solver {
           starDegree = starDegreeSave;
           starBound = starBoundSave;
         }
-        ta = wall();
+        clock_.mark(PhaseClock::kFilter);
         if (!completedBasis) {
           if (mixed)
             eng_.filter_mixed(subspaceSize, starDegree, rep, starBound, shift, fTheta, fRitz);
@@ -1880,15 +2027,13 @@ class Solver {
         }
         applyCount += 1;
         count_apply_ += starDegree * subspaceSize;
-        t_filter_ += wall() - ta;  // enqueue time; device time is read after the next sync
 
-        ta = wall();
+        clock_.mark(PhaseClock::kOrtho);
         eng_.deflate(subspaceSize, foundSize);
         count_ortho_ += 1;
-        t_ortho_ += wall() - ta;
 
-        // orthogonalize and project (the CholQR passes count as t_rr)
-        ta = wall();
+        // orthogonalize and project (the CholQR passes count as Rayleigh-Ritz)
+        clock_.mark(PhaseClock::kRR);
         eng_.ortho_project(subspaceSize, H);
         count_apply_ += subspaceSize;
         const double te = wall();
@@ -1900,7 +2045,6 @@ class Solver {
         fRitz.assign(subspaceSize, 1);
         count_apply_ += residualCheckCount;
         count_eig_ += 1;
-        t_rr_ += wall() - ta;
 
         // convergence bookkeeping (LM:1285-1457)
         maxResidual = 0.0;
@@ -2025,12 +2169,12 @@ class Solver {
     for (long i = 0; i + 1 < foundSize; ++i)
       if (eigValues[i] > eigValues[i + 1]) nonMonotone = true;
     if (nonMonotone || ((p_.refine || mixed) && foundSize > 0)) {
-      const double tr = wall();
+      clock_.mark(PhaseClock::kRR);
       repair<S>(foundSize, eigValues, eigRes);
       count_apply_ += 2 * foundSize;
       count_eig_ += 1;
-      t_rr_ += wall() - tr;
     }
+    clock_.mark(PhaseClock::kOther);
     if (intervalFlag) {  // trim (LM:1715-1732)
       long fc = 0;
       for (double e : eigValues)
@@ -2051,10 +2195,12 @@ class Solver {
     R->final_maxres = fmaxres;
     R->final_maxdiff = fmaxdiff;
     R->final_maxgap = fmaxgap;
+    clock_.finish();
     R->t_total = wall() - t0;
-    R->t_filter = t_filter_;
-    R->t_ortho = t_ortho_;
-    R->t_rr = t_rr_;
+    R->t_lanczos = clock_.seconds(PhaseClock::kLanczos);
+    R->t_filter = clock_.seconds(PhaseClock::kFilter);
+    R->t_ortho = clock_.seconds(PhaseClock::kOrtho);
+    R->t_rr = clock_.seconds(PhaseClock::kRR);
     R->t_eig = t_eig_;
     R->filter_degree_sum = degree_sum_;
     R->t_filter_device = eng_.step_seconds();
@@ -2111,8 +2257,9 @@ class Solver {
   Engine eng_;
   DevBuf Vb_, Tb_, Wb_, Yb_, Mb_;
   long found_ = 0;
+  PhaseClock clock_;
   long count_ortho_ = 0, count_apply_ = 0, count_eig_ = 0, degree_sum_ = 0;
-  double guard_ = 0.0, t_filter_ = 0.0, t_ortho_ = 0.0, t_rr_ = 0.0, t_eig_ = 0.0;
+  double guard_ = 0.0, t_eig_ = 0.0;
 };
 
 // allow_cx: a complex operator with complex (C128) blocks is accepted too.
@@ -2522,6 +2669,34 @@ int rcg_operator_csr_create(rcg_context ctx, int64_t n, int64_t nnz, const int32
   });
 }
 
+int rcg_operator_csr_share(rcg_operator base, const void* val, int mem, rcg_operator* out) {
+  return guarded([&] {
+    need(base && val && out, "NULL argument");
+    need(base->is_csr(), "the pattern comes from a CSR operator");
+    need(mem == RCG_MEM_HOST || mem == RCG_MEM_DEVICE, "bad memory flag");
+    DeviceGuard g(base->ctx->device);
+    auto o = std::make_unique<rcg_operator_s>();
+    o->ctx = base->ctx;
+    o->n = base->n;
+    o->nnz = base->nnz;
+    o->dtype = base->dtype;
+    o->rowptr = base->rowptr;
+    o->col = base->col;
+    o->gx = base->gx;
+    o->gy = base->gy;
+    o->gz = base->gz;
+    if (mem == RCG_MEM_DEVICE) {
+      o->val = static_cast<const double*>(val);
+    } else {
+      const size_t vs = rcg::dtype_size(base->dtype);
+      o->own_val.ensure(vs * (size_t)std::max<int64_t>(base->nnz, 1));
+      RCG_HIP(hipMemcpy(o->own_val.p, val, vs * base->nnz, hipMemcpyHostToDevice));
+      o->val = o->own_val.as<double>();
+    }
+    *out = o.release();
+  });
+}
+
 int rcg_operator_dense_create(rcg_context ctx, int64_t n, const double* A, int64_t lda, int mem,
                               rcg_operator* out) {
   return guarded([&] {
@@ -2549,10 +2724,28 @@ int rcg_operator_dense_create(rcg_context ctx, int64_t n, const double* A, int64
   });
 }
 
+int rcg_operator_callback_create(rcg_context ctx, int64_t n, int dtype, rcg_apply_fn apply,
+                                 void* user, rcg_operator* out) {
+  return guarded([&] {
+    need(ctx && apply && out, "NULL argument");
+    need(n > 0, "bad operator size");
+    need(dtype == RCG_F64 || dtype == RCG_C128, "a caller's operator is RCG_F64 or RCG_C128");
+    auto o = std::make_unique<rcg_operator_s>();
+    o->ctx = ctx;
+    o->n = n;
+    o->nnz = 0;
+    o->dtype = dtype;
+    o->kind = 2;
+    o->apply_fn = apply;
+    o->user = user;
+    *out = o.release();
+  });
+}
+
 int rcg_operator_set_grid(rcg_operator op, int64_t nx, int64_t ny, int64_t nz) {
   return guarded([&] {
     need(op != nullptr, "operator is NULL");
-    need(!op->is_dense(), "a grid hint applies to CSR operators");
+    need(op->is_csr(), "a grid hint applies to CSR operators");
     need(nx >= 1 && ny >= 1 && nz >= 1 && nx * ny * nz == op->n, "grid does not match the operator size");
     DeviceGuard g(op->ctx->device);
     RCG_HIP(hipStreamSynchronize(op->ctx->stream));
@@ -2580,7 +2773,12 @@ int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alph
     need(in != out, "in and out must differ");
     DeviceGuard g(op->ctx->device);
     const bool cx = op->dtype == RCG_C128;
-    if (op->is_dense())
+    if (op->is_callback()) {
+      Engine e(op->ctx, op, op->n);
+      e.attach(nullptr, nullptr, nullptr, real_ld(in), nullptr, 0);
+      e.cb_apply_ab(static_cast<const double*>(in->d()), static_cast<double*>(out->d()), cols,
+                    alpha, beta);
+    } else if (op->is_dense())
       rcg::launch_dense_step<double>(op->n, op->dense, op->lda, static_cast<const double*>(in->d()),
                                      nullptr, static_cast<double*>(out->d()), (int)in->ld,
                                      (int)cols, alpha, beta, 0.0,
@@ -2753,6 +2951,9 @@ int rcg_solve_lowest(rcg_operator op, const rcg_solve_params* params, rcg_solve_
     need(params->k >= 1, "eigCount must be >= 1");
     need(params->filter_dtype == RCG_F64 || params->filter_dtype == RCG_F32,
          "filter_dtype must be RCG_F64 or RCG_F32");
+    if (op->is_callback() && params->filter_dtype == RCG_F32)
+      throw Error(RCG_ERR_UNSUPPORTED, "mixed-precision filtering needs the operator's values "
+                                       "(a CSR or dense operator), not a caller's apply");
     if (evecs)
       need(evecs->n == op->n && evecs->dtype == op->dtype && evecs->ctx == op->ctx,
            "evecs block mismatch (rows, dtype of the operator, context)");
@@ -2825,6 +3026,9 @@ void server_enqueue(rcg_server s, rcg_operator op, const rcg_solve_params* param
        "served solves start from random vectors or, with rcg_server_submit_warm, from a "
        "finished ticket's vectors");
   need(op->ctx->device == s->ctx->device, "operator on another device");
+  if (op->is_callback())
+    throw Error(RCG_ERR_UNSUPPORTED, "the server takes CSR and dense operators (a caller's apply "
+                                     "would run on the server's worker threads)");
   auto t = std::make_unique<Ticket>();
   t->op = op;
   t->p = *params;

@@ -126,18 +126,51 @@ def _grid_neighbours(shape, periodic):
 
 
 def _laplacian(shape, bc, potential):
+    """CSR of the 2d+1-point Laplacian on `shape` (x fastest), built row by
+    row in column order: each row holds its -x..-z and +x..+z neighbours
+    (those inside a Dirichlet grid, all of them on a periodic one) and the
+    diagonal, columns sorted."""
     periodic = bc == "periodic"
     if periodic and min(shape) < 3:
         raise ValueError("periodic grids need at least 3 points per side")
+    dims = len(shape)
     n = int(np.prod(shape))
-    r, c = _grid_neighbours(shape, periodic)
-    diag = np.full(n, 2.0 * len(shape))
+    rev = tuple(reversed(shape))  # array axes (z, y, x): C order is x fastest
+    idx = np.arange(n, dtype=np.int64).reshape(rev)
+    strides = [int(np.prod(shape[:d])) for d in range(dims)]
+    w = 2 * dims + 1
+    cols = np.empty(rev + (w,), dtype=np.int64)
+    keep = np.ones(rev + (w,), dtype=bool)
+    vals = np.full(rev + (w,), -1.0)
+    diag = np.full(n, 2.0 * dims)
     if potential is not None:
         diag = diag + np.asarray(potential, dtype=np.float64)
-    rows = np.concatenate([np.arange(n), r])
-    cols = np.concatenate([np.arange(n), c])
-    vals = np.concatenate([diag, -np.ones(r.shape[0])])
-    A = _from_coo(n, rows, cols, vals)
+    for d in range(dims):
+        L = shape[d]
+        view = [1] * dims
+        view[dims - 1 - d] = L
+        c = np.arange(L).reshape(view)  # coordinate d, broadcast over the grid
+        lo, hi = dims - 1 - d, dims + 1 + d  # slots of the -d and +d neighbours
+        cols[..., lo] = idx - strides[d]
+        cols[..., hi] = idx + strides[d]
+        if periodic:
+            cols[..., lo] += np.where(c == 0, L * strides[d], 0)
+            cols[..., hi] -= np.where(c == L - 1, L * strides[d], 0)
+        else:
+            keep[..., lo] = c > 0
+            keep[..., hi] = c < L - 1
+    cols[..., dims] = idx
+    vals[..., dims] = diag.reshape(rev)
+    cols = cols.reshape(n, w)
+    vals = vals.reshape(n, w)
+    keep = keep.reshape(n, w)
+    if periodic:  # wrapped neighbours break the slot order
+        order = np.argsort(cols, axis=1, kind="stable")
+        cols = np.take_along_axis(cols, order, axis=1)
+        vals = np.take_along_axis(vals, order, axis=1)
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.count_nonzero(keep, axis=1), out=rowptr[1:])
+    A = Csr(n, rowptr, cols[keep], vals[keep])
     A.grid = tuple(shape)  # locality hint for the device kernels (row order x fastest)
     return A
 

@@ -36,12 +36,41 @@ class ServingRequest:
     def n(self) -> int:
         return self.side ** self.dim
 
+    def potential(self):
+        return ops.anderson_potential(self.n, self.W, self.seed) if self.W > 0 else None
+
     def operator(self) -> ops.Csr:
         """The request's 5- or 7-point Dirichlet Laplacian plus its potential."""
-        pot = ops.anderson_potential(self.n, self.W, self.seed) if self.W > 0 else None
         if self.dim == 2:
-            return ops.laplacian_2d(self.side, bc="dirichlet", potential=pot)
-        return ops.laplacian_3d(self.side, bc="dirichlet", potential=pot)
+            return ops.laplacian_2d(self.side, bc="dirichlet", potential=self.potential())
+        return ops.laplacian_3d(self.side, bc="dirichlet", potential=self.potential())
+
+
+class OperatorCache:
+    """The stream's operators on the device, one sparsity pattern per grid:
+    the first request of a (dimension, side) uploads its CSR; later ones share
+    that pattern's row pointers and columns (rcg_operator_csr_share) and upload
+    only their values -- the same arrays as ``ServingRequest.operator()``."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx
+        self.base = {}
+
+    def operator(self, r: ServingRequest):
+        from .solver import CsrOperator
+        key = (r.dim, r.side)
+        if key not in self.base:
+            A0 = (ops.laplacian_2d(r.side, bc="dirichlet") if r.dim == 2
+                  else ops.laplacian_3d(r.side, bc="dirichlet"))
+            rows = np.repeat(np.arange(A0.n), np.diff(A0.rowptr))
+            diag = np.flatnonzero(A0.col == rows)
+            self.base[key] = (CsrOperator(A0, self.ctx), A0.val, diag)
+        op0, val0, diag = self.base[key]
+        val = val0.copy()
+        pot = r.potential()
+        if pot is not None:
+            val[diag] = val0[diag] + pot  # 2d + V_i, as the host builder forms it
+        return op0.share(val)
 
 
 def serving_requests(count: int = 4096, rate: float = 200.0, seed: int = 4,

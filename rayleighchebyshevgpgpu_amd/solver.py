@@ -153,6 +153,22 @@ class CsrOperator:
         self.h = h
         return self
 
+    def share(self, val) -> "CsrOperator":
+        """An operator with this operator's sparsity pattern (row pointers and
+        columns shared on the device) and its own values (nnz entries)."""
+        o = CsrOperator.__new__(CsrOperator)
+        o.ctx, o.n, o.nnz = self.ctx, self.n, self.nnz
+        o.is_complex, o.dtype = self.is_complex, self.dtype
+        val = np.ascontiguousarray(val, dtype=self.dtype)
+        if val.shape != (self.nnz,):
+            raise ValueError("one value per stored entry of the pattern")
+        h = C.c_void_p()
+        check(lib().rcg_operator_csr_share(self.h, val.ctypes.data_as(C.c_void_p), capi.MEM_HOST,
+                                           C.byref(h)))
+        o.h = h
+        o._keep = self  # the pattern's owner outlives the sharer
+        return o
+
     @classmethod
     def from_csr64(cls, n, rowptr, col, val, ctx: Context | None = None) -> "CsrOperator":
         """An operator from 64-bit CSR index arrays (narrowed by the library)."""
@@ -171,6 +187,79 @@ class CsrOperator:
                                               C.byref(h)))
         self.h = h
         return self
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rcg_operator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def check_op(code: int, op) -> None:
+    """check() that re-raises a caller operator's own exception as the cause."""
+    err = getattr(op, "error", None)
+    if code != capi.RCG_OK and err is not None:
+        op.error = None
+        raise capi.RcgError(code, capi.last_error()) from err
+    check(code)
+
+
+class CallbackOperator:
+    """A caller's operator as code (the reference's Otype::apply(in, out,
+    alpha, beta), LanczosCpolyOperatorLMCuda.h:236,245).
+
+    ``apply(in_ptr, out_ptr, n, cols, ld, alpha, beta, stream)`` gets device
+    pointers (ints) to column-major n x cols blocks (leading dimension ld,
+    float64 or complex128) and the library stream (int); it must make
+    out = alpha*A*in + beta*out on that stream, typically by calling compiled
+    code of its own.  ``CallbackOperator.from_c`` takes such a function already
+    compiled (an ``rcg_apply_fn`` pointer and its user pointer) and keeps
+    Python out of the solve.  The operator must be Hermitian.  An exception in
+    ``apply`` fails the library call and is re-raised as its cause."""
+
+    def __init__(self, n: int, apply=None, dtype=np.float64, ctx: Context | None = None,
+                 c_function=None, c_user=None):
+        self.ctx = ctx or default_context()
+        self.n = int(n)
+        self.nnz = 0
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float64, np.complex128):
+            raise ValueError("a caller's operator is float64 or complex128")
+        self.is_complex = self.dtype == np.complex128
+        self.error = None
+        if c_function is not None:
+            self._cb = capi.APPLY_FN(c_function) if isinstance(c_function, int) else c_function
+            user = c_user
+        else:
+            def trampoline(user, pin, pout, n, cols, ld, alpha, beta, stream):
+                try:
+                    apply(pin, pout, n, cols, ld, alpha, beta, stream)
+                    return 0
+                except BaseException as e:  # reported through the library's status
+                    self.error = e
+                    return 1
+            self._cb = capi.APPLY_FN(trampoline)  # kept alive with the operator
+            user = None
+        h = C.c_void_p()
+        check(lib().rcg_operator_callback_create(self.ctx.h, self.n,
+                                                 capi.C128 if self.is_complex else capi.F64,
+                                                 self._cb, user, C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_c(cls, n: int, function, user, dtype=np.float64, ctx: Context | None = None):
+        """A compiled apply: ``function`` an rcg_apply_fn (address or ctypes
+        function pointer), ``user`` its first argument."""
+        return cls(n, dtype=dtype, ctx=ctx, c_function=function, c_user=user)
+
+    def apply(self, X: "Block", Y: "Block", alpha: float = 1.0, beta: float = 0.0, cols=None):
+        check_op(lib().rcg_operator_apply(self.h, X.h, Y.h, alpha, beta,
+                                          X.cols if cols is None else cols), self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -340,10 +429,10 @@ class RayleighChebyshevLM:
         rr = np.zeros(cap)
         trace = np.zeros((200000, 10))
         r = capi.SolveResult()
-        check(lib().rcg_solve_lowest(
+        check_op(lib().rcg_solve_lowest(
             oP.h, C.byref(p), C.byref(r), ev.ctypes.data_as(C.c_void_p),
             rr.ctypes.data_as(C.c_void_p), cap, eigVectors.h if eigVectors is not None else None,
-            trace.ctypes.data_as(C.c_void_p), trace.shape[0]))
+            trace.ctypes.data_as(C.c_void_p), trace.shape[0]), oP)
         self._result = r
         f = int(r.found)
         self._residuals = rr[:f].tolist()

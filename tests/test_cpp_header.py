@@ -49,3 +49,25 @@ def test_cpp_caller_solves():
     words = lines[-2].split()
     assert words[0] == "interval" and words[4] == words[6]
     assert lines[-1].startswith("found 10 ")
+
+
+@pytest.mark.gpu
+def test_cpp_matrix_free_caller_solves():
+    """A C++ caller whose operator is its own HIP stencil kernel (no matrix),
+    through rcg::CallbackOperator: the Dirichlet spectrum in closed form."""
+    import __graft_entry__ as ge
+    exe = ge.CPP_MATRIX_FREE
+    deps = [exe + ".hip", LIB, os.path.join(ROOT, "include", "rcg_c_api.h"),
+            os.path.join(ROOT, "include", "rcg", "RayleighChebyshevLM.hpp")]
+    if not os.path.exists(exe) or os.path.getmtime(exe) < max(os.path.getmtime(d) for d in deps):
+        ge.build_cpp_matrix_free()
+    print("cpp matrix-free caller: 20x20 Dirichlet stencil kernel, k=8", flush=True)
+    p = subprocess.run([exe, "20", "8", "1e-10"], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr)
+    assert p.returncode == 0
+    lines = p.stdout.strip().splitlines()
+    rows = np.array([[float(t) for t in ln.split()] for ln in lines[:-1]])
+    assert rows.shape == (8, 3)
+    assert np.max(np.abs(rows[:, 0] - rows[:, 1])) < 1e-10
+    words = lines[-1].split()
+    assert words[0] == "found" and words[1] == "8" and int(words[3]) > 8

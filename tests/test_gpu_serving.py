@@ -217,3 +217,33 @@ def test_warm_start_rejects_bad_tickets(rcg):
         srv.submit(opa, 4, 1e-10, 4, 4, precision="fp64", start_ticket=999)
     srv.release(t)
     srv.close()
+
+
+def test_shared_pattern_operators_match_host(rcg):
+    """The serving bench uploads one sparsity pattern per grid and each
+    request's values (rcg_operator_csr_share): every product equals the one of
+    the request's own host-built operator, bit for bit."""
+    print("case: shared-pattern serving operators vs their own CSR", flush=True)
+    from rayleighchebyshevgpgpu_amd import capi
+    reqs = serving.serving_requests(60, rate=200.0, seed=4, n_min=300, n_max=9000)
+    # requests on repeated grids, 2-D and 3-D
+    reqs = reqs + [serving.ServingRequest(100 + i, 0.0, r.dim, r.side, 3.0 + i, r.k, seed=2000 + i)
+                   for i, r in enumerate(reqs[:4])]
+    cache = serving.OperatorCache()
+    rng = np.random.default_rng(0)
+    shared = 0
+    for r in reqs:
+        op = cache.operator(r)
+        ref = rcg.CsrOperator(r.operator())
+        assert op.nnz == ref.nnz
+        X = rng.standard_normal((r.n, 5))
+        bx, b1, b2 = rcg.Block.from_host(X), rcg.Block(r.n, 5), rcg.Block(r.n, 5)
+        op.apply(bx, b1)
+        ref.apply(bx, b2)
+        assert np.array_equal(b1.download(), b2.download())
+        shared += (r.dim, r.side) in cache.base and op is not cache.base[(r.dim, r.side)][0]
+    assert len(cache.base) < len(reqs) and shared == len(reqs)
+    # a pattern from a non-CSR operator is refused
+    D = rcg.DenseOperator(np.eye(4))
+    h = capi.C.c_void_p()
+    assert capi.lib().rcg_operator_csr_share(D.h, np.ones(16).ctypes.data, 0, capi.C.byref(h)) == 1
