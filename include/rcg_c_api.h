@@ -142,8 +142,8 @@ int rcg_operator_dense_create(rcg_context ctx, int64_t n, const double* A, int64
  * The operator must be Hermitian.  Its filter runs unfused: per degree one
  * copy of the block into the column-major layout, the caller's product and
  * one pass that forms the Chebyshev step from it.  Mixed-precision filtering
- * (it needs FP32 operator values) and the server take CSR and dense
- * operators only (RCG_ERR_UNSUPPORTED). */
+ * (it needs FP32 operator values), the server and rcg_residuals_noalloc take
+ * CSR and dense operators only (RCG_ERR_UNSUPPORTED). */
 typedef int (*rcg_apply_fn)(void* user, const void* in, void* out, int64_t n, int64_t cols,
                             int64_t ld, double alpha, double beta, void* hip_stream);
 int rcg_operator_callback_create(rcg_context ctx, int64_t n, int dtype, rcg_apply_fn apply,
@@ -221,6 +221,18 @@ int rcg_small_eigensystem(int64_t n, const double* H, double* evals, double* eve
  * c < count (real theta; C128 blocks for a C128 operator). */
 int rcg_residuals(rcg_operator op, rcg_block V, rcg_block W, int64_t count, const double* theta,
                   double* res_host);
+/* No counterpart in the reference: rcg_residuals for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; it calls no
+ * hipMalloc, hipFree or allocating routine.  Its reduction partials, Ritz
+ * values and sums live in a library-owned device array of 16 MiB per device
+ * (2^21 doubles), held until the process ends: 3 doubles per real column
+ * (2 per complex column) plus one per real column and 512 rows must fit, else
+ * RCG_ERR_INVALID.  The operator's CSR rows are visited in the default order
+ * (no autotuning).  All calls on one device share that array: calls that
+ * overlap on one device race.  Waits for its stream before returning. */
+int rcg_residuals_noalloc(rcg_operator op, rcg_block V, rcg_block W, int64_t count,
+                          const double* theta, double* res_host);
 /* LanczosMaxMinFinder::getMinMaxEigenvalues (LanczosMaxMinFinder.h:248-544)
  * on the device: stop_flags bit0 = converge the maximum, bit1 = the minimum.
  * The start vector takes n draws of `seed` from *counter. */

@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
         "rcg_rotate": ([vp, vp, i64, vp], i32),
         "rcg_small_eigensystem": ([i64, vp, vp, vp], i32),
         "rcg_residuals": ([vp, vp, vp, i64, vp, vp], i32),
+        "rcg_residuals_noalloc": ([vp, vp, vp, i64, vp, vp], i32),
         "rcg_lanczos_bounds": ([vp, d, i32, i64, u64, P(u64), P(d), P(d), P(i64)], i32),
         "rcg_solve_lowest": ([vp, P(SolveParams), P(SolveResult), vp, vp, i64, vp, vp, i64], i32),
         "rcg_server_create": ([vp, i64, i32, i32, P(vp)], i32),

@@ -192,6 +192,10 @@ void launch_cheb_combine(int64_t n, int cols, int cw, const double* AX, int64_t 
 void launch_lz_dot_c(int64_t n, const double* w, const double* v, double* alpha_k,
                      double* alpha_im_k, double* ws, size_t ws_elems, hipStream_t st);
 
+// out[c] = sqrt(|sums[cw c] + ... + sums[cw c + cw - 1]|), c < count: the
+// residual norm of each (complex, cw = 2) column from its real columns' sums.
+void launch_group_sqrt(const double* sums, int cw, int count, double* out, hipStream_t st);
+
 // X[i,c] *= 1/sqrt(sums[c]), c < nc.
 template <typename T>
 void launch_normalize_cols(int64_t n, T* X, int ld, int nc, const double* sums, hipStream_t st);

@@ -1720,6 +1720,23 @@ void launch_residual_sumsq(int64_t n, const T* V, const T* W, int ld, int nc, co
   RCG_HIP(hipGetLastError());
 }
 
+__global__ __launch_bounds__(kThreads) void group_sqrt_kernel(const double* __restrict__ sums,
+                                                              int cw, int count,
+                                                              double* __restrict__ out) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= count) return;
+  double t = 0.0;
+  for (int e = 0; e < cw; ++e) t += sums[cw * c + e];
+  out[c] = sqrt(fabs(t));
+}
+
+void launch_group_sqrt(const double* sums, int cw, int count, double* out, hipStream_t st) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(group_sqrt_kernel, dim3((unsigned)((count + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, sums, cw, count, out);
+  RCG_HIP(hipGetLastError());
+}
+
 template <typename T>
 void launch_resid_norm_sumsq(int64_t n, const T* V, const T* W, int ld, int nc,
                              const double* theta, double* sums, double* ws, size_t ws_elems,

@@ -357,6 +357,37 @@ struct DeviceGuard {
 
 int64_t block_ld(int64_t cols) { return std::max<int64_t>(4, rcg::round_up(cols, 4)); }
 
+// ---------------------------------------------- no-allocation workspace
+// rcg_residuals_noalloc keeps its reduction partials, Ritz values and sums
+// in `scratch`, a file-scope device array of kNoAllocDoubles (16 MiB) that
+// each device holds for the life of the process.  In host code `scratch` is
+// only a stand-in for the symbol: its device address, one per device, comes
+// from hipGetSymbolAddress (made once per device when a context is created
+// there, with that device current) and only that address is used.
+constexpr size_t kNoAllocDoubles = size_t(2) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices)
+    throw Error(RCG_ERR_UNSUPPORTED, "device index beyond the no-allocation workspace table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    RCG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(RCG_ERR_HIP, "hipGetSymbolAddress returned no address for the workspace");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
 double wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -2488,6 +2519,7 @@ int rcg_context_create(int device, void* hip_stream, rcg_context* out) {
       c->own_stream = true;
     }
     c->bind_scratch();
+    (void)noalloc_scratch(device);
     *out = c.release();
   });
 }
@@ -2922,6 +2954,55 @@ int rcg_residuals(rcg_operator op, rcg_block V, rcg_block W, int64_t count, cons
     std::vector<double> r;
     e.residuals(count, theta, r);
     std::memcpy(res_host, r.data(), sizeof(double) * count);
+  });
+}
+
+int rcg_residuals_noalloc(rcg_operator op, rcg_block V, rcg_block W, int64_t count,
+                          const double* theta, double* res_host) {
+  return guarded([&] {
+    check_pair(op, V, W, count, true);
+    need(V != W && theta && res_host, "bad arguments");
+    if (op->is_callback())
+      throw Error(RCG_ERR_UNSUPPORTED, "rcg_residuals_noalloc takes CSR and dense operators");
+    if (count == 0) return;
+    DeviceGuard g(op->ctx->device);
+    double* base = noalloc_scratch(op->ctx->device);
+    const int cw = op->dtype == RCG_C128 ? 2 : 1;
+    const int64_t rc = cw * count;
+    const int64_t ld = real_ld(V);
+    const size_t part = rcg::colred_ws_elems(op->n, (int)rc);
+    if ((size_t)(3 * rc) + part > kNoAllocDoubles)
+      throw Error(RCG_ERR_INVALID,
+                  "rcg_residuals_noalloc: " + std::to_string(3 * rc + (int64_t)part) +
+                      " workspace doubles needed (3 per real column plus one per column and "
+                      "512 rows), the no-allocation workspace holds " +
+                      std::to_string(kNoAllocDoubles));
+    // every region below is written in full before it is read: nothing
+    // left by an earlier call is used
+    double* th = base;           // theta per real column
+    double* sums = base + rc;    // per real column
+    double* res = base + 2 * rc; // per column
+    double* ws = base + 3 * rc;  // partials
+    const hipStream_t st = op->ctx->stream;
+    if (cw == 1) {
+      RCG_HIP(hipMemcpyAsync(th, theta, sizeof(double) * count, hipMemcpyHostToDevice, st));
+    } else {
+      for (int e = 0; e < cw; ++e)
+        RCG_HIP(hipMemcpy2DAsync(th + e, sizeof(double) * cw, theta, sizeof(double),
+                                 sizeof(double), (size_t)count, hipMemcpyHostToDevice, st));
+    }
+    const double* v = static_cast<const double*>(V->d());
+    double* w = static_cast<double*>(W->d());
+    if (op->is_dense())
+      rcg::launch_dense_step<double>(op->n, op->dense, op->lda, v, nullptr, w, (int)ld, (int)count,
+                                     1.0, 0.0, 0.0, rcg::kStepApply, nullptr, nullptr, st);
+    else
+      rcg::launch_csr_apply<double>(op->n, op->rowptr, op->col, op->val, v, w, (int)ld, (int)rc,
+                                    1.0, 0.0, st, nullptr, cw == 2);
+    rcg::launch_residual_sumsq<double>(op->n, v, w, (int)ld, (int)rc, th, sums, ws, part, st);
+    rcg::launch_group_sqrt(sums, cw, (int)count, res, st);
+    RCG_HIP(hipMemcpyAsync(res_host, res, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+    RCG_HIP(hipStreamSynchronize(st));
   });
 }
 
