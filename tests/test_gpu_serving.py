@@ -65,6 +65,34 @@ def test_alone_equals_mix(rcg, precision):
     srv.close()
 
 
+@pytest.mark.skip(reason="the persistent one-launch sweep of a batch (tasks ordered by step, "
+                         "workgroups waiting on per-request completion counters) stalled the "
+                         "device on its first served request in two runs; it is not in the "
+                         "library, and RCG_BATCH_SWEEP selects nothing until it is rebuilt")
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_sweep_equals_per_step_launches(rcg, precision, monkeypatch):
+    """The persistent sweep (every step of a batch in one launch, tasks ordered
+    by step with per-request completion counters) computes each row exactly as
+    the per-step batched launches do: the served results agree as alone and in
+    the mix (1e-12)."""
+    reqs = small_requests(8, seed=21)
+    print(f"case: 8 served requests, persistent sweep vs per-step launches, {precision}", flush=True)
+    outs = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("RCG_BATCH_SWEEP", mode)
+        srv = serving.Server(max_active=8, window_us=300)
+        ops_, tickets = submit_all(srv, rcg, reqs, precision)
+        outs[mode] = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        assert st["batched_requests"] > st["filter_batches"]
+        srv.close()
+    for r, a, b in zip(reqs, outs["1"], outs["0"]):
+        assert a["status"] == 0 and b["status"] == 0, (a["error"], b["error"])
+        assert a["found"] == b["found"] == r.k
+        d = np.max(np.abs(a["eigenvalues"] - b["eigenvalues"]) / np.maximum(np.abs(a["eigenvalues"]), 0.1))
+        assert d <= 1e-12, (r, d)
+
+
 def test_served_results_vs_analytic_and_oracle(rcg, orc):
     print("case: served W=0 Dirichlet Laplacians vs closed forms and the oracle", flush=True)
     shapes = [(2, 40, 16), (3, 14, 8), (2, 64, 32), (3, 10, 16)]
