@@ -134,6 +134,29 @@ def cpu_baseline(A, gpu_counts, max_inner=1):
     }
 
 
+def step_ceiling(nx=GRID, iters=10):
+    """The FP64 step's memory pattern without the CSR operator (bench_ceiling.hip):
+    the same row reads and writes of the 64-column block alone ("stream") and
+    with the 7-point periodic gathers computed from the row index ("stencil"),
+    HIP-event timed.  Rates are the block streams' 3 n b 8 bytes per launch."""
+    import ctypes as C
+    path = os.path.join(ROOT, "bench_ceiling.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.ceiling_ms.restype = C.c_double
+    L.ceiling_ms.argtypes = [C.c_int, C.c_int64, C.c_int]
+    n = nx ** 3
+    out = {"block_bytes_per_launch": 3 * n * 64 * 8}
+    for kind, name in ((0, "stream"), (1, "stencil")):
+        ms = L.ceiling_ms(kind, nx, iters)
+        if ms <= 0:
+            return None
+        out[name] = {"us_per_launch": round(ms * 1e3, 1),
+                     "GBps": round(3 * n * 64 * 8 / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
 def cpu_model_check(grid=32):
     """The cpu_baseline model against a whole timed oracle solve on a smaller
     operator of the same family ({grid}^3 periodic Anderson, k 64, inc 32,
@@ -636,6 +659,9 @@ def main():
                     f"; the same model on a whole {chk['operator']} oracle solve estimates "
                     f"{chk['model_estimate_s']} s against {chk['measured_solve_s']} s measured "
                     f"(x{chk['model_over_measured']})")
+    if args.workload == "anderson128":
+        log("step-pattern ceiling kernels ...")
+        line["roofline"]["ceiling"] = step_ceiling(args.grid)
     if args.workload == "anderson128" and not args.no_serving:
         # the batched serving workload beside the single-operator solve: the
         # mixed-precision served stream against the plain FP64 path
