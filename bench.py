@@ -401,6 +401,46 @@ def serving_run(args, cpu=True, cpu_budget_s=15.0):
     }
 
 
+def batch_filter_bench(args):
+    """--workload batchfilter: the shared filter launches alone.  The first
+    --requests requests of the served stream (their operators, k + max(4, k/2)
+    FP64 columns) filtered together by rcg_cheb_filter_batch, degree 24 each,
+    HIP-synchronised wall time per call; the roofline is the requests' summed
+    algorithmic step bytes (CSR + 3 block passes per step) over that time."""
+    import ctypes as C
+    from rayleighchebyshevgpgpu_amd import capi, serving, solver
+    reqs = serving.serving_requests(args.requests, rate=args.rate, seed=4)
+    ctx = solver.Context(0)
+    cache = serving.OperatorCache(ctx)
+    m = 24
+    held = []
+    for r in reqs:
+        op = cache.operator(r)
+        cols = r.k + max(4, r.k // 2)
+        X, T = solver.Block(r.n, cols, ctx=ctx), solver.Block(r.n, cols, ctx=ctx)
+        X.randomize(0, cols, 1, 0)
+        held.append((r, op, X, T, cols))
+    n = len(held)
+    arr = lambda t, v: (t * n)(*v)  # noqa: E731
+    argv = (n, arr(C.c_void_p, [h[1].h.value for h in held]), arr(C.c_void_p, [h[2].h.value for h in held]),
+            arr(C.c_void_p, [h[3].h.value for h in held]), arr(C.c_int64, [h[4] for h in held]),
+            arr(C.c_int64, [m] * n), arr(C.c_double, [4.0 * h[0].dim + 4.1 for h in held]),
+            arr(C.c_double, [0.0] * n))
+    L = capi.lib()
+    for _ in range(args.warmup):
+        capi.check(L.rcg_cheb_filter_batch(*argv))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        capi.check(L.rcg_cheb_filter_batch(*argv))
+    dt = (time.perf_counter() - t0) / args.steps
+    alg = sum(m * (h[1].nnz * 12 + 4 * (h[0].n + 1) + 3 * h[0].n * h[4] * 8) for h in held)
+    rows = sum(h[0].n for h in held)
+    line = {"workload": f"shared filter launches: {n} served requests ({rows} rows), FP64, degree {m}",
+            "ms_per_call": round(dt * 1e3, 3), "steps_per_call": m,
+            "alg_GBps": round(alg / dt / 1e9, 1), "frac_of_8TBps": round(alg / dt / 8e12, 4)}
+    print(json.dumps(line), flush=True)
+
+
 def serving_bench(args):
     """--workload serving: the serving measurements as the bench line."""
     r = serving_run(args, cpu=not args.no_cpu)
@@ -436,7 +476,7 @@ def main():
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
-    ap.add_argument("--workload", default="anderson128", choices=sorted(WORKLOADS) + ["serving"])
+    ap.add_argument("--workload", default="anderson128", choices=sorted(WORKLOADS) + ["serving", "batchfilter"])
     ap.add_argument("--dense-n", type=int, default=16384, help="dense16384: matrix size")
     ap.add_argument("--requests", type=int, default=4096, help="serving: stream length")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
@@ -466,6 +506,8 @@ def main():
     global K, INC, BUF, TOL, WL
     if args.workload == "serving":
         return serving_bench(args)
+    if args.workload == "batchfilter":
+        return batch_filter_bench(args)
     WL = WORKLOADS[args.workload]
     K, INC, BUF, TOL = WL["k"], WL["inc"], WL["buf"], WL["tol"]
     from rayleighchebyshevgpgpu_amd import capi, solver

@@ -183,6 +183,17 @@ int rcg_operator_apply(rcg_operator op, rcg_block in, rcg_block out, double alph
  * written.  C128 blocks with a C128 operator (complex columns). */
 int rcg_cheb_filter(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols, int64_t degree,
                     int64_t repetition, double lambda_max, double shift);
+/* No counterpart in the reference (it filters one operator per call): the
+ * FP64 filter of rcg_cheb_filter for `count` CSR operators of one context in
+ * shared launches -- one per Chebyshev step and launch class over the 64-row
+ * blocks of every operator whose degree reaches that step -- as the server
+ * filters its concurrent solves.  For each q: X[q] <- P_m(A_q')^1 X[q] over
+ * cols[q] columns with m = degree[q], lambda_max[q], shift[q]; tmp[q] is the
+ * second recurrence buffer.  Each operator's result equals rcg_cheb_filter's
+ * on it, bit for bit.  Waits for the context's stream. */
+int rcg_cheb_filter_batch(int64_t count, const rcg_operator* ops, const rcg_block* X,
+                          const rcg_block* tmp, const int64_t* cols, const int64_t* degree,
+                          const double* lambda_max, const double* shift);
 /* No counterpart in the reference (its filter is FP64 only): the same filter
  * with FP32 arithmetic.  Columns with ritz[c] != 0 are split as
  * X_j = u_j(theta[c]) x + E_j (u_j the filter's scalar recurrence at the

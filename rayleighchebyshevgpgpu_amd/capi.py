@@ -95,6 +95,7 @@ def lib() -> C.CDLL:
         "rcg_operator_apply": ([vp, vp, vp, d, d, i64], i32),
         "rcg_cheb_filter": ([vp, vp, vp, i64, i64, i64, d, d], i32),
         "rcg_cheb_filter_mixed": ([vp, vp, vp, i64, i64, i64, d, d, vp, vp], i32),
+        "rcg_cheb_filter_batch": ([i64, vp, vp, vp, vp, vp, vp, vp], i32),
         "rcg_orthonormalize": ([vp, vp, i64], i32),
         "rcg_deflate": ([vp, i64, vp, i64], i32),
         "rcg_project": ([vp, vp, vp, i64, vp], i32),

@@ -357,6 +357,13 @@ struct DeviceGuard {
 
 int64_t block_ld(int64_t cols) { return std::max<int64_t>(4, rcg::round_up(cols, 4)); }
 
+// Rows per task of the batched filter steps: the served stream's operators
+// (n ~ 1e4) spread over twice the workgroups of 128-row tasks (served 1186 ->
+// 1236-1258 eigenpairs/s; 32-row tasks 1213-1229).  The shared launches of 8
+// and 32 served requests alone (rcg_cheb_filter_batch, FP64, degree 24) run
+// within 7% of each other for 32, 64 and 128 rows.
+constexpr int kBatchRows = 64;
+
 // ---------------------------------------------- no-allocation workspace
 // rcg_residuals_noalloc keeps its reduction partials, Ritz values and sums
 // in `scratch`, a file-scope device array of kNoAllocDoubles (16 MiB) that
@@ -999,10 +1006,7 @@ class Engine {
     d.ncv = (int)((rb + w - 1) / w);
     d.tail = (int)(rb - (int64_t)(d.ncv - 1) * w);
     d.m = (int)m;
-    // 64-row tasks: the served stream's operators (n ~ 1e4) spread over twice
-    // the workgroups of 128-row tasks (served 1186 -> 1236-1258 eigenpairs/s;
-    // 32-row tasks 1213-1229)
-    d.rows_per_blk = 64;
+    d.rows_per_blk = kBatchRows;
     d.g1 = g1;
     d.g2 = g2;
     return d;
@@ -2835,6 +2839,101 @@ int rcg_cheb_filter(rcg_operator op, rcg_block X, rcg_block tmp, int64_t cols, i
     Piece pc(X, tmp, op);
     pc.eng.filter(cols, (long)degree, (long)repetition, lambda_max, shift);
     pc.settle(X, cols);
+  });
+}
+
+int rcg_cheb_filter_batch(int64_t count, const rcg_operator* ops, const rcg_block* X,
+                          const rcg_block* tmp, const int64_t* cols, const int64_t* degree,
+                          const double* lambda_max, const double* shift) {
+  return guarded([&] {
+    need(count >= 1 && ops && X && tmp && cols && degree && lambda_max && shift, "bad arguments");
+    rcg_context_s* ctx = ops[0] ? ops[0]->ctx : nullptr;
+    need(ctx != nullptr, "NULL operator");
+    DeviceGuard g(ctx->device);
+    // descriptors in the order the launches want: one launch per step and
+    // launch class over the requests whose degree reaches the step
+    struct Req {
+      rcg::BatchDesc d;
+      int cls;
+      bool cx;
+      int64_t q;
+    };
+    std::vector<Req> reqs;
+    for (int64_t q = 0; q < count; ++q) {
+      rcg_operator op = ops[q];
+      check_pair(op, X[q], tmp[q], cols[q], true);
+      need(X[q] != tmp[q], "X and tmp must differ");
+      need(op->is_csr(), "the batched filter takes CSR operators");
+      need(op->ctx == ctx, "operators of one context");
+      need(degree[q] >= 1 && degree[q] <= INT32_MAX, "degree must be >= 1");
+      if (cols[q] == 0) continue;
+      const bool cx = op->dtype == RCG_C128;
+      const int64_t rb = (cx ? 2 : 1) * cols[q];
+      const auto c = rcg::cpoly::coefficients((long)degree[q], lambda_max[q], shift[q]);
+      rcg::BatchDesc d{};
+      d.rowptr = op->rowptr;
+      d.col = op->col;
+      d.val = op->val;
+      d.buf[0] = X[q]->d();
+      d.buf[1] = tmp[q]->d();
+      d.n = op->n;
+      d.ld = (int)real_ld(X[q]);
+      d.ncv = (int)((rb + 1) / 2);
+      d.tail = (int)(rb - (int64_t)(d.ncv - 1) * 2);
+      d.m = (int)degree[q];
+      d.rows_per_blk = kBatchRows;
+      d.g1 = c.g1;
+      d.g2 = c.g2;
+      reqs.push_back({d, rcg::csr_class(d.ncv), cx, q});
+    }
+    if (reqs.empty()) return;
+    std::stable_sort(reqs.begin(), reqs.end(), [](const Req& a, const Req& b) {
+      const int ka = (a.cx ? 100 : 0) + a.cls, kb = (b.cx ? 100 : 0) + b.cls;
+      if (ka != kb) return ka < kb;
+      return a.d.m > b.d.m;
+    });
+    std::vector<rcg::BatchDesc> descs;
+    std::vector<rcg::BatchTask> tasks;
+    std::vector<int> first(reqs.size() + 1, 0);
+    for (size_t i = 0; i < reqs.size(); ++i) {
+      descs.push_back(reqs[i].d);
+      first[i] = (int)tasks.size();
+      const int64_t nb = (reqs[i].d.n + kBatchRows - 1) / kBatchRows;
+      need((int64_t)tasks.size() + nb <= INT32_MAX, "too many row blocks");
+      for (int64_t b = 0; b < nb; ++b) tasks.push_back({(int)i, (int)b});
+    }
+    first[reqs.size()] = (int)tasks.size();
+    const hipStream_t st = ctx->stream;
+    DevBuf dd(st), dt(st);
+    dd.ensure(sizeof(rcg::BatchDesc) * descs.size());
+    dt.ensure(sizeof(rcg::BatchTask) * tasks.size());
+    RCG_HIP(hipMemcpyAsync(dd.p, descs.data(), sizeof(rcg::BatchDesc) * descs.size(),
+                           hipMemcpyHostToDevice, st));
+    RCG_HIP(hipMemcpyAsync(dt.p, tasks.data(), sizeof(rcg::BatchTask) * tasks.size(),
+                           hipMemcpyHostToDevice, st));
+    const int mmax = reqs[0].d.m > 0 ? std::max_element(reqs.begin(), reqs.end(), [](const Req& a,
+                                           const Req& b) { return a.d.m < b.d.m; })->d.m : 0;
+    for (int j = 1; j <= mmax; ++j)
+      for (size_t g0 = 0; g0 < reqs.size();) {
+        size_t g1 = g0;
+        while (g1 < reqs.size() && reqs[g1].cx == reqs[g0].cx && reqs[g1].cls == reqs[g0].cls) ++g1;
+        size_t e = g0;  // degree descending within the group: a prefix reaches step j
+        while (e < g1 && reqs[e].d.m >= j) ++e;
+        if (e > g0)
+          rcg::launch_cheb_batch(dd.as<rcg::BatchDesc>(), dt.as<rcg::BatchTask>() + first[g0],
+                                 first[e] - first[g0], j, false, false, reqs[g0].cx, reqs[g0].cls,
+                                 st);
+        g0 = g1;
+      }
+    // an odd degree leaves X_m in tmp
+    for (const Req& r : reqs)
+      if (r.d.m % 2 == 1) {
+        rcg_block b = X[r.q];
+        rcg::launch_copy_cols<double>(b->n, static_cast<const double*>(tmp[r.q]->d()),
+                                      (int)real_ld(b), 0, static_cast<double*>(b->d()),
+                                      (int)real_ld(b), 0, (int)((r.cx ? 2 : 1) * cols[r.q]), st);
+      }
+    RCG_HIP(hipStreamSynchronize(st));
   });
 }
 
