@@ -24,6 +24,7 @@ __global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __
   __shared__ int s_rp[kMaxRpb + 1];
   __shared__ int s_col[kStageNz];
   __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
+  const MixedStartArgs kNoStart{};  // the batched steps are never the mixed filter's start
   for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
     const BatchTask tk = tasks[t];
     const BatchDesc& d = descs[tk.req];
@@ -39,17 +40,17 @@ __global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __
       csr_rows<T, G, NV, kChebSrc, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, d.tail, g1,
                                           g2, T(1), i0, i1, static_cast<const T*>(d.R),
                                           static_cast<const T*>(d.kap) + (int64_t)j * d.ld, s_rp,
-                                          s_col, s_val);
+                                          s_col, s_val, kNoStart);
     } else if (j == 1) {
       csr_rows<T, G, NV, kChebFirst, 1, CX>(d.n, d.rowptr, d.col, val, b0, nullptr, b1, d.ld,
                                             d.ncv, d.tail, g1, g2, T(d.m == 1 ? 0.5 : 1.0), i0, i1,
-                                            nullptr, nullptr, s_rp, s_col, s_val);
+                                            nullptr, nullptr, s_rp, s_col, s_val, kNoStart);
     } else {
       T* xin = (j % 2 == 0) ? b1 : b0;
       T* xz = (j % 2 == 0) ? b0 : b1;
       const T sc = (j == d.m) ? T(1.0 / double(d.m + 1)) : T(1);
       csr_rows<T, G, NV, kCheb, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, d.tail, g1, g2,
-                                       sc, i0, i1, nullptr, nullptr, s_rp, s_col, s_val);
+                                       sc, i0, i1, nullptr, nullptr, s_rp, s_col, s_val, kNoStart);
     }
     __syncthreads();  // the LDS stages are reused by the next task
   }

@@ -116,7 +116,28 @@ struct V16<float> {
 // kChebSrc: kCheb plus a per-column multiple of a source block,
 //   out = p2 * (p1*(p0*A*X + X) - Z + kap[c] * R),
 // the correction recurrence of the mixed-precision filter.
-enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3, kChebSrc = 4 };
+// kApplyStart (FP64): W = A V, whose entries go straight into the mixed
+// filter's start (mixed_start_elem) instead of being stored.
+enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3, kChebSrc = 4, kApplyStart = 5 };
+
+// The mixed filter's start for one element, v = V[i][c] and w = (A V)[i][c]:
+// a Ritz column (flag != 0) gets R = w - theta v, E0 = 0, E1 = e1c R; a plain
+// column R = 0, E0 = v, E1 = g2 (g1 w + v).  Shared by the fused apply and
+// the stand-alone start kernel, so that both round alike.
+__device__ __forceinline__ void mixed_start_elem(double v, double w, double theta, double flag,
+                                                 double e1c, double g1, double g2, float& e0,
+                                                 float& e1, float& r) {
+  if (flag != 0.0) {
+    const double d = fma(-theta, v, w);
+    r = float(d);
+    e0 = 0.0f;
+    e1 = float(e1c * d);
+  } else {
+    r = 0.0f;
+    e0 = float(v);
+    e1 = float(g2 * fma(g1, w, v));
+  }
+}
 
 // Rows per workgroup block (cap) and nonzeros staged in LDS per block.
 constexpr int kMaxRpb = 512;
@@ -143,12 +164,25 @@ __device__ __forceinline__ void csr_rows(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
     int tail, T p0, T p1, T p2, int64_t i0, int64_t i1, const T* __restrict__ R,
-    const T* __restrict__ kap, int* s_rp, int* s_col, T* s_val) {
+    const T* __restrict__ kap, int* s_rp, int* s_col, T* s_val, const MixedStartArgs& ms) {
   using VT = V16<T>;
   constexpr int W = VT::W;
   constexpr int RS = kThreads / G;
+  static_assert(MODE != kApplyStart || std::is_same<T, double>::value, "the start is FP64");
   const int g = threadIdx.x % G;
   const int r = threadIdx.x / G;
+  // kApplyStart: this lane's columns' (theta, flag), the same for every row
+  double th[MODE == kApplyStart ? NV : 1][W], fl[MODE == kApplyStart ? NV : 1][W];
+  if constexpr (MODE == kApplyStart) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int c = (g + k * G) * W + w;
+        th[k][w] = c < ms.nc ? ms.cp[c] : 0.0;
+        fl[k][w] = c < ms.nc ? ms.cp[ms.nc + c] : 0.0;
+      }
+  }
   const int nr = (int)(i1 - i0);
   for (int t = threadIdx.x; t <= nr; t += kThreads) s_rp[t] = ld_once(rowptr + i0 + t);
   __syncthreads();
@@ -189,7 +223,8 @@ __device__ __forceinline__ void csr_rows(
       for (int k = 0; k < NV; ++k) {
         const int cv = g + k * G;
         if (cv < ncv) {
-          if constexpr (MODE == kChebFirst || MODE == kCheb || MODE == kChebSrc)
+          if constexpr (MODE == kChebFirst || MODE == kCheb || MODE == kChebSrc ||
+                        MODE == kApplyStart)
             VT::ld(X + rowoff + cv * W, xs[q][k]);
           if constexpr (MODE == kCheb || MODE == kChebSrc) VT::ld_stream(Z + rowoff + cv * W, zs[q][k]);
           if constexpr (MODE == kChebSrc) VT::ld_stream(R + rowoff + cv * W, rs[q][k]);
@@ -283,6 +318,29 @@ __device__ __forceinline__ void csr_rows(
       for (int k = 0; k < NV; ++k) {
         const int cv = g + k * G;
         if (cv >= ncv) continue;
+        if constexpr (MODE == kApplyStart) {
+          float e0[W], e1[W], rr[W];
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            mixed_start_elem(xs[q][k][w], p0 * acc[q][k][w], th[k][w], fl[k][w], ms.e1c, ms.g1,
+                             ms.g2, e0[w], e1[w], rr[w]);
+          const int64_t e = rowoff + cv * W;
+          if (tail < W && cv == ncv - 1) {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+              if (w < tail) {
+                ms.e0[e + w] = e0[w];
+                ms.e1[e + w] = e1[w];
+                ms.r[e + w] = rr[w];
+              }
+          } else {
+            typedef float fvec2 __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<fvec2*>(ms.e0 + e) = fvec2{e0[0], e0[1]};
+            *reinterpret_cast<fvec2*>(ms.e1 + e) = fvec2{e1[0], e1[1]};
+            *reinterpret_cast<fvec2*>(ms.r + e) = fvec2{rr[0], rr[1]};
+          }
+          continue;
+        }
         T o[W];
         if constexpr (MODE == kApply) {
 #pragma unroll

@@ -124,6 +124,20 @@ void launch_cheb_step_src(int64_t n, const int* rowptr, const int* col, const fl
                           bool cx = false);
 // E0, E1, R from V and AV (see the kernel); cp[c] = theta_c, cp[nc+c] = 1 for
 // Ritz columns, 0 for plain ones (E_j = X_j, R = 0).
+struct MixedStartArgs {
+  const double* cp = nullptr;  // theta_c at cp[c], the Ritz flag at cp[nc + c]
+  float* e0 = nullptr;
+  float* e1 = nullptr;
+  float* r = nullptr;
+  double e1c = 0.0, g1 = 0.0, g2 = 0.0;
+  int nc = 0;  // real columns of the call
+};
+// W = A V and the start above in one pass: the apply's epilogue turns each
+// (v, w) pair into the E0, E1, R entries, so W is never stored.  The results
+// are bit-identical to launch_csr_apply followed by launch_mixed_start.
+void launch_csr_apply_start(int64_t n, const int* rowptr, const int* col, const double* val,
+                            const double* V, int ld, int ncols, const MixedStartArgs& ms,
+                            hipStream_t st, const CsrSched* sched = nullptr, bool cx = false);
 void launch_mixed_start(int64_t n, const double* V, const double* AV, int ld, int nc,
                         const double* cp, double e1, double g1, double g2, float* E0, float* E1,
                         float* R, hipStream_t st);

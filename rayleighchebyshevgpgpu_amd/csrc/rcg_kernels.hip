@@ -28,7 +28,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
     int tail, T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk, const int* __restrict__ order,
-    int remap, const T* __restrict__ R, const T* __restrict__ kap) {
+    int remap, const T* __restrict__ R, const T* __restrict__ kap, MixedStartArgs ms) {
   __shared__ int s_rp[kMaxRpb + 1];
   __shared__ int s_col[kStageNz];
   __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
   const int64_t i0 = lb * rows_per_blk;
   const int64_t i1 = min(n, i0 + rows_per_blk);
   csr_rows<T, G, NV, MODE, RPI, CX>(n, rowptr, col, val, X, Z, out, ld, ncv, tail, p0, p1, p2, i0,
-                                    i1, R, kap, s_rp, s_col, s_val);
+                                    i1, R, kap, s_rp, s_col, s_val, ms);
 }
 
 #ifndef RCG_CSR_RPI
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
 template <typename T, int MODE, bool CX, int G, int NV>
 void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
                   T* out, int ld, int ncv, int tail, T p0, T p1, T p2, hipStream_t st,
-                  const CsrSched* sc, const T* R, const T* kap) {
+                  const CsrSched* sc, const T* R, const T* kap, const MixedStartArgs& ms) {
   constexpr int RS = kThreads / G;
   int64_t rows_per_blk, nblk;
   const int* order = nullptr;
@@ -80,20 +80,20 @@ void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T*
   const int64_t grid = round_up(nblk, kXcds);
   hipLaunchKernelGGL((csr_block_kernel<T, G, NV, MODE, RCG_CSR_RPI, CX>), dim3((unsigned)grid), dim3(kThreads), 0,
                      st, n, rp, ci, va, X, Z, out, ld, ncv, tail, p0, p1, p2, rows_per_blk, nblk, order,
-                     remap, R, kap);
+                     remap, R, kap, ms);
   RCG_HIP(hipGetLastError());
 }
 
 template <typename T, int MODE, bool CX>
 void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
                    T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
-                   const T* R, const T* kap) {
+                   const T* R, const T* kap, const MixedStartArgs& ms) {
   constexpr int W = V16<T>::W;
   const int ncv = (ncols + W - 1) / W;
   const int tail = ncols - (ncv - 1) * W;
   if (ld % 4 != 0) throw Error(1, "block leading dimension must be a multiple of 4");
 #define RCG_CSR_CASE(LIM, GG, NN) \
-  if (ncv <= LIM) return csr_launch_g<T, MODE, CX, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, tail, p0, p1, p2, st, sc, R, kap);
+  if (ncv <= LIM) return csr_launch_g<T, MODE, CX, GG, NN>(n, rp, ci, va, X, Z, out, ld, ncv, tail, p0, p1, p2, st, sc, R, kap, ms);
   // (16-byte vectors per row, lanes per row G, vectors per lane NV).  From 8
   // vectors on, each lane carries two or more vectors: half the lanes per row
   // of one vector each, twice the rows per wave (FP64 step at 128^3, b = 64:
@@ -122,12 +122,13 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
 template <typename T, int MODE>
 void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
                 T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
-                const T* R = nullptr, const T* kap = nullptr, bool cx = false) {
+                const T* R = nullptr, const T* kap = nullptr, bool cx = false,
+                const MixedStartArgs& ms = MixedStartArgs{}) {
   if (cx) {
     if (ncols % 2 != 0) throw Error(1, "complex blocks have an even number of real columns");
-    csr_launch_cx<T, MODE, true>(n, rp, ci, va, X, Z, out, ld, ncols, p0, p1, p2, st, sc, R, kap);
+    csr_launch_cx<T, MODE, true>(n, rp, ci, va, X, Z, out, ld, ncols, p0, p1, p2, st, sc, R, kap, ms);
   } else {
-    csr_launch_cx<T, MODE, false>(n, rp, ci, va, X, Z, out, ld, ncols, p0, p1, p2, st, sc, R, kap);
+    csr_launch_cx<T, MODE, false>(n, rp, ci, va, X, Z, out, ld, ncols, p0, p1, p2, st, sc, R, kap, ms);
   }
 }
 
@@ -1149,17 +1150,11 @@ __global__ __launch_bounds__(kThreads) void mixed_start_kernel(
   const int c = (int)(e % nc);
   if (i >= n) return;
   const int64_t k = i * ld + c;
-  const double v = V[k], w = AV[k];
-  if (cp[nc + c] != 0.0) {
-    const double r = w - cp[c] * v;
-    R[k] = float(r);
-    E0[k] = 0.0f;
-    E1[k] = float(e1 * r);
-  } else {
-    R[k] = 0.0f;
-    E0[k] = float(v);
-    E1[k] = float(g2 * fma(g1, w, v));
-  }
+  float e0, e1v, r;
+  mixed_start_elem(V[k], AV[k], cp[c], cp[nc + c], e1, g1, g2, e0, e1v, r);
+  R[k] = r;
+  E0[k] = e0;
+  E1[k] = e1v;
 }
 
 // V <- s * (um[c] * V + E), c < nc (E FP32).
@@ -1465,6 +1460,16 @@ void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val
   else
     csr_launch<T, kApplyBeta>(n, rowptr, col, val, X, nullptr, out, ld, ncols, alpha, beta, T(0), st,
                               sc, nullptr, nullptr, cx);
+}
+
+void launch_csr_apply_start(int64_t n, const int* rowptr, const int* col, const double* val,
+                            const double* V, int ld, int ncols, const MixedStartArgs& ms,
+                            hipStream_t st, const CsrSched* sc, bool cx) {
+  if (n <= 0 || ncols <= 0) return;
+  if (ms.nc != ncols || !ms.cp || !ms.e0 || !ms.e1 || !ms.r)
+    throw Error(1, "fused start: coefficient table and FP32 blocks must cover the call's columns");
+  csr_launch<double, kApplyStart>(n, rowptr, col, val, V, nullptr, nullptr, ld, ncols, 1.0, 0.0,
+                                  0.0, st, sc, nullptr, nullptr, cx, ms);
 }
 
 template <typename T>

@@ -1177,10 +1177,29 @@ class Engine {
     const double csr = dn ? 4.0 * double(n_) * double(n_)
                           : double(op_->nnz) * (4.0 + 4.0 * cw_) + 4.0 * double(n_ + 1);
     const double blk = 4.0 * double(n_) * double(rb);
+    // CSR operators: W = A V and the start in one pass (W is not stored);
+    // RCG_MIXED_UNFUSED=1 runs the apply and the start kernel separately
+    // (bit-identical results).
+    const char* unf = std::getenv("RCG_MIXED_UNFUSED");
+    const bool fused = !dn && !(unf && unf[0] == '1');
     for (long r = 0; r < rep; ++r) {
-      apply(V(), W_, b);
-      rcg::launch_mixed_start(n_, V(), W_, (int)ld_, (int)rb, dcp, g12, c.g1, c.g2, E_[0], E_[1],
-                              R32_, st_);
+      if (fused) {
+        rcg::MixedStartArgs ms;
+        ms.cp = dcp;
+        ms.e0 = E_[0];
+        ms.e1 = E_[1];
+        ms.r = R32_;
+        ms.e1c = g12;
+        ms.g1 = c.g1;
+        ms.g2 = c.g2;
+        ms.nc = (int)rb;
+        rcg::launch_csr_apply_start(n_, op_->rowptr, op_->col, op_->val, V(), (int)ld_, (int)rb,
+                                    ms, st_, sched_for(rb), cx_);
+      } else {
+        apply(V(), W_, b);
+        rcg::launch_mixed_start(n_, V(), W_, (int)ld_, (int)rb, dcp, g12, c.g1, c.g2, E_[0],
+                                E_[1], R32_, st_);
+      }
       if (batcher_ && !dn) {
         rcg::BatchDesc d = batch_desc(m, c.g1, c.g2, rb, 4);
         d.val = val32;

@@ -180,6 +180,53 @@ def test_autotuned_filter_vs_oracle(rcg, orc, mixed):
     assert np.array_equal(again, got)
 
 
+FUSED_CASES = {
+    # case: (operator, complex, block columns)
+    "real5": ("anderson16", False, 5),
+    "real13": ("anderson16", False, 13),
+    "real64": ("anderson16", False, 64),
+    "real100": ("anderson16", False, 100),
+    "cx3": ("hofstadter32", True, 3),
+    "cx17": ("hofstadter32", True, 17),
+    "tuned16": ("anderson64", False, 16),
+}
+
+
+@pytest.mark.parametrize("case", list(FUSED_CASES))
+def test_mixed_fused_start_bit_identical(rcg, case):
+    """The mixed filter's start computed in the FP64 apply's epilogue (the
+    default for CSR operators) against the apply and the start kernel run
+    separately (RCG_MIXED_UNFUSED=1): bit-identical outputs, for Ritz and
+    plain columns, odd widths (a partly used last vector), complex blocks and
+    the autotuned schedule (64^3 x 16 columns)."""
+    name, cx, cols = FUSED_CASES[case]
+    print(f"case: fused mixed start vs separate kernels, {case}", flush=True)
+    if name == "hofstadter32":
+        A = ops.hofstadter_2d(32, phi=1 / 8, W=1.0, seed=2)
+        lmax, shift = 8.5, 1.7
+    else:
+        A = anderson3d(16 if name == "anderson16" else 64)
+        lmax, shift = 14.5, 1.9
+    op = rcg.CsrOperator(A)
+    rng = np.random.default_rng(7)
+    X = rng.uniform(-1, 1, (A.n, cols))
+    if cx:
+        X = X + 1j * rng.uniform(-1, 1, (A.n, cols))
+    theta = rng.uniform(-1.0, 1.0, cols)
+    ritz = (np.arange(cols) % 3 != 2).astype(np.int32)
+    fused = run_filter(rcg, op, X, 7, lmax, shift, mixed=True, theta=theta, ritz=ritz)
+    old = os.environ.get("RCG_MIXED_UNFUSED")
+    os.environ["RCG_MIXED_UNFUSED"] = "1"
+    try:
+        apart = run_filter(rcg, op, X, 7, lmax, shift, mixed=True, theta=theta, ritz=ritz)
+    finally:
+        if old is None:
+            os.environ.pop("RCG_MIXED_UNFUSED", None)
+        else:
+            os.environ["RCG_MIXED_UNFUSED"] = old
+    assert np.array_equal(fused, apart)
+
+
 # FP64 widths reaching every (lanes per row, vectors per lane) instance:
 # 16-byte vectors per row 1, 2, 3-4, 5-8, 9-16, 17-32, 33-64, 65-128, 129-256
 @pytest.mark.parametrize("cols", [1, 4, 7, 16, 30, 64, 100, 130, 256, 300, 512])
