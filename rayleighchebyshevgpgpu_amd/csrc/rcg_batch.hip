@@ -58,6 +58,20 @@ __global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __
 
 }  // namespace
 
+// Launch classes of the batched step: lanes per row G = the row's 16-byte
+// vectors up to 64, one vector per lane; G need not divide the workgroup
+// (3, 6, 12, 24, 48), so no lane of a row sits idle at the served widths
+// (k + max(4, k/2) columns, k in {8, 16, 32}: 3, 6 and 12 FP32 vectors, 6,
+// 12 and 24 FP64).  Against power-of-two G with a quarter of the lanes idle:
+// 32 served operators filtered together (FP64, degree 24) 10.1 -> 8.9 ms,
+// the served stream 1203 -> 1254 eigenpairs/s.
+namespace {
+constexpr int kBatchLim[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 128, 256};
+constexpr int kBatchG[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 64, 64};
+}  // namespace
+// the launch switch below instantiates exactly these (G, vectors per lane)
+static_assert(sizeof(kBatchLim) == sizeof(kBatchG), "one G per class");
+
 template <typename T, bool SRC, bool CX>
 void cheb_batch_t(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, int cls,
                   hipStream_t st) {
@@ -70,17 +84,45 @@ void cheb_batch_t(const BatchDesc* descs, const BatchTask* tasks, int ntasks, in
   switch (cls) {
     RCG_BATCH_CASE(0, 1, 1)
     RCG_BATCH_CASE(1, 2, 1)
-    RCG_BATCH_CASE(2, 4, 1)
-    RCG_BATCH_CASE(3, 8, 1)
-    RCG_BATCH_CASE(4, 16, 1)
-    RCG_BATCH_CASE(5, 32, 1)
-    RCG_BATCH_CASE(6, 64, 1)
-    RCG_BATCH_CASE(7, 64, 2)
-    RCG_BATCH_CASE(8, 64, 4)
+    RCG_BATCH_CASE(2, 3, 1)
+    RCG_BATCH_CASE(3, 4, 1)
+    RCG_BATCH_CASE(4, 6, 1)
+    RCG_BATCH_CASE(5, 8, 1)
+    RCG_BATCH_CASE(6, 12, 1)
+    RCG_BATCH_CASE(7, 16, 1)
+    RCG_BATCH_CASE(8, 24, 1)
+    RCG_BATCH_CASE(9, 32, 1)
+    RCG_BATCH_CASE(10, 48, 1)
+    RCG_BATCH_CASE(11, 64, 1)
+    RCG_BATCH_CASE(12, 64, 2)
+    RCG_BATCH_CASE(13, 64, 4)
     default: throw Error(1, "bad CSR launch class");
   }
 #undef RCG_BATCH_CASE
   RCG_HIP(hipGetLastError());
+}
+
+int batch_class(int ncv) {
+  constexpr int nc = (int)(sizeof(kBatchLim) / sizeof(kBatchLim[0]));
+  for (int c = 0; c < nc; ++c)
+    if (ncv <= kBatchLim[c]) return c;
+  throw Error(6, "CSR block kernels support at most 256 16-byte column vectors per row");
+}
+
+// Rows per task: the served stream's operators (n ~ 1e4) spread over twice
+// the workgroups of 128-row tasks (served 1186 -> 1236-1258 eigenpairs/s;
+// 32-row tasks 1213-1229).  The shared launches of 8 and 32 served requests
+// alone (rcg_cheb_filter_batch, FP64, degree 24) run within 7% of each other
+// for 32, 64 and 128 rows.  When G does not divide the workgroup, whole
+// passes of kThreads / G rows nearest 64.
+constexpr int kBatchRows = 64;
+int batch_rows(int cls) {
+  constexpr int nc = (int)(sizeof(kBatchLim) / sizeof(kBatchLim[0]));
+  if (cls < 0 || cls >= nc) throw Error(1, "bad CSR launch class");
+  const int g = kBatchG[cls];
+  if (kThreads % g == 0) return kBatchRows;
+  const int rs = kThreads / g;
+  return rs * std::max(1, (kBatchRows + rs / 2) / rs);
 }
 
 void launch_cheb_batch(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, bool f32,

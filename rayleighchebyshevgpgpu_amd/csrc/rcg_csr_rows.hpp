@@ -201,8 +201,10 @@ __device__ __forceinline__ void csr_rows(
     }
   __syncthreads();
   // RPI rows per lane iteration: rows li and li + RS, ... so that each wave
-  // has RPI rows' gathers in flight at once.
-  for (int li0 = r; li0 < nr; li0 += RS * RPI) {
+  // has RPI rows' gathers in flight at once.  When G does not divide the
+  // workgroup, the last kThreads % G lanes own no row.
+  const int rfirst = (kThreads % G == 0 || r < RS) ? r : nr;
+  for (int li0 = rfirst; li0 < nr; li0 += RS * RPI) {
     int pb[RPI], pe[RPI];
     bool live_row[RPI];
 #pragma unroll

@@ -83,10 +83,12 @@ struct BatchTask {
   int req;
   int blk;
 };
-// Launch class (lanes per row, vectors per lane) of a request with ncv
-// 16-byte vectors per row; requests of one class share a launch.
-int csr_class(int ncv);
-// Step j of every task's request (all tasks of one csr_class); src: FP32
+// Launch class (lanes per row, one vector per lane) of a request with ncv
+// 16-byte vectors per row; requests of one class share a launch.  Tasks of
+// class cls span batch_rows(cls) rows.
+int batch_class(int ncv);
+int batch_rows(int cls);
+// Step j of every task's request (all tasks of one batch_class); src: FP32
 // correction steps (j >= 2).  Descriptors and tasks live in device memory.
 void launch_cheb_batch(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j,
                        bool f32, bool src, bool cx, int cls, hipStream_t st);

@@ -105,6 +105,10 @@ void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T
   // batched step (rcg_batch.hip) keeps one vector per lane: the same change
   // there served 1168-1182 against 1181-1186 eigenpairs/s (and, while both
   // kernels shared this file, made the single-operator FP64 step 3% slower).
+  // Lane counts that do not divide the workgroup (3, 6, 12, 24 lanes at 3-6,
+  // 9-12, 17-24, 65-96 vectors and so on), which the batched step uses, were
+  // flat or slower here: 1024^2 complex FP64 solve 29.7 -> 29.7, complex64
+  // mixed 33.5 -> 32.3 eigenpairs/s, plain FP64 solves of served requests +1.4%.
   RCG_CSR_CASE(1, 1, 1)
   RCG_CSR_CASE(2, 2, 1)
   RCG_CSR_CASE(4, 4, 1)
@@ -1567,13 +1571,6 @@ void launch_dense_step(int64_t n, const T* A, int64_t lda, const T* X, const T* 
   }
 #undef RCG_DENSE
   RCG_HIP(hipGetLastError());
-}
-
-int csr_class(int ncv) {
-  const int lim[] = {1, 2, 4, 8, 16, 32, 64, 128, 256};
-  for (int c = 0; c < 9; ++c)
-    if (ncv <= lim[c]) return c;
-  throw Error(6, "CSR block kernels support at most 256 16-byte column vectors per row");
 }
 
 template <typename T>

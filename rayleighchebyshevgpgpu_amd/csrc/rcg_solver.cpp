@@ -357,13 +357,6 @@ struct DeviceGuard {
 
 int64_t block_ld(int64_t cols) { return std::max<int64_t>(4, rcg::round_up(cols, 4)); }
 
-// Rows per task of the batched filter steps: the served stream's operators
-// (n ~ 1e4) spread over twice the workgroups of 128-row tasks (served 1186 ->
-// 1236-1258 eigenpairs/s; 32-row tasks 1213-1229).  The shared launches of 8
-// and 32 served requests alone (rcg_cheb_filter_batch, FP64, degree 24) run
-// within 7% of each other for 32, 64 and 128 rows.
-constexpr int kBatchRows = 64;
-
 // ---------------------------------------------- no-allocation workspace
 // rcg_residuals_noalloc keeps its reduction partials, Ritz values and sums
 // in `scratch`, a file-scope device array of kNoAllocDoubles (16 MiB) that
@@ -598,7 +591,7 @@ class FilterBatcher {
       const FilterReq* r = batch[q];
       if (d.n <= 0 || d.rows_per_blk <= 0 || d.rows_per_blk > rcg::kBatchMaxRows ||
           d.ncv <= 0 || d.tail < 1 || d.ld < 4 * d.ncv / (r->f32 ? 1 : 2) || d.m < 1 ||
-          r->jstart < 1 || (r->f32 && r->jstart < 2) || r->cls != rcg::csr_class(d.ncv) ||
+          r->jstart < 1 || (r->f32 && r->jstart < 2) || r->cls != rcg::batch_class(d.ncv) ||
           !d.rowptr || !d.col || !d.val || !d.buf[0] || !d.buf[1] ||
           (r->src && (!d.R || !d.kap)))
         throw Error(RCG_ERR_INVALID, "malformed batched filter request");
@@ -985,7 +978,7 @@ class Engine {
     r.src = f32;
     r.cx = cx;
     r.jstart = jstart;
-    r.cls = rcg::csr_class(d.ncv);
+    r.cls = rcg::batch_class(d.ncv);
     {
       const double vs = (f32 ? 4.0 : 8.0) * (cx ? 2.0 : 1.0);  // bytes per matrix value
       const double xs = f32 ? 4.0 : 8.0;                       // bytes per block element
@@ -1006,7 +999,7 @@ class Engine {
     d.ncv = (int)((rb + w - 1) / w);
     d.tail = (int)(rb - (int64_t)(d.ncv - 1) * w);
     d.m = (int)m;
-    d.rows_per_blk = kBatchRows;
+    d.rows_per_blk = rcg::batch_rows(rcg::batch_class(d.ncv));
     d.g1 = g1;
     d.g2 = g2;
     return d;
@@ -2900,10 +2893,11 @@ int rcg_cheb_filter_batch(int64_t count, const rcg_operator* ops, const rcg_bloc
       d.ncv = (int)((rb + 1) / 2);
       d.tail = (int)(rb - (int64_t)(d.ncv - 1) * 2);
       d.m = (int)degree[q];
-      d.rows_per_blk = kBatchRows;
+      const int cls = rcg::batch_class(d.ncv);
+      d.rows_per_blk = rcg::batch_rows(cls);
       d.g1 = c.g1;
       d.g2 = c.g2;
-      reqs.push_back({d, rcg::csr_class(d.ncv), cx, q});
+      reqs.push_back({d, cls, cx, q});
     }
     if (reqs.empty()) return;
     std::stable_sort(reqs.begin(), reqs.end(), [](const Req& a, const Req& b) {
@@ -2917,7 +2911,7 @@ int rcg_cheb_filter_batch(int64_t count, const rcg_operator* ops, const rcg_bloc
     for (size_t i = 0; i < reqs.size(); ++i) {
       descs.push_back(reqs[i].d);
       first[i] = (int)tasks.size();
-      const int64_t nb = (reqs[i].d.n + kBatchRows - 1) / kBatchRows;
+      const int64_t nb = (reqs[i].d.n + reqs[i].d.rows_per_blk - 1) / reqs[i].d.rows_per_blk;
       need((int64_t)tasks.size() + nb <= INT32_MAX, "too many row blocks");
       for (int64_t b = 0; b < nb; ++b) tasks.push_back({(int)i, (int)b});
     }

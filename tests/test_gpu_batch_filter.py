@@ -28,7 +28,7 @@ def arr(ptype, items):
 
 
 def test_batched_filter_equals_single_and_oracle(rcg, orc):
-    print("case: batched filter of 6 operators vs rcg_cheb_filter and the oracle", flush=True)
+    print("case: batched filter of 9 operators vs rcg_cheb_filter and the oracle", flush=True)
     rng = np.random.default_rng(5)
     specs = [  # operator, columns, degree, lambda_max, shift
         (ops.laplacian_3d(12, bc="periodic", potential=ops.anderson_potential(1728, 4.0, 1)), 24, 7, 14.5, 1.5),
@@ -37,6 +37,10 @@ def test_batched_filter_equals_single_and_oracle(rcg, orc):
         (ops.laplacian_3d(9, bc="dirichlet"), 48, 20, 12.2, -0.3),
         (ops.hofstadter_2d(16, phi=1.0 / 8, W=1.0, seed=2), 10, 9, 9.0, 4.1),
         (ops.laplacian_2d(30), 0, 5, 8.1, 0.0),
+        # lanes per row that do not divide the workgroup: 3, 48 (real), 24 (complex)
+        (ops.laplacian_2d(37, potential=ops.anderson_potential(1369, 1.0, 4)), 5, 6, 8.6, 0.3),
+        (ops.laplacian_3d(7, bc="dirichlet"), 80, 4, 12.1, 0.0),
+        (ops.hofstadter_2d(12, phi=1.0 / 4, W=0.5, seed=3), 20, 5, 8.6, 4.0),
     ]
     held, Xs, Ts, refs = [], [], [], []
     for A, cols, m, lmax, shift in specs:

@@ -229,7 +229,9 @@ def test_mixed_fused_start_bit_identical(rcg, case):
 
 # FP64 widths reaching every (lanes per row, vectors per lane) instance:
 # 16-byte vectors per row 1, 2, 3-4, 5-8, 9-16, 17-32, 33-64, 65-128, 129-256
-@pytest.mark.parametrize("cols", [1, 4, 7, 16, 30, 64, 100, 130, 256, 300, 512])
+# (and partly used vectors in each)
+@pytest.mark.parametrize("cols", [1, 4, 5, 7, 11, 16, 23, 30, 40, 64, 90, 100, 130, 190, 256,
+                                  300, 380, 512])
 def test_fp64_step_every_width(rcg, orc, cols):
     print(f"case: FP64 filter degree 6, {cols} columns", flush=True)
     A = anderson3d(12)
