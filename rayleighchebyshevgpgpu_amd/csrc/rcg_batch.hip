@@ -64,7 +64,9 @@ __global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __
 // (k + max(4, k/2) columns, k in {8, 16, 32}: 3, 6 and 12 FP32 vectors, 6,
 // 12 and 24 FP64).  Against power-of-two G with a quarter of the lanes idle:
 // 32 served operators filtered together (FP64, degree 24) 10.1 -> 8.9 ms,
-// the served stream 1203 -> 1254 eigenpairs/s.
+// the served stream 1203 -> 1254 eigenpairs/s.  Two vectors per lane on
+// half the lanes (3x2, 6x2, 12x2, ...) lose again: 8.8 -> 9.6 ms, 1245 ->
+// 1172 eigenpairs/s.
 namespace {
 constexpr int kBatchLim[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 128, 256};
 constexpr int kBatchG[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 64, 64};
