@@ -221,11 +221,49 @@ def cpu_baseline_dense(A, gpu_counts, b):
     }
 
 
+def kernel_only_dense(args):
+    """The dense workload's FP64 (or, with --kernel-mixed, FP32 correction)
+    MFMA filter steps alone, on its 256-column block, for rocprofv3 counter
+    runs.  The matrix comes from the cache file dense_operator_host writes, so
+    run the dense workload once (without a profiler) first."""
+    from rayleighchebyshevgpgpu_amd import capi, solver
+    Ad, _ = dense_operator_host(args.dense_n)
+    ctx = solver.Context(0)
+    op = solver.DenseOperator(np.asarray(Ad), ctx)
+    b = INC + BUF
+    X = solver.Block(op.n, b, ctx=ctx)
+    T = solver.Block(op.n, b, ctx=ctx)
+    X.randomize(0, b, 0, 0)
+    m = args.kernel_only + (args.kernel_only % 2)
+    theta = np.full(b, 0.01)
+    ritz = np.ones(b, dtype=np.int32)
+
+    def run():
+        if args.kernel_mixed:
+            capi.check(capi.lib().rcg_cheb_filter_mixed(op.h, X.h, T.h, b, m, 1, 1.001, -0.001,
+                                                        theta.ctypes.data, ritz.ctypes.data))
+        else:
+            capi.check(capi.lib().rcg_cheb_filter(op.h, X.h, T.h, b, m, 1, 1.001, -0.001))
+        ctx.synchronize()
+    run()  # warm-up (FP32 copy of A for the mixed steps)
+    t0 = time.perf_counter()
+    run()
+    dt = time.perf_counter() - t0
+    nn = float(op.n)
+    print(json.dumps({"mode": "kernel-only", "workload": "dense16384",
+                      "precision": "mixed" if args.kernel_mixed else "fp64", "cheb_steps": m,
+                      "n": op.n, "b": b, "flops_per_step": 2.0 * nn * nn * b,
+                      "ms_per_step": round(dt / m * 1e3, 4),
+                      "tflops": round(2.0 * nn * nn * b * m / dt / 1e12, 2)}), flush=True)
+
+
 def kernel_only(args):
     """Filter steps alone (for rocprofv3 counter runs) plus a device-to-device
     copy of the 64-column block (a plain copy kernel of known size) whose
     FETCH_SIZE/WRITE_SIZE calibrate the counters."""
     from rayleighchebyshevgpgpu_amd import capi, solver
+    if WL.get("dense"):
+        return kernel_only_dense(args)
     A = build_operator(args.grid)
     ctx = solver.Context(0)
     op = solver.CsrOperator(A, ctx)
