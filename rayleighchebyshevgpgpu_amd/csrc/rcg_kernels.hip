@@ -836,6 +836,13 @@ struct DenseCfg {
   static constexpr int BM = sizeof(T) == 8 ? 128 : 64;
   static constexpr int BN = 64;
   static constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+  // LDS row padding (elements) of the A and X tiles.  FP32: rows of 34 and 80
+  // floats put the 16 rows x 2 K of a half-wave's A fragment reads, and the
+  // 16 columns x 2 K of its X reads, in distinct banks; with 33 and 65 bank
+  // conflicts took 25% of the step's cycles, but removing them gains only 1%
+  // (1403 -> 1384 us at n = 16384, b = 256): the MFMA issue, not LDS, binds.
+  static constexpr int PA = sizeof(T) == 8 ? 1 : 2;
+  static constexpr int PX = sizeof(T) == 8 ? 1 : 16;
 };
 template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads, 2) void dense_step_kernel(
@@ -848,8 +855,8 @@ __global__ __launch_bounds__(kThreads, 2) void dense_step_kernel(
   constexpr int XE = BK * BN / kThreads;  // X elements per thread per chunk
   constexpr int WR = BM / 4;              // rows per wave
   constexpr int NA = WR / 16, NB = BN / 16;
-  __shared__ T As[2][BM][BK + 1];
-  __shared__ T Xs[2][BK][BN + 1];
+  __shared__ T As[2][BM][BK + DenseCfg<T>::PA];
+  __shared__ T Xs[2][BK][BN + DenseCfg<T>::PX];
   using Acc = typename MfmaAcc<T>::type;
   const int64_t L = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t ntm = (n + BM - 1) / BM;
