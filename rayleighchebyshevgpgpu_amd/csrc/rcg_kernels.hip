@@ -833,6 +833,7 @@ struct DenseCfg {
   // FP32 runs 64-row tiles (16 rows per wave): at n = 16384, b = 256 that is
   // 1024 workgroups instead of 512, 4.5% faster per step than 128 rows; 32-column
   // tiles, K chunks of 16 or 64, and 32x32 per wave were slower
+  // (FP64 in 64-row tiles too, 4 waves per SIMD: 2.44 -> 2.69 ms per step)
   static constexpr int BM = sizeof(T) == 8 ? 128 : 64;
   static constexpr int BN = 64;
   static constexpr int BK = sizeof(T) == 8 ? 16 : 32;
