@@ -451,7 +451,8 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
 // Each workgroup owns one 64x64 output tile (enumerated as in gram_mfma_kernel)
 // and one row block; 32-row chunks of X[:, a0:a0+64] and Y[:, b0:b0+64] are
 // loaded with 16-byte loads one chunk ahead and staged in double-buffered LDS
-// (one barrier per chunk).  Wave w owns the 32x32 quadrant (w & 1, w >> 1) of
+// (one barrier per chunk; loads two chunks ahead, as in the dense step, were
+// 5% faster for X^T X and 3-5% slower for the other two forms at 1024^2 x 192).  Wave w owns the 32x32 quadrant (w & 1, w >> 1) of
 // the tile, so no cross-wave reduction is needed; in a diagonal tile of X^T X
 // the quadrant below the diagonal is the mirror of the one above and its wave
 // idles.  Row blocks are sized so that all workgroups are resident at once.
