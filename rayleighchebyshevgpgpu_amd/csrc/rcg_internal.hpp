@@ -166,6 +166,11 @@ void launch_dense_step(int64_t n, const T* A, int64_t lda, const T* X, const T* 
 template <typename T>
 void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, int by, T* G,
                  T* ws, size_t ws_elems, hipStream_t st, bool lower_only = false);
+// G = X^T X and, right after it (G + b*b), X^T Y, b <= 64 columns, in one
+// launch and one reduction: the same values as two launch_gram calls.
+// Needs 2 gram_ws_elems(n, b, b) of workspace.
+void launch_gram_pair(int64_t n, const double* X, int ldx, const double* Y, int ldy, int b,
+                      double* G, double* ws, size_t ws_elems, hipStream_t st);
 template <typename T>
 size_t gram_ws_elems(int64_t n, int bx, int by);
 // Bound of gram_ws_elems over all widths <= bmax.

@@ -381,17 +381,21 @@ __device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[4][4], const dou
 // callers that read only the lower triangle of X^H Y (the Hermitian
 // projection); the upper tiles get the transposed lower values so that every
 // entry of the result is defined.  kGramDiag: the one tile of X^T X when
-// bx <= 64.
-enum { kGramFull = 0, kGramSym = 1, kGramDiag = 2, kGramLower = 3 };
+// bx <= 64.  kGramPair (bx == by <= 64): X^T X and X^T Y in one launch, tile 0
+// and tile 1 of each row block, with the partials of both side by side.
+enum { kGramFull = 0, kGramSym = 1, kGramDiag = 2, kGramLower = 3, kGramPair = 4 };
 
 // Partial Gram matrices of one 64x64 tile, one bx x by block per row block:
 // the one-tile launches (kGramFull with bx, by <= 64, or kGramDiag, compiled
-// without the general path so that it keeps two waves per SIMD).
+// without the general path so that it keeps two waves per SIMD; kGramPair
+// runs either per workgroup).  A row block's partials land at
+// P + (rblk * ntiles + tile) * bx * by.
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ Y, int ldy,
     int by, double* __restrict__ P, int64_t rows_per_blk, int nta, int ntiles, int64_t nblk) {
-  static_assert(MODE == kGramFull || MODE == kGramDiag, "one-tile launches only");
+  static_assert(MODE == kGramFull || MODE == kGramDiag || MODE == kGramPair,
+                "one-tile launches only");
   __shared__ double red[kTile * kTile];
   // one 1-D grid: the tiles of a row block are consecutive logical workgroups,
   // which the remap keeps on one XCD, so its rows are read from HBM once
@@ -399,9 +403,10 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   if (L >= nblk * ntiles) return;
   const int tile = (int)(L % ntiles);
   const int64_t rblk = L / ntiles;
-  const int ta = tile % nta;
-  const int tb = tile / nta;
-  constexpr bool diag = MODE == kGramDiag;
+  const int ta = MODE == kGramPair ? 0 : tile % nta;
+  const int tb = MODE == kGramPair ? 0 : tile / nta;
+  // uniform per workgroup: kGramPair's tile 0 is X^T X
+  const bool diag = MODE == kGramDiag || (MODE == kGramPair && tile == 0);
   const int a0 = ta * kTile;
   const int b0 = tb * kTile;
   const int lane = threadIdx.x & 63;
@@ -415,7 +420,15 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
-  gram_tile_accumulate<diag>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
+  if constexpr (MODE == kGramPair) {
+    if (diag)
+      gram_tile_accumulate<true>(acc, X, ldx, bx, X, ldx, bx, a0, b0, i0 + 4 * wave, i1, kr, cc);
+    else
+      gram_tile_accumulate<false>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
+  } else {
+    gram_tile_accumulate<MODE == kGramDiag>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave,
+                                            i1, kr, cc);
+  }
   // the four waves' tiles are summed in LDS in wave order
   for (int w = 0; w < 4; ++w) {
     if (wave == w) {
@@ -433,14 +446,15 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     }
     __syncthreads();
   }
-  double* Pb = P + rblk * (int64_t)bx * by;
+  double* Pb = P + (rblk * (MODE == kGramPair ? ntiles : 1) + (MODE == kGramPair ? tile : 0)) *
+                      (int64_t)bx * by;
   for (int e = threadIdx.x; e < kTile * kTile; e += kThreads) {
     const int al = e % kTile;
     const int bl = e / kTile;
     const int a = a0 + al;
     const int b = b0 + bl;
     if (a >= bx || b >= by) continue;
-    if constexpr (diag)
+    if (diag)
       Pb[a + (int64_t)b * bx] = (al / 16 <= bl / 16) ? red[al + kTile * bl] : red[bl + kTile * al];
     else
       Pb[a + (int64_t)b * bx] = red[al + kTile * bl];
@@ -1664,6 +1678,28 @@ void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, in
   const int64_t m = (int64_t)bx * by;
   hipLaunchKernelGGL((reduce_partials_kernel<T, T>), dim3(reduce_grid(m)), dim3(kThreads), 0, st, ws,
                      nblk, m, G);
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_gram_pair(int64_t n, const double* X, int ldx, const double* Y, int ldy, int b,
+                      double* G, double* ws, size_t ws_elems, hipStream_t st) {
+  if (b <= 0) return;
+  if (b > kTile) throw Error(1, "paired Gram products take at most 64 columns");
+  if (n <= 0) {
+    RCG_HIP(hipMemsetAsync(G, 0, sizeof(double) * 2 * b * b, st));
+    return;
+  }
+  // the row blocks of the single launches, so that the partials and their
+  // sums are the same
+  int64_t rpb;
+  const int64_t nblk = gram_blocks<double>(n, b, b, &rpb);
+  if ((size_t)(2 * nblk) * b * b > ws_elems) throw Error(3, "gram workspace too small");
+  hipLaunchKernelGGL(gram_mfma_kernel<kGramPair>, dim3((unsigned)round_up(2 * nblk, kXcds)),
+                     dim3(kThreads), 0, st, n, X, ldx, b, Y, ldy, b, ws, rpb, 1, 2, nblk);
+  RCG_HIP(hipGetLastError());
+  const int64_t m = 2 * (int64_t)b * b;
+  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(reduce_grid(m)), dim3(kThreads),
+                     0, st, ws, nblk, m, G);
   RCG_HIP(hipGetLastError());
 }
 

@@ -733,6 +733,9 @@ class Engine {
   void reserve(int64_t bmax, int64_t kmax) {
     bb_ = std::max<int64_t>({bmax, kmax, 4});
     size_t e = rcg::gram_ws_bound(n_, (int)bb_);
+    // ortho_project's paired Gram products (at most kFoldCols real columns)
+    const int fold = (int)std::min<int64_t>(kFoldCols, cw_ * bb_);
+    e = std::max(e, 2 * rcg::gram_ws_elems<double>(n_, fold, fold));
     e = std::max(e, rcg::colred_ws_elems(n_, 2 * (int)bb_));
     e = std::max(e, rcg::lz_ws_elems(n_));
     ctx_->ws.ensure(e * sizeof(double));
@@ -1457,11 +1460,22 @@ class Engine {
     }
     apply(V(), W_, b);
     const int64_t rb = cw_ * b;
-    rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, V(), (int)ld_, (int)rb, dmatA(), ws(),
-                             ws_elems_, st_);
-    // H1 right after G, so that one copy brings both back
-    rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, W_, (int)ld_, (int)rb, dmatA() + rb * rb,
-                             ws(), ws_elems_, st_, true);
+    // G = V^H V and H1 = V^H W right after it, so that one copy brings both
+    // back.  Below 2^22 block entries both products go in one paired launch
+    // (the same values; plain solves of served-size requests 568 -> 586
+    // eigenpairs/s); above, the paired kernel's one wave per SIMD costs more
+    // than the launch it saves (128^3 x 64 solve 1987 -> 2003 ms).
+    // RCG_GRAM_UNPAIRED=1 always runs them separately.
+    const char* unp = std::getenv("RCG_GRAM_UNPAIRED");
+    if ((unp && unp[0] == '1') || n_ * rb >= (int64_t(1) << 22)) {
+      rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, V(), (int)ld_, (int)rb, dmatA(), ws(),
+                               ws_elems_, st_);
+      rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, W_, (int)ld_, (int)rb,
+                               dmatA() + rb * rb, ws(), ws_elems_, st_, true);
+    } else {
+      rcg::launch_gram_pair(n_, V(), (int)ld_, W_, (int)ld_, (int)rb, dmatA(), ws(), ws_elems_,
+                            st_);
+    }
     RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * 2 * rb * rb, hipMemcpyDeviceToHost,
                            st_));
     sync();

@@ -227,6 +227,46 @@ def test_mixed_fused_start_bit_identical(rcg, case):
     assert np.array_equal(fused, apart)
 
 
+@pytest.mark.parametrize("kind", ["real", "complex", "mixed"])
+def test_paired_gram_bit_identical(rcg, kind):
+    """The projection's two Gram products (V^H V and V^H AV) in one paired launch
+    (the default below 2^22 block entries) against two separate launches
+    (RCG_GRAM_UNPAIRED=1): the whole solve, eigenvalues and eigenvectors, is
+    bit-identical."""
+    print(f"case: paired vs separate Gram products, {kind} solve", flush=True)
+    if kind == "complex":
+        A = ops.hofstadter_2d(24, phi=1 / 8, W=1.0, seed=2)
+        k, inc, buf = 12, 8, 8
+    else:
+        A = anderson3d(14)
+        k, inc, buf = 24, 16, 16
+    op = rcg.CsrOperator(A)
+
+    def solve():
+        rc = rcg.RayleighChebyshevLM()
+        rc.setStopCondition("RESIDUAL_ONLY")
+        if kind == "mixed":
+            rc.filterPrecision = np.float32
+        V = rcg.Block(A.n, inc + buf, dtype=np.complex128 if A.is_complex else np.float64)
+        ev = []
+        f = rc.getMinEigenSystem(k, 1e-10, inc, buf, None, op, rcg.RandomizeOp(0), ev, V)
+        return f, np.array(ev), V.download(f)
+
+    f1, ev1, X1 = solve()
+    old = os.environ.get("RCG_GRAM_UNPAIRED")
+    os.environ["RCG_GRAM_UNPAIRED"] = "1"
+    try:
+        f2, ev2, X2 = solve()
+    finally:
+        if old is None:
+            os.environ.pop("RCG_GRAM_UNPAIRED", None)
+        else:
+            os.environ["RCG_GRAM_UNPAIRED"] = old
+    assert f1 == f2 == k
+    assert np.array_equal(ev1, ev2)
+    assert np.array_equal(X1, X2)
+
+
 # FP64 widths reaching every (lanes per row, vectors per lane) instance:
 # 16-byte vectors per row 1, 2, 3-4, 5-8, 9-16, 17-32, 33-64, 65-128, 129-256
 # (and partly used vectors in each)
