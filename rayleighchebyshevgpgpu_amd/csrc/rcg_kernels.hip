@@ -320,8 +320,8 @@ __device__ __forceinline__ d4v mfma16x16x4(double a, double b, d4v c) {
 // which are exactly the A and B fragments of the 16 MFMA subtiles.  DIAG (a
 // tile on the diagonal of X^T X) multiplies only subtiles p <= q and needs
 // no Y loads.
-template <bool DIAG>
-__device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[4][4], const double* __restrict__ X,
+template <bool DIAG, int NT>
+__device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[NT][NT], const double* __restrict__ X,
                                                      int ldx, int bx, const double* __restrict__ Y,
                                                      int ldy, int by, int a0, int b0, int64_t ib,
                                                      int64_t i1, int kr, int cc) {
@@ -329,20 +329,20 @@ __device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[4][4], const dou
   // MFMAs that consume them, and the next pass's loads before this pass's
   // MFMAs
   constexpr int U = 4;
-  double xa[U][4], yb[U][4], xn[U][4], yn[U][4];
-  auto load = [&](int64_t jb, double (&xv)[U][4], double (&yv)[U][4]) {
+  double xa[U][NT], yb[U][NT], xn[U][NT], yn[U][NT];
+  auto load = [&](int64_t jb, double (&xv)[U][NT], double (&yv)[U][NT]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t row = jb + 16 * u + kr;
       const bool ok = row < i1;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < NT; ++t) {
         const int c = a0 + 16 * t + cc;
         xv[u][t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
       }
       if constexpr (!DIAG) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < NT; ++t) {
           const int c = b0 + 16 * t + cc;
           yv[u][t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
         }
@@ -356,9 +356,9 @@ __device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[4][4], const dou
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
+      for (int p = 0; p < NT; ++p)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NT; ++q) {
           if (DIAG && p > q) continue;
           acc[p][q] = mfma16x16x4(xa[u][p], DIAG ? xa[u][q] : yb[u][q], acc[p][q]);
         }
@@ -366,7 +366,7 @@ __device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[4][4], const dou
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < NT; ++t) {
           xa[u][t] = xn[u][t];
           if constexpr (!DIAG) yb[u][t] = yn[u][t];
         }
@@ -389,8 +389,10 @@ enum { kGramFull = 0, kGramSym = 1, kGramDiag = 2, kGramLower = 3, kGramPair = 4
 // the one-tile launches (kGramFull with bx, by <= 64, or kGramDiag, compiled
 // without the general path so that it keeps two waves per SIMD; kGramPair
 // runs either per workgroup).  A row block's partials land at
-// P + (rblk * ntiles + tile) * bx * by.
-template <int MODE>
+// P + (rblk * ntiles + tile) * bx * by.  NT = ceil(max(bx, by) / 16): only
+// the NT x NT 16x16 subtiles that hold output are multiplied, with registers
+// for those alone (the sums of the others were zero).
+template <int MODE, int NT>
 __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ Y, int ldy,
     int by, double* __restrict__ P, int64_t rows_per_blk, int nta, int ntiles, int64_t nblk) {
@@ -415,27 +417,29 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
   const int cc = lane & 15;
   const int64_t i0 = rblk * rows_per_blk;
   const int64_t i1 = min(n, i0 + rows_per_blk);
-  d4v acc[4][4];
+  d4v acc[NT][NT];
 #pragma unroll
-  for (int p = 0; p < 4; ++p)
+  for (int p = 0; p < NT; ++p)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < NT; ++q) acc[p][q] = d4v{0.0, 0.0, 0.0, 0.0};
   if constexpr (MODE == kGramPair) {
     if (diag)
-      gram_tile_accumulate<true>(acc, X, ldx, bx, X, ldx, bx, a0, b0, i0 + 4 * wave, i1, kr, cc);
+      gram_tile_accumulate<true, NT>(acc, X, ldx, bx, X, ldx, bx, a0, b0, i0 + 4 * wave, i1, kr,
+                                     cc);
     else
-      gram_tile_accumulate<false>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr, cc);
+      gram_tile_accumulate<false, NT>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave, i1, kr,
+                                      cc);
   } else {
-    gram_tile_accumulate<MODE == kGramDiag>(acc, X, ldx, bx, Y, ldy, by, a0, b0, i0 + 4 * wave,
-                                            i1, kr, cc);
+    gram_tile_accumulate<MODE == kGramDiag, NT>(acc, X, ldx, bx, Y, ldy, by, a0, b0,
+                                                i0 + 4 * wave, i1, kr, cc);
   }
   // the four waves' tiles are summed in LDS in wave order
   for (int w = 0; w < 4; ++w) {
     if (wave == w) {
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
+      for (int p = 0; p < NT; ++p)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NT; ++q) {
           if (diag && p > q) continue;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
@@ -1627,6 +1631,27 @@ size_t gram_ws_bound(int64_t n, int bmax) {
   return (size_t)std::max<int64_t>(1, std::min(by_tiles, by_rows));
 }
 
+// 16x16 subtiles per side of a one-tile Gram launch
+inline int gram_nt(int bx, int by) { return (std::max(bx, by) + 15) / 16; }
+template <int MODE>
+void gram_one_tile(int nt, dim3 grid, hipStream_t st, int64_t n, const double* X, int ldx, int bx,
+                   const double* Y, int ldy, int by, double* P, int64_t rpb, int nta, int ntiles,
+                   int64_t nblk) {
+#define RCG_GRAM_NT(NT)                                                                          \
+  case NT:                                                                                       \
+    hipLaunchKernelGGL((gram_mfma_kernel<MODE, NT>), grid, dim3(kThreads), 0, st, n, X, ldx, bx, \
+                       Y, ldy, by, P, rpb, nta, ntiles, nblk);                                   \
+    break;
+  switch (nt) {
+    RCG_GRAM_NT(1)
+    RCG_GRAM_NT(2)
+    RCG_GRAM_NT(3)
+    RCG_GRAM_NT(4)
+    default: throw Error(1, "one-tile Gram launches take at most 64 columns");
+  }
+#undef RCG_GRAM_NT
+}
+
 template <typename T>
 void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, int by, T* G,
                  T* ws, size_t ws_elems, hipStream_t st, bool lower_only) {
@@ -1664,12 +1689,12 @@ void launch_gram(int64_t n, const T* X, int ldx, int bx, const T* Y, int ldy, in
         hipLaunchKernelGGL(gram_lds_kernel<kGramFull>, grid, dim3(kThreads), 0, st, n, X, ldx, bx,
                            Y, ldy, by, ws, rpb, nta, nl, nblk);
     } else if (sym) {
-      hipLaunchKernelGGL(gram_mfma_kernel<kGramDiag>, dim3((unsigned)round_up(nblk, kXcds)),
-                         dim3(kThreads), 0, st, n, X, ldx, bx, Y, ldy, by, ws, rpb, nta, 1, nblk);
-    } else
-      hipLaunchKernelGGL(gram_mfma_kernel<kGramFull>,
-                         dim3((unsigned)round_up(nblk * ntiles, kXcds)), dim3(kThreads), 0, st, n,
-                         X, ldx, bx, Y, ldy, by, ws, rpb, nta, ntiles, nblk);
+      gram_one_tile<kGramDiag>(gram_nt(bx, by), dim3((unsigned)round_up(nblk, kXcds)), st, n, X,
+                               ldx, bx, Y, ldy, by, ws, rpb, nta, 1, nblk);
+    } else {
+      gram_one_tile<kGramFull>(gram_nt(bx, by), dim3((unsigned)round_up(nblk * ntiles, kXcds)), st,
+                               n, X, ldx, bx, Y, ldy, by, ws, rpb, nta, ntiles, nblk);
+    }
   } else {
     hipLaunchKernelGGL(gram_partial_kernel<T>, dim3((unsigned)nblk, ntiles), dim3(kThreads), 0, st,
                        n, X, ldx, bx, Y, ldy, by, ws, rpb, ntiles);
@@ -1694,8 +1719,8 @@ void launch_gram_pair(int64_t n, const double* X, int ldx, const double* Y, int 
   int64_t rpb;
   const int64_t nblk = gram_blocks<double>(n, b, b, &rpb);
   if ((size_t)(2 * nblk) * b * b > ws_elems) throw Error(3, "gram workspace too small");
-  hipLaunchKernelGGL(gram_mfma_kernel<kGramPair>, dim3((unsigned)round_up(2 * nblk, kXcds)),
-                     dim3(kThreads), 0, st, n, X, ldx, b, Y, ldy, b, ws, rpb, 1, 2, nblk);
+  gram_one_tile<kGramPair>(gram_nt(b, b), dim3((unsigned)round_up(2 * nblk, kXcds)), st, n, X, ldx,
+                          b, Y, ldy, b, ws, rpb, 1, 2, nblk);
   RCG_HIP(hipGetLastError());
   const int64_t m = 2 * (int64_t)b * b;
   hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(reduce_grid(m)), dim3(kThreads),
