@@ -606,7 +606,7 @@ __global__ __launch_bounds__(kThreads, 2) void gram_lds_kernel(
 // j, so 16 lanes store 16 consecutive columns of one row.  Workgroups loop
 // over row tiles; C is staged once when bx <= 64.  blockIdx.y selects 64
 // output columns.
-template <bool BETA>
+template <bool BETA, int NTC>
 __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ C,
     int ldc, int by, double alpha, double beta, const double* Z, double* out, int ldo,
@@ -637,9 +637,9 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
   }
   for (int64_t tile = stripe; tile < ntiles; tile += nstripe) {
     const int64_t row = tile * kTile + wave * 16 + r;
-    d4v acc[4];
+    d4v acc[NTC];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < NTC; ++t) acc[t] = d4v{0.0, 0.0, 0.0, 0.0};
     for (int ci = 0; ci < nk; ++ci) {
       const int k0 = ci * kTile;
       if (nk > 1) {
@@ -663,20 +663,24 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
       }
 #pragma unroll
       for (int sidx = 0; sidx < 16; ++sidx) {
-        const int k = 16 * kk + sidx;
+        // step sidx contracts k0 + sidx + {0, 16, 32, 48}: none is a column of X
+        // once k0 + sidx >= bx (uniform)
+        if (NTC == 4 || k0 + sidx < bx) {
+          const int k = 16 * kk + sidx;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = mfma16x16x4(xa[sidx], cs[k][16 * t + r], acc[t]);
+          for (int t = 0; t < NTC; ++t) acc[t] = mfma16x16x4(xa[sidx], cs[k][16 * t + r], acc[t]);
+        }
       }
     }
     // Z may be `out` itself: all of the lane's Z values are read before its
     // first store, so the loads are issued together
-    double zv[4][4];
+    double zv[4][NTC];
     if constexpr (BETA) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int64_t ro = tile * kTile + wave * 16 + 4 * j + kk;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < NTC; ++t) {
           const int c = c0 + 16 * t + r;
           zv[j][t] = (ro < n && c < by) ? Z[ro * ldo + c] : 0.0;
         }
@@ -687,7 +691,7 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
       const int64_t ro = tile * kTile + wave * 16 + 4 * j + kk;
       if (ro >= n) continue;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < NTC; ++t) {
         const int c = c0 + 16 * t + r;
         if (c >= by) continue;
         double v = alpha * acc[t][j];
@@ -1759,14 +1763,25 @@ void launch_tall_gemm(int64_t n, const T* X, int ldx, int bx, const T* C, int ld
     // 256-CU part (four per CU, three with the registers that hold beta Z)
     const int64_t wgs = beta != 0.0 ? 768 : 1024;
     const int64_t nstripe = std::max<int64_t>(1, std::min<int64_t>(nrb, wgs / (int64_t)ct));
-    if (beta != 0.0)
-      hipLaunchKernelGGL(tall_gemm_mfma_kernel<true>, dim3((unsigned)round_up(nstripe * ct, kXcds)),
-                         dim3(kThreads), 0, st, n, X, ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo,
-                         nrb, (int)ct, nstripe);
-    else
-      hipLaunchKernelGGL(tall_gemm_mfma_kernel<false>, dim3((unsigned)round_up(nstripe * ct, kXcds)),
-                         dim3(kThreads), 0, st, n, X, ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo,
-                         nrb, (int)ct, nstripe);
+    const dim3 grid((unsigned)round_up(nstripe * ct, kXcds));
+    const int ntc = by >= kTile ? 4 : (by + 15) / 16;
+#define RCG_TALL_NTC(NTC)                                                                           \
+  case NTC:                                                                                         \
+    if (beta != 0.0)                                                                                \
+      hipLaunchKernelGGL((tall_gemm_mfma_kernel<true, NTC>), grid, dim3(kThreads), 0, st, n, X, ldx, \
+                         bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb, (int)ct, nstripe);          \
+    else                                                                                            \
+      hipLaunchKernelGGL((tall_gemm_mfma_kernel<false, NTC>), grid, dim3(kThreads), 0, st, n, X,    \
+                         ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb, (int)ct, nstripe);     \
+    break;
+    switch (ntc) {
+      RCG_TALL_NTC(1)
+      RCG_TALL_NTC(2)
+      RCG_TALL_NTC(3)
+      RCG_TALL_NTC(4)
+      default: throw Error(1, "bad tall product width");
+    }
+#undef RCG_TALL_NTC
   } else {
     hipLaunchKernelGGL(tall_gemm_kernel<T>, dim3((unsigned)grid, ct), dim3(kThreads), 0, st, n, X,
                        ldx, bx, C, ldc, by, alpha, beta, Z, out, ldo, nrb);

@@ -419,36 +419,11 @@ class FilterBatcher {
     DevBuf ddesc, dtask;
     HostBuf hdesc, htask;
     hipEvent_t end = nullptr;  // the slot's last batch has finished
-    hipGraphExec_t exec = nullptr;  // the slot's last batch as a graph (graph mode)
     bool used = false;
-  };
-  // Ends a stream capture left open by an exception.
-  struct Capture {
-    hipStream_t st = nullptr;
-    explicit Capture(hipStream_t s) : st(s) {
-      if (st) RCG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    }
-    hipGraph_t end() {
-      hipGraph_t g = nullptr;
-      const hipStream_t s = st;
-      st = nullptr;
-      RCG_HIP(hipStreamEndCapture(s, &g));
-      return g;
-    }
-    ~Capture() {
-      if (!st) return;
-      hipGraph_t g = nullptr;
-      if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
-      (void)hipGetLastError();
-    }
   };
 
  public:
   FilterBatcher(int device, int window_us) : device_(device), window_us_(window_us) {
-    // RCG_BATCH_GRAPH=1: each batch's step launches are captured into a
-    // hipGraph and launched as one
-    const char* gm = std::getenv("RCG_BATCH_GRAPH");
-    graph_ = gm && gm[0] == '1';
     RCG_HIP(hipSetDevice(device_));
     RCG_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     for (Slot& sl : slots_) {
@@ -475,7 +450,6 @@ class FilterBatcher {
       ev_free_.clear();
     }
     for (Slot& sl : slots_) {  // bound to st_
-      if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
       sl.ddesc.release();
       sl.dtask.release();
       sl.hdesc.release();
@@ -633,10 +607,6 @@ class FilterBatcher {
     // kDepth are in flight
     if (sl.used) wait_event(sl.end, true);
     sl.used = true;
-    if (sl.exec) {
-      RCG_HIP(hipGraphExecDestroy(sl.exec));
-      sl.exec = nullptr;
-    }
     sl.hdesc.ensure(db);
     sl.htask.ensure(tb);
     sl.ddesc.ensure(db);
@@ -673,8 +643,6 @@ class FilterBatcher {
     RCG_HIP(hipEventRecord(t0, st_));
     int64_t nl = 0;
     double nb = 0;
-    Capture cap(graph_ ? st_ : nullptr);
-    const unsigned rec = graph_ ? hipEventRecordExternal : 0u;
     for (int j = jmin; j <= mmax; ++j) {
       for (Group& gr : groups) {
         if (j < gr.jstart) continue;
@@ -688,14 +656,7 @@ class FilterBatcher {
         for (size_t q = gr.g0; q < gr.end; ++q) nb += batch[q]->step_bytes;
       }
       for (FilterReq* r : batch)
-        if (r->desc.m == j && j >= r->jstart) RCG_HIP(hipEventRecordWithFlags(r->done, st_, rec));
-    }
-    if (graph_) {
-      hipGraph_t gr = cap.end();
-      const hipError_t ie = hipGraphInstantiate(&sl.exec, gr, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(gr);
-      RCG_HIP(ie);
-      RCG_HIP(hipGraphLaunch(sl.exec, st_));
+        if (r->desc.m == j && j >= r->jstart) RCG_HIP(hipEventRecord(r->done, st_));
     }
     RCG_HIP(hipEventRecord(t1, st_));
     RCG_HIP(hipEventRecord(sl.end, st_));
@@ -725,7 +686,6 @@ class FilterBatcher {
 
   int device_;
   int window_us_;
-  bool graph_ = false;
   hipStream_t st_ = nullptr;
   std::thread thr_;
   std::mutex mu_;
