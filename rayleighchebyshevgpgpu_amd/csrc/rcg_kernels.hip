@@ -1050,6 +1050,10 @@ __global__ __launch_bounds__(kThreads) void dense_gemv_kernel(int64_t n,
 // ------------------------------------------------------ column reductions
 // Threads are laid out as (row slot, column): CW columns per pass.
 constexpr int kColRows = 512;  // rows per partial block
+// Rows whose loads a column-reduction thread issues together before adding
+// them in row order (the sums are those of the one-row loop): at n ~ 5e4,
+// b = 24 a thread walks ~50 rows, one load latency each when taken singly.
+constexpr int kColDepth = 8;
 
 template <typename T, bool RESID>
 __global__ __launch_bounds__(kThreads) void col_sumsq_kernel(int64_t n, const T* __restrict__ V,
@@ -1069,7 +1073,23 @@ __global__ __launch_bounds__(kThreads) void col_sumsq_kernel(int64_t n, const T*
     double s = 0.0;
     if (slot < slots && c < nc) {
       const double th = RESID ? theta[c] : 0.0;
-      for (int64_t i = i0 + slot; i < i1; i += slots) {
+      int64_t i = i0 + slot;
+      // kColDepth rows' loads in flight, summed in row order as one at a time
+      for (; i + (kColDepth - 1) * slots < i1; i += kColDepth * slots) {
+        double v[kColDepth], w[kColDepth];
+#pragma unroll
+        for (int k = 0; k < kColDepth; ++k) {
+          v[k] = double(V[(i + k * slots) * ld + c]);
+          if (RESID) w[k] = double(Wm[(i + k * slots) * ld + c]);
+        }
+#pragma unroll
+        for (int k = 0; k < kColDepth; ++k) {
+          double x = v[k];
+          if (RESID) x = th * x - w[k];
+          s += x * x;
+        }
+      }
+      for (; i < i1; i += slots) {
         double v = double(V[i * ld + c]);
         if (RESID) v = th * v - double(Wm[i * ld + c]);
         s += v * v;
@@ -1106,7 +1126,22 @@ __global__ __launch_bounds__(kThreads) void resid_norm_kernel(int64_t n, const T
     double s = 0.0, q = 0.0;
     if (slot < slots && c < nc) {
       const double th = theta[c];
-      for (int64_t i = i0 + slot; i < i1; i += slots) {
+      int64_t i = i0 + slot;
+      for (; i + (kColDepth - 1) * slots < i1; i += kColDepth * slots) {
+        double v[kColDepth], w[kColDepth];
+#pragma unroll
+        for (int k = 0; k < kColDepth; ++k) {
+          v[k] = double(V[(i + k * slots) * ld + c]);
+          w[k] = double(Wm[(i + k * slots) * ld + c]);
+        }
+#pragma unroll
+        for (int k = 0; k < kColDepth; ++k) {
+          const double r = th * v[k] - w[k];
+          s += r * r;
+          q += v[k] * v[k];
+        }
+      }
+      for (; i < i1; i += slots) {
         const double v = double(V[i * ld + c]);
         const double r = th * v - double(Wm[i * ld + c]);
         s += r * r;
