@@ -212,6 +212,19 @@ __global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const T* __re
   T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   if (o < m) {
     int64_t b = sl;
+    // four rounds' loads in flight, added in the order of the one-round loop
+    for (; b + 60 < nblk; b += 64) {
+      T x[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = P[(b + 4 * k) * m + o];
+#pragma unroll
+      for (int k = 0; k < 16; k += 4) {
+        a0 += x[k];
+        a1 += x[k + 1];
+        a2 += x[k + 2];
+        a3 += x[k + 3];
+      }
+    }
     for (; b + 12 < nblk; b += 16) {
       a0 += P[b * m + o];
       a1 += P[(b + 4) * m + o];
