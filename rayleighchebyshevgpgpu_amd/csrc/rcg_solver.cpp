@@ -244,6 +244,33 @@ struct HostBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// Pinned host memory that reduction kernels write their results into, read on
+// the host once the stream has been waited for: coherent, so the results need
+// no device-to-host copy (three copies fewer per inner iteration).  Kept for
+// the context's lifetime and only grown.
+struct CohBuf {
+  void* h = nullptr;
+  void* d = nullptr;  // the same memory as the device addresses it
+  size_t bytes = 0;
+  CohBuf() = default;
+  CohBuf(const CohBuf&) = delete;
+  CohBuf& operator=(const CohBuf&) = delete;
+  ~CohBuf() {
+    if (h) (void)hipHostFree(h);
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    if (h) {
+      RCG_HIP(hipHostFree(h));
+      h = d = nullptr;
+      bytes = 0;
+    }
+    RCG_HIP(hipHostMalloc(&h, b, hipHostMallocCoherent | hipHostMallocMapped));
+    RCG_HIP(hipHostGetDevicePointer(&d, h, 0));
+    bytes = b;
+  }
+};
+
 // ----------------------------------------------------------------- handles
 struct rcg_context_s {
   int device = 0;
@@ -255,6 +282,7 @@ struct rcg_context_s {
   DevBuf ws;      // reduction partials
   DevBuf small;   // b x b matrices and b-vectors on the device
   HostBuf pinned; // host staging of the same
+  CohBuf results; // reduction results the host reads (not released with the scratch)
   void bind_scratch() {
     ws.use_stream(stream);
     small.use_stream(stream);
@@ -743,6 +771,8 @@ class Engine {
     const size_t sm = sizeof(double) * (size_t)(2 * bb_ * bb_ + 4 * bb_ + 64);
     ctx_->small.ensure(sm);
     ctx_->pinned.ensure(sm);
+    const size_t rw = (size_t)(cw_ * bb_);
+    ctx_->results.ensure(sizeof(double) * (2 * rw * rw + 4 * rw + 64));
   }
   void attach(double* V, double* T, double* W, int64_t ld, double* Y, int64_t ldy) {
     buf_[0] = V;
@@ -782,9 +812,11 @@ class Engine {
   double* dsums() { return ctx_->small.as<double>() + 2 * bb_ * bb_; }
   double* dtheta() { return dsums() + bb_; }
   double* hmatA() { return ctx_->pinned.as<double>(); }
+  // reduction results written by the device into host memory (hres) at dres
+  double* hres() { return static_cast<double*>(ctx_->results.h); }
+  double* dres() { return static_cast<double*>(ctx_->results.d); }
   double* hmatB() { return ctx_->pinned.as<double>() + bb_ * bb_; }
   double* hvecUp() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + bb_; }
-  double* hvecDown() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + 2 * bb_; }
   double* hlz() { return ctx_->pinned.as<double>() + 2 * bb_ * bb_ + 3 * bb_; }
 
   double* ws() { return ctx_->ws.as<double>(); }
@@ -1290,11 +1322,10 @@ class Engine {
                     int64_t by, S* host, bool lower_only = false) {
     constexpr int w = std::is_same<S, cplx>::value ? 2 : 1;
     const int64_t rx = w * bx, ry = w * by;
-    rcg::launch_gram<double>(n_, X, (int)ldx, (int)rx, Yp, (int)ldy, (int)ry, dmatA(), ws(),
+    rcg::launch_gram<double>(n_, X, (int)ldx, (int)rx, Yp, (int)ldy, (int)ry, dres(), ws(),
                              ws_elems_, st_, lower_only);
-    RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * rx * ry, hipMemcpyDeviceToHost, st_));
     sync();
-    const double* g = hmatA();
+    const double* g = hres();
     if constexpr (w == 1) {
       std::memcpy(host, g, sizeof(double) * bx * by);
     } else {
@@ -1468,20 +1499,18 @@ class Engine {
     // RCG_GRAM_UNPAIRED=1 always runs them separately.
     const char* unp = std::getenv("RCG_GRAM_UNPAIRED");
     if ((unp && unp[0] == '1') || n_ * rb >= (int64_t(1) << 22)) {
-      rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, V(), (int)ld_, (int)rb, dmatA(), ws(),
+      rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, V(), (int)ld_, (int)rb, dres(), ws(),
                                ws_elems_, st_);
       rcg::launch_gram<double>(n_, V(), (int)ld_, (int)rb, W_, (int)ld_, (int)rb,
-                               dmatA() + rb * rb, ws(), ws_elems_, st_, true);
+                               dres() + rb * rb, ws(), ws_elems_, st_, true);
     } else {
-      rcg::launch_gram_pair(n_, V(), (int)ld_, W_, (int)ld_, (int)rb, dmatA(), ws(), ws_elems_,
+      rcg::launch_gram_pair(n_, V(), (int)ld_, W_, (int)ld_, (int)rb, dres(), ws(), ws_elems_,
                             st_);
     }
-    RCG_HIP(hipMemcpyAsync(hmatA(), dmatA(), sizeof(double) * 2 * rb * rb, hipMemcpyDeviceToHost,
-                           st_));
     sync();
     std::vector<S> G(b * b), H1(b * b);
-    combine_gram<S>(hmatA(), b, b, G.data());
-    combine_gram<S>(hmatA() + rb * rb, b, b, H1.data());
+    combine_gram<S>(hres(), b, b, G.data());
+    combine_gram<S>(hres() + rb * rb, b, b, H1.data());
     std::vector<S> R, Ri;
     if (!rcg::dense::cholesky_upper(G.data(), b, R)) {
       // V1 is left as it is: the plain second (and third) pass, then project
@@ -1556,16 +1585,16 @@ class Engine {
     const int64_t rc = cw_ * count;
     for (int64_t c = 0; c < rc; ++c) hvecUp()[c] = th[c / cw_];
     RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * rc, hipMemcpyHostToDevice, st_));
-    rcg::launch_resid_norm_sumsq<double>(n_, V(), W_, (int)ld_, (int)rc, dtheta(), dsums(), ws(),
+    rcg::launch_resid_norm_sumsq<double>(n_, V(), W_, (int)ld_, (int)rc, dtheta(), dres(), ws(),
                                          ws_elems_, st_);
-    RCG_HIP(hipMemcpyAsync(hvecDown(), dsums(), sizeof(double) * 2 * rc, hipMemcpyDeviceToHost, st_));
     sync();
+    const double* sums = hres();
     last_norms_.assign(count, 1.0);
     for (int64_t c = 0; c < count; ++c) {
       double t = 0.0, q = 0.0;
       for (int e = 0; e < cw_; ++e) {
-        t += hvecDown()[cw_ * c + e];
-        q += hvecDown()[rc + cw_ * c + e];
+        t += sums[cw_ * c + e];
+        q += sums[rc + cw_ * c + e];
       }
       last_norms_[c] = q > 0.0 ? std::sqrt(q) : 1.0;
       res[c] = std::sqrt(std::abs(t)) / last_norms_[c];
@@ -1592,13 +1621,13 @@ class Engine {
     const int64_t rc = cw_ * count;
     for (int64_t c = 0; c < rc; ++c) hvecUp()[c] = th[c / cw_];
     RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * rc, hipMemcpyHostToDevice, st_));
-    rcg::launch_residual_sumsq<double>(n_, V(), W_, (int)ld_, (int)rc, dtheta(), dsums(), ws(),
+    rcg::launch_residual_sumsq<double>(n_, V(), W_, (int)ld_, (int)rc, dtheta(), dres(), ws(),
                                        ws_elems_, st_);
-    RCG_HIP(hipMemcpyAsync(hvecDown(), dsums(), sizeof(double) * rc, hipMemcpyDeviceToHost, st_));
     sync();
+    const double* sums = hres();
     for (int64_t c = 0; c < count; ++c) {
       double t = 0.0;
-      for (int e = 0; e < cw_; ++e) t += hvecDown()[cw_ * c + e];
+      for (int e = 0; e < cw_; ++e) t += sums[cw_ * c + e];
       res[c] = std::sqrt(std::abs(t));
     }
   }
