@@ -115,7 +115,8 @@ int batch_class(int ncv) {
 // the workgroups of 128-row tasks (served 1186 -> 1236-1258 eigenpairs/s;
 // 32-row tasks 1213-1229).  The shared launches of 8 and 32 served requests
 // alone (rcg_cheb_filter_batch, FP64, degree 24) run within 7% of each other
-// for 32, 64 and 128 rows.  When G does not divide the workgroup, whole
+// for 32, 64 and 128 rows.  With the exact lane counts below, the served stream
+// took 32, 64, 128 and 256 rows at 1574, 1580, 1543 and 1451 eigenpairs/s.  When G does not divide the workgroup, whole
 // passes of kThreads / G rows nearest 64.
 constexpr int kBatchRows = 64;
 int batch_rows(int cls) {

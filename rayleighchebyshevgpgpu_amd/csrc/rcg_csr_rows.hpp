@@ -200,6 +200,18 @@ __device__ __forceinline__ void csr_rows(
       }
     }
   __syncthreads();
+  // kChebSrc: this lane's columns' kap, the same for every row (loaded once,
+  // not with each row's operands)
+  T kv[MODE == kChebSrc ? NV : 1][W];
+  if constexpr (MODE == kChebSrc) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int cv = g + k * G;
+        kv[k][w] = cv < ncv ? kap[cv * W + w] : T(0);
+      }
+  }
   // RPI rows per lane iteration: rows li and li + RS, ... so that each wave
   // has RPI rows' gathers in flight at once.  When G does not divide the
   // workgroup, the last kThreads % G lanes own no row.
@@ -356,7 +368,7 @@ __device__ __forceinline__ void csr_rows(
         } else if constexpr (MODE == kChebSrc) {
 #pragma unroll
           for (int w = 0; w < W; ++w)
-            o[w] = p2 * fma(kap[cv * W + w], rs[q][k][w],
+            o[w] = p2 * fma(kv[k][w], rs[q][k][w],
                             fma(p1, fma(p0, acc[q][k][w], xs[q][k][w]), -zs[q][k][w]));
         } else {
 #pragma unroll
