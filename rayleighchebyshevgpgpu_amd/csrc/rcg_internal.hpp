@@ -197,6 +197,19 @@ template <typename T>
 void launch_resid_norm_sumsq(int64_t n, const T* V, const T* W, int ld, int nc,
                              const double* theta, double* sums, double* ws, size_t ws_elems,
                              hipStream_t st);
+// The same with theta (nc <= kThetaArgs values) passed in the kernel's
+// argument block instead of device memory (no host-to-device copy).
+constexpr int kThetaArgs = 128;
+struct ThetaArgs {
+  double v[kThetaArgs];
+#ifdef __HIPCC__
+  __host__ __device__
+#endif
+  double operator[](int c) const { return v[c]; }
+};
+void launch_resid_norm_sumsq_args(int64_t n, const double* V, const double* W, int ld, int nc,
+                                  const ThetaArgs& theta, double* sums, double* ws,
+                                  size_t ws_elems, hipStream_t st);
 
 // Caller operators (column-major blocks) around the row-interleaved solver
 // blocks: `cols` entries per row (cw = 2: complex entries of two doubles),

@@ -1121,10 +1121,11 @@ __global__ __launch_bounds__(kThreads) void col_sumsq_kernel(int64_t n, const T*
 
 // Residual and norm sums together: P[blk*2nc + c] = sum_i |theta_c V - W|^2,
 // P[blk*2nc + nc + c] = sum_i |V|^2 over the block's rows.
-template <typename T>
+// TH: const double* (theta in device memory) or ThetaArgs (in the arguments)
+template <typename T, typename TH>
 __global__ __launch_bounds__(kThreads) void resid_norm_kernel(int64_t n, const T* __restrict__ V,
                                                               const T* __restrict__ Wm, int ld,
-                                                              int nc, const double* theta,
+                                                              int nc, const TH theta,
                                                               double* __restrict__ P) {
   __shared__ double red[2 * kThreads];
   const int64_t blk = blockIdx.x;
@@ -1893,8 +1894,23 @@ void launch_resid_norm_sumsq(int64_t n, const T* V, const T* W, int ld, int nc,
   if (nc <= 0) return;
   const int64_t nblk = (n + kColRows - 1) / kColRows;
   if ((size_t)(nblk * 2 * nc) > ws_elems) throw Error(3, "column-reduction workspace too small");
-  hipLaunchKernelGGL((resid_norm_kernel<T>), dim3((unsigned)nblk), dim3(kThreads), 0, st, n, V, W,
-                     ld, nc, theta, ws);
+  hipLaunchKernelGGL((resid_norm_kernel<T, const double*>), dim3((unsigned)nblk), dim3(kThreads),
+                     0, st, n, V, W, ld, nc, theta, ws);
+  RCG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(reduce_grid(2 * nc)),
+                     dim3(kThreads), 0, st, ws, nblk, (int64_t)(2 * nc), sums);
+  RCG_HIP(hipGetLastError());
+}
+
+void launch_resid_norm_sumsq_args(int64_t n, const double* V, const double* W, int ld, int nc,
+                                  const ThetaArgs& theta, double* sums, double* ws,
+                                  size_t ws_elems, hipStream_t st) {
+  if (nc <= 0) return;
+  if (nc > kThetaArgs) throw Error(1, "more Ritz values than the argument block holds");
+  const int64_t nblk = (n + kColRows - 1) / kColRows;
+  if ((size_t)(nblk * 2 * nc) > ws_elems) throw Error(3, "column-reduction workspace too small");
+  hipLaunchKernelGGL((resid_norm_kernel<double, ThetaArgs>), dim3((unsigned)nblk), dim3(kThreads),
+                     0, st, n, V, W, ld, nc, theta, ws);
   RCG_HIP(hipGetLastError());
   hipLaunchKernelGGL((reduce_partials_kernel<double, double>), dim3(reduce_grid(2 * nc)),
                      dim3(kThreads), 0, st, ws, nblk, (int64_t)(2 * nc), sums);

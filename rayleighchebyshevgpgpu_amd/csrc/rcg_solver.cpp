@@ -1583,10 +1583,18 @@ class Engine {
     if (count <= 0) return;
     apply(V(), W_, count);
     const int64_t rc = cw_ * count;
-    for (int64_t c = 0; c < rc; ++c) hvecUp()[c] = th[c / cw_];
-    RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * rc, hipMemcpyHostToDevice, st_));
-    rcg::launch_resid_norm_sumsq<double>(n_, V(), W_, (int)ld_, (int)rc, dtheta(), dres(), ws(),
-                                         ws_elems_, st_);
+    if (rc <= rcg::kThetaArgs) {
+      // the Ritz values travel with the launch
+      rcg::ThetaArgs ta{};
+      for (int64_t c = 0; c < rc; ++c) ta.v[c] = th[c / cw_];
+      rcg::launch_resid_norm_sumsq_args(n_, V(), W_, (int)ld_, (int)rc, ta, dres(), ws(),
+                                        ws_elems_, st_);
+    } else {
+      for (int64_t c = 0; c < rc; ++c) hvecUp()[c] = th[c / cw_];
+      RCG_HIP(hipMemcpyAsync(dtheta(), hvecUp(), sizeof(double) * rc, hipMemcpyHostToDevice, st_));
+      rcg::launch_resid_norm_sumsq<double>(n_, V(), W_, (int)ld_, (int)rc, dtheta(), dres(), ws(),
+                                           ws_elems_, st_);
+    }
     sync();
     const double* sums = hres();
     last_norms_.assign(count, 1.0);
