@@ -338,10 +338,13 @@ __device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[NT][NT], const d
                                                      int ldx, int bx, const double* __restrict__ Y,
                                                      int ldy, int by, int a0, int b0, int64_t ib,
                                                      int64_t i1, int kr, int cc) {
-  // four 16-row groups per pass: all of their loads are issued before the
-  // MFMAs that consume them, and the next pass's loads before this pass's
-  // MFMAs
-  constexpr int U = 4;
+  // U 16-row groups per pass: all of their loads are issued before the MFMAs
+  // that consume them, and the next pass's loads before this pass's MFMAs.
+  // A wave's rows and the order it sums them in do not depend on U.  Two
+  // groups at NT = 3 (142 instead of 198 VGPRs for X^T Y: served stream
+  // 1602 -> 1611 eigenpairs/s); at NT = 4 two groups made the 128^3 x 64
+  // solve 1975 -> 1989 ms.
+  constexpr int U = NT == 3 ? 2 : 4;
   double xa[U][NT], yb[U][NT], xn[U][NT], yn[U][NT];
   auto load = [&](int64_t jb, double (&xv)[U][NT], double (&yv)[U][NT]) {
 #pragma unroll
