@@ -77,6 +77,8 @@ static_assert(sizeof(kBatchLim) == sizeof(kBatchG), "one G per class");
 template <typename T, bool SRC, bool CX>
 void cheb_batch_t(const BatchDesc* descs, const BatchTask* tasks, int ntasks, int j, int cls,
                   hipStream_t st) {
+  // one workgroup per task up to 8192 (workgroups striding over 1024 or 2048
+  // tasks' worth served 1540 and 1580 eigenpairs/s against 1614)
   const unsigned grid = (unsigned)std::min(ntasks, 8192);
 #define RCG_BATCH_CASE(C, GG, NN)                                                                 \
   case C:                                                                                         \
