@@ -24,7 +24,7 @@ __global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __
   __shared__ int s_rp[kMaxRpb + 1];
   __shared__ int s_col[kStageNz];
   __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
-  const MixedStartArgs kNoStart{};  // the batched steps are never the mixed filter's start
+  const MixedArgs kNoStart{};  // the batched steps are never the mixed filter's start
   for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
     const BatchTask tk = tasks[t];
     const BatchDesc& d = descs[tk.req];

@@ -118,7 +118,17 @@ struct V16<float> {
 // the correction recurrence of the mixed-precision filter.
 // kApplyStart (FP64): W = A V, whose entries go straight into the mixed
 // filter's start (mixed_start_elem) instead of being stored.
-enum CsrMode { kApply = 0, kApplyBeta = 1, kChebFirst = 2, kCheb = 3, kChebSrc = 4, kApplyStart = 5 };
+// kChebSrcEnd (FP32): kChebSrc whose output x goes straight into the mixed
+// filter's recombination, V <- s_end * (um[c] V + x) (V FP64), not stored.
+enum CsrMode {
+  kApply = 0,
+  kApplyBeta = 1,
+  kChebFirst = 2,
+  kCheb = 3,
+  kChebSrc = 4,
+  kApplyStart = 5,
+  kChebSrcEnd = 6
+};
 
 // The mixed filter's start for one element, v = V[i][c] and w = (A V)[i][c]:
 // a Ritz column (flag != 0) gets R = w - theta v, E0 = 0, E1 = e1c R; a plain
@@ -164,11 +174,13 @@ __device__ __forceinline__ void csr_rows(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
     int tail, T p0, T p1, T p2, int64_t i0, int64_t i1, const T* __restrict__ R,
-    const T* __restrict__ kap, int* s_rp, int* s_col, T* s_val, const MixedStartArgs& ms) {
+    const T* __restrict__ kap, int* s_rp, int* s_col, T* s_val, const MixedArgs& ms) {
   using VT = V16<T>;
   constexpr int W = VT::W;
   constexpr int RS = kThreads / G;
   static_assert(MODE != kApplyStart || std::is_same<T, double>::value, "the start is FP64");
+  static_assert(MODE != kChebSrcEnd || std::is_same<T, float>::value, "the end follows FP32 steps");
+  constexpr bool kSrc = MODE == kChebSrc || MODE == kChebSrcEnd;
   const int g = threadIdx.x % G;
   const int r = threadIdx.x / G;
   // kApplyStart: this lane's columns' (theta, flag), the same for every row
@@ -200,16 +212,18 @@ __device__ __forceinline__ void csr_rows(
       }
     }
   __syncthreads();
-  // kChebSrc: this lane's columns' kap, the same for every row (loaded once,
-  // not with each row's operands)
-  T kv[MODE == kChebSrc ? NV : 1][W];
-  if constexpr (MODE == kChebSrc) {
+  // kChebSrc: this lane's columns' kap (and, ending, um), the same for every
+  // row (loaded once, not with each row's operands)
+  T kv[kSrc ? NV : 1][W];
+  double uv[MODE == kChebSrcEnd ? NV : 1][W];
+  if constexpr (kSrc) {
 #pragma unroll
     for (int k = 0; k < NV; ++k)
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         const int cv = g + k * G;
         kv[k][w] = cv < ncv ? kap[cv * W + w] : T(0);
+        if constexpr (MODE == kChebSrcEnd) uv[k][w] = cv < ncv ? ms.um[cv * W + w] : 0.0;
       }
   }
   // RPI rows per lane iteration: rows li and li + RS, ... so that each wave
@@ -228,7 +242,7 @@ __device__ __forceinline__ void csr_rows(
     }
     // own-row operands first: they do not depend on the gathers
     T xs[RPI][NV][W], zs[RPI][NV][W];
-    T rs[MODE == kChebSrc ? RPI : 1][NV][W];
+    T rs[kSrc ? RPI : 1][NV][W];
 #pragma unroll
     for (int q = 0; q < RPI; ++q) {
       if (!live_row[q]) continue;
@@ -237,11 +251,10 @@ __device__ __forceinline__ void csr_rows(
       for (int k = 0; k < NV; ++k) {
         const int cv = g + k * G;
         if (cv < ncv) {
-          if constexpr (MODE == kChebFirst || MODE == kCheb || MODE == kChebSrc ||
-                        MODE == kApplyStart)
+          if constexpr (MODE == kChebFirst || MODE == kCheb || kSrc || MODE == kApplyStart)
             VT::ld(X + rowoff + cv * W, xs[q][k]);
-          if constexpr (MODE == kCheb || MODE == kChebSrc) VT::ld_stream(Z + rowoff + cv * W, zs[q][k]);
-          if constexpr (MODE == kChebSrc) VT::ld_stream(R + rowoff + cv * W, rs[q][k]);
+          if constexpr (MODE == kCheb || kSrc) VT::ld_stream(Z + rowoff + cv * W, zs[q][k]);
+          if constexpr (kSrc) VT::ld_stream(R + rowoff + cv * W, rs[q][k]);
           if constexpr (MODE == kApplyBeta) VT::ld(out + rowoff + cv * W, zs[q][k]);
         }
       }
@@ -365,7 +378,7 @@ __device__ __forceinline__ void csr_rows(
         } else if constexpr (MODE == kChebFirst) {
 #pragma unroll
           for (int w = 0; w < W; ++w) o[w] = p2 * (p1 * fma(p0, acc[q][k][w], xs[q][k][w]));
-        } else if constexpr (MODE == kChebSrc) {
+        } else if constexpr (kSrc) {
 #pragma unroll
           for (int w = 0; w < W; ++w)
             o[w] = p2 * fma(kv[k][w], rs[q][k][w],
@@ -375,7 +388,25 @@ __device__ __forceinline__ void csr_rows(
           for (int w = 0; w < W; ++w)
             o[w] = p2 * fma(p1, fma(p0, acc[q][k][w], xs[q][k][w]), -zs[q][k][w]);
         }
-        if (tail < W && cv == ncv - 1) {
+        if constexpr (MODE == kChebSrcEnd) {
+          // V <- s_end (um V + x) on this vector's columns (FP64)
+          double* vp = ms.v_end + rowoff + cv * W;
+          if (tail < W && cv == ncv - 1) {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+              if (w < tail) vp[w] = ms.s_end * fma(uv[k][w], vp[w], double(o[w]));
+          } else {
+            // W = 4 floats: four doubles, 32-byte aligned (ld is a multiple of 4)
+            const dvec2 v0 = reinterpret_cast<const dvec2*>(vp)[0];
+            const dvec2 v1 = reinterpret_cast<const dvec2*>(vp)[1];
+            const double vv[4] = {v0.x, v0.y, v1.x, v1.y};
+            double r[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) r[w] = ms.s_end * fma(uv[k][w], vv[w], double(o[w]));
+            reinterpret_cast<dvec2*>(vp)[0] = dvec2{r[0], r[1]};
+            reinterpret_cast<dvec2*>(vp)[1] = dvec2{r[2], r[3]};
+          }
+        } else if (tail < W && cv == ncv - 1) {
           // the last vector holds columns past the caller's count: only the
           // first `tail` entries are stored
 #pragma unroll

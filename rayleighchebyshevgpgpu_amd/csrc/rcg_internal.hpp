@@ -113,6 +113,23 @@ void launch_cheb_step(int64_t n, const int* rowptr, const int* col, const T* val
                       hipStream_t st, int panel = 0, const CsrSched* sched = nullptr,
                       bool cx = false);
 
+// The mixed filter's FP64 ends fused into its kernels: the start (below) and
+// the recombination V <- s_end (um[c] V + E_m) computed in the last FP32
+// step's epilogue (launch_cheb_step_src_end; single-operator filters only:
+// in the batched steps a kernel of its own for the ending requests served
+// 1588 -> 1580 eigenpairs/s).
+struct MixedArgs {
+  const double* cp = nullptr;  // theta_c at cp[c], the Ritz flag at cp[nc + c]
+  float* e0 = nullptr;
+  float* e1 = nullptr;
+  float* r = nullptr;
+  double e1c = 0.0, g1 = 0.0, g2 = 0.0;
+  int nc = 0;  // real columns of the call
+  double* v_end = nullptr;       // the recombination's V (same element layout as E)
+  const double* um = nullptr;    // u_m(theta_c) per column
+  double s_end = 0.0;            // 1 / (m + 1)
+};
+
 // Mixed-precision filter.  The FP64 filter output s*X_m of X_0 = v is split
 // per column as X_j = u_j(theta_c) v + E_j, u_j the scalar recurrence of the
 // step at the Ritz value theta_c; E_j (FP32) then obeys
@@ -124,21 +141,21 @@ void launch_cheb_step_src(int64_t n, const int* rowptr, const int* col, const fl
                           const float* kap, int ld, int ncols, float g1, float g2, float s,
                           hipStream_t st, int panel = 0, const CsrSched* sched = nullptr,
                           bool cx = false);
+// The last correction step with the recombination in its epilogue
+// (end.v_end, end.um, end.s_end over the same ncols columns): V is written,
+// X_m is not.  Bit-identical to launch_cheb_step_src + launch_mixed_combine.
+void launch_cheb_step_src_end(int64_t n, const int* rowptr, const int* col, const float* val,
+                              const float* X1, const float* X0, const float* R, const float* kap,
+                              int ld, int ncols, float g1, float g2, const MixedArgs& end,
+                              hipStream_t st, int panel = 0, const CsrSched* sched = nullptr,
+                              bool cx = false);
 // E0, E1, R from V and AV (see the kernel); cp[c] = theta_c, cp[nc+c] = 1 for
 // Ritz columns, 0 for plain ones (E_j = X_j, R = 0).
-struct MixedStartArgs {
-  const double* cp = nullptr;  // theta_c at cp[c], the Ritz flag at cp[nc + c]
-  float* e0 = nullptr;
-  float* e1 = nullptr;
-  float* r = nullptr;
-  double e1c = 0.0, g1 = 0.0, g2 = 0.0;
-  int nc = 0;  // real columns of the call
-};
 // W = A V and the start above in one pass: the apply's epilogue turns each
 // (v, w) pair into the E0, E1, R entries, so W is never stored.  The results
 // are bit-identical to launch_csr_apply followed by launch_mixed_start.
 void launch_csr_apply_start(int64_t n, const int* rowptr, const int* col, const double* val,
-                            const double* V, int ld, int ncols, const MixedStartArgs& ms,
+                            const double* V, int ld, int ncols, const MixedArgs& ms,
                             hipStream_t st, const CsrSched* sched = nullptr, bool cx = false);
 void launch_mixed_start(int64_t n, const double* V, const double* AV, int ld, int nc,
                         const double* cp, double e1, double g1, double g2, float* E0, float* E1,

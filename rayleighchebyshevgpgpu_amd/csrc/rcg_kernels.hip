@@ -28,7 +28,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
     int tail, T p0, T p1, T p2, int64_t rows_per_blk, int64_t nblk, const int* __restrict__ order,
-    int remap, const T* __restrict__ R, const T* __restrict__ kap, MixedStartArgs ms) {
+    int remap, const T* __restrict__ R, const T* __restrict__ kap, MixedArgs ms) {
   __shared__ int s_rp[kMaxRpb + 1];
   __shared__ int s_col[kStageNz];
   __shared__ T s_val[kStageNz * (CX ? 2 : 1)];
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
 template <typename T, int MODE, bool CX, int G, int NV>
 void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
                   T* out, int ld, int ncv, int tail, T p0, T p1, T p2, hipStream_t st,
-                  const CsrSched* sc, const T* R, const T* kap, const MixedStartArgs& ms) {
+                  const CsrSched* sc, const T* R, const T* kap, const MixedArgs& ms) {
   constexpr int RS = kThreads / G;
   int64_t rows_per_blk, nblk;
   const int* order = nullptr;
@@ -87,7 +87,7 @@ void csr_launch_g(int64_t n, const int* rp, const int* ci, const T* va, const T*
 template <typename T, int MODE, bool CX>
 void csr_launch_cx(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
                    T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
-                   const T* R, const T* kap, const MixedStartArgs& ms) {
+                   const T* R, const T* kap, const MixedArgs& ms) {
   constexpr int W = V16<T>::W;
   const int ncv = (ncols + W - 1) / W;
   const int tail = ncols - (ncv - 1) * W;
@@ -127,7 +127,7 @@ template <typename T, int MODE>
 void csr_launch(int64_t n, const int* rp, const int* ci, const T* va, const T* X, const T* Z,
                 T* out, int ld, int ncols, T p0, T p1, T p2, hipStream_t st, const CsrSched* sc,
                 const T* R = nullptr, const T* kap = nullptr, bool cx = false,
-                const MixedStartArgs& ms = MixedStartArgs{}) {
+                const MixedArgs& ms = MixedArgs{}) {
   if (cx) {
     if (ncols % 2 != 0) throw Error(1, "complex blocks have an even number of real columns");
     csr_launch_cx<T, MODE, true>(n, rp, ci, va, X, Z, out, ld, ncols, p0, p1, p2, st, sc, R, kap, ms);
@@ -1564,7 +1564,7 @@ void launch_csr_apply(int64_t n, const int* rowptr, const int* col, const T* val
 }
 
 void launch_csr_apply_start(int64_t n, const int* rowptr, const int* col, const double* val,
-                            const double* V, int ld, int ncols, const MixedStartArgs& ms,
+                            const double* V, int ld, int ncols, const MixedArgs& ms,
                             hipStream_t st, const CsrSched* sc, bool cx) {
   if (n <= 0 || ncols <= 0) return;
   if (ms.nc != ncols || !ms.cp || !ms.e0 || !ms.e1 || !ms.r)
@@ -1605,6 +1605,24 @@ void launch_cheb_step_src(int64_t n, const int* rowptr, const int* col, const fl
     const int pc = std::min(panel, ncols - c0);
     csr_launch<float, kChebSrc>(n, rowptr, col, val, X1 + c0, X0 + c0, out + c0, ld, pc, g1, g2, s,
                                 st, sc, R + c0, kap + c0, cx);
+  }
+}
+
+void launch_cheb_step_src_end(int64_t n, const int* rowptr, const int* col, const float* val,
+                              const float* X1, const float* X0, const float* R, const float* kap,
+                              int ld, int ncols, float g1, float g2, const MixedArgs& end,
+                              hipStream_t st, int panel, const CsrSched* sc, bool cx) {
+  if (n <= 0 || ncols <= 0) return;
+  if (!end.v_end || !end.um) throw Error(1, "fused recombination: V and um are required");
+  if (panel <= 0 || panel >= ncols) panel = ncols;
+  if (panel % 4 != 0) panel = round_up(panel, 4);
+  for (int c0 = 0; c0 < ncols; c0 += panel) {
+    const int pc = std::min(panel, ncols - c0);
+    MixedArgs e = end;
+    e.v_end = end.v_end + c0;
+    e.um = end.um + c0;
+    csr_launch<float, kChebSrcEnd>(n, rowptr, col, val, X1 + c0, X0 + c0, nullptr, ld, pc, g1, g2,
+                                   1.0f, st, sc, R + c0, kap + c0, cx, e);
   }
 }
 

@@ -1212,7 +1212,7 @@ class Engine {
     const bool fused = !dn && !(unf && unf[0] == '1');
     for (long r = 0; r < rep; ++r) {
       if (fused) {
-        rcg::MixedStartArgs ms;
+        rcg::MixedArgs ms;
         ms.cp = dcp;
         ms.e0 = E_[0];
         ms.e1 = E_[1];
@@ -1228,6 +1228,14 @@ class Engine {
         rcg::launch_mixed_start(n_, V(), W_, (int)ld_, (int)rb, dcp, g12, c.g1, c.g2, E_[0],
                                 E_[1], R32_, st_);
       }
+      // the recombination V <- (um V + E_m) / (m + 1): in the last FP32 step's
+      // epilogue for a single CSR operator (FP32 step at 128^3 x 64 with the
+      // filter's ends, 561 -> 546 us; RCG_MIXED_UNFUSED=1: a kernel of its
+      // own), after the batched steps a kernel of its own
+      rcg::MixedArgs end;
+      end.v_end = V();
+      end.um = dum;
+      end.s_end = 1.0 / double(m + 1);
       if (batcher_ && !dn) {
         rcg::BatchDesc d = batch_desc(m, c.g1, c.g2, rb, 4);
         d.val = val32;
@@ -1236,25 +1244,30 @@ class Engine {
         d.R = R32_;
         d.kap = dk;
         batch_submit(d, true, cx_, 2);
-        rcg::launch_mixed_combine(n_, V(), (int)ld_, (int)rb, dum, 1.0 / double(m + 1),
-                                  E_[m % 2], st_);
+        rcg::launch_mixed_combine(n_, V(), (int)ld_, (int)rb, dum, end.s_end, E_[m % 2], st_);
         continue;
       }
       float* X0 = E_[0];
       float* X1 = E_[1];
       step_begin((m - 1) * ((rb + tn.panel - 1) / tn.panel), double(m - 1) * (csr + 4.0 * blk));
       for (long j = 2; j <= m; ++j) {
-        if (dn)
+        if (dn) {
           rcg::launch_dense_step<float>(n_, val32, op_->lda, X1, X0, X0, (int)ld_, (int)rb, g1f,
                                         g2f, 1.0f, rcg::kStepSrc, R32_, dk + (size_t)j * ld_, st_);
-        else
+        } else if (j == m && fused) {
+          rcg::launch_cheb_step_src_end(n_, op_->rowptr, op_->col, val32, X1, X0, R32_,
+                                        dk + (size_t)j * ld_, (int)ld_, (int)rb, g1f, g2f, end,
+                                        st_, tn.panel, sc, cx_);
+        } else {
           rcg::launch_cheb_step_src(n_, op_->rowptr, op_->col, val32, X1, X0, X0, R32_,
                                     dk + (size_t)j * ld_, (int)ld_, (int)rb, g1f, g2f, 1.0f, st_,
                                     tn.panel, sc, cx_);
+        }
         std::swap(X0, X1);
       }
       step_end();
-      rcg::launch_mixed_combine(n_, V(), (int)ld_, (int)rb, dum, 1.0 / double(m + 1), X1, st_);
+      if (dn || !fused)
+        rcg::launch_mixed_combine(n_, V(), (int)ld_, (int)rb, dum, end.s_end, X1, st_);
     }
   }
   void attach_mixed(float* E0, float* E1, float* R) {
