@@ -251,7 +251,10 @@ inline unsigned reduce_grid(int64_t m) { return (unsigned)((m + 63) / 64); }
 template <typename T>
 int64_t gram_blocks(int64_t n, int bx, int by, int64_t* rows_per_blk) {
   const int ntiles = ((bx + kTile - 1) / kTile) * ((by + kTile - 1) / kTile);
-  int64_t target = std::max<int64_t>(1, 1024 / ntiles);
+  // row blocks: 1024 over the tiles, 512 from 2^20 rows (half the partials to
+  // write and reduce: the 128^3 x 64 FP64 solve 1977 -> 1967 ms; 256: 1973 ms)
+  const int64_t blocks = n >= (int64_t(1) << 20) ? 512 : 1024;
+  int64_t target = std::max<int64_t>(1, blocks / ntiles);
   int64_t rpb = round_up((n + target - 1) / target, kRb);
   // at least 4 rows per output column, so a block's bx x by partial is at
   // most a quarter of the rows it reads (small n would otherwise write about
