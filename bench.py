@@ -342,16 +342,20 @@ def serving_run(args, cpu=True, cpu_budget_s=15.0):
     def inc_buf(k):
         return k, max(4, k // 2)
 
-    def serve(saturate):
+    def serve(saturate, count=None, scale=1.0):
+        """One pass of the stream through a fresh server: the first `count`
+        requests (all by default), arrival times multiplied by `scale` (a
+        Poisson stream at rate/scale), or all queued at once when saturate."""
         srv = serving.Server(ctx, hbm_budget=args.hbm_budget << 30, max_active=args.slots,
                              window_us=args.window_us)
         import resource
+        sel = reqs[:count] if count else reqs
         tickets = []
         ru0 = resource.getrusage(resource.RUSAGE_SELF)
         start = time.perf_counter()
-        for r, op in zip(reqs, ops_):
+        for r, op in zip(sel, ops_):
             if not saturate:
-                delay = r.arrival - (time.perf_counter() - start)
+                delay = scale * r.arrival - (time.perf_counter() - start)
                 if delay > 0:
                     time.sleep(delay)
             inc, buf = inc_buf(r.k)
@@ -364,10 +368,10 @@ def serving_run(args, cpu=True, cpu_budget_s=15.0):
         fp = srv.filter_profile()
         # latency = completion - arrival, on the library's steady clock
         t_first = min(o["t_submit"] for o in outs)
-        arrival0 = t_first - (0.0 if saturate else reqs[0].arrival)
-        lat = np.array([o["t_done"] - (arrival0 + (0.0 if saturate else r.arrival))
-                        for o, r in zip(outs, reqs)]) * 1e3
-        bad = [i for i, o in enumerate(outs) if o["status"] != 0 or o["found"] < reqs[i].k]
+        arrival0 = t_first - (0.0 if saturate else scale * sel[0].arrival)
+        lat = np.array([o["t_done"] - (arrival0 + (0.0 if saturate else scale * r.arrival))
+                        for o, r in zip(outs, sel)]) * 1e3
+        bad = [i for i, o in enumerate(outs) if o["status"] != 0 or o["found"] < sel[i].k]
         found = sum(o["found"] for o in outs)
         # per-request host wall time by phase (admitted to done), averaged
         res = [o["result"] for o in outs]
@@ -378,7 +382,7 @@ def serving_run(args, cpu=True, cpu_budget_s=15.0):
         for t in tickets:
             srv.release(t)
         srv.close()
-        return {"eigenpairs": found, "makespan_s": round(makespan, 3),
+        return {"requests": len(sel), "eigenpairs": found, "makespan_s": round(makespan, 3),
                 "value": round(found / makespan, 2), "p50_ms": round(float(np.percentile(lat, 50)), 2),
                 "p99_ms": round(float(np.percentile(lat, 99)), 2), "failed": len(bad),
                 "filter_batches": st["filter_batches"],
@@ -389,7 +393,7 @@ def serving_run(args, cpu=True, cpu_budget_s=15.0):
                                   "alg_GBps": round(fp["alg_bytes"] / max(fp["device_s"], 1e-9) / 1e9, 1),
                                   "busy_fraction": round(fp["device_s"] / makespan, 3)}}
 
-    offered = saturated = None
+    offered = saturated = light = None
     if not args.plain_only:
         log("serving (Poisson arrivals) ...")
         offered = serve(False)
@@ -397,6 +401,12 @@ def serving_run(args, cpu=True, cpu_budget_s=15.0):
         saturated = serve(True) if args.saturate else None
         if saturated:
             log(f"saturated run: {saturated}")
+        if args.light_requests > 0 and args.light_rate > 0:
+            # the same stream's first requests at a rate the server keeps up
+            # with: latency is then service time rather than queueing
+            light = serve(False, args.light_requests, args.rate / args.light_rate)
+            light["rate"] = args.light_rate
+            log(f"light-load run: {light}")
     # plain FP64 path: the first requests one by one through rcg_solve_lowest
     m = min(args.plain_requests, len(reqs))
     tp = time.perf_counter()
@@ -426,7 +436,7 @@ def serving_run(args, cpu=True, cpu_budget_s=15.0):
                      "U[-W/2,W/2], W in [0,8], k in {8,16,32}, tol 1e-10, mixed precision"),
         "slots": args.slots, "window_us": args.window_us, "hbm_budget_gb": args.hbm_budget,
         "p50_ms": offered["p50_ms"], "p99_ms": offered["p99_ms"],
-        "offered_load": offered, "saturated": saturated,
+        "offered_load": offered, "saturated": saturated, "light_load": light,
         "plain_fp64": plain_rec,
         "speedup_vs_plain_fp64": round(best["value"] / plain, 2) if plain > 0 else None,
         # the shared filter launches: algorithmic bytes over their device time
@@ -501,8 +511,9 @@ def serving_bench(args):
         "config": {"workload": r["workload"], "slots": r["slots"], "window_us": r["window_us"],
                    "hbm_budget_gb": r["hbm_budget_gb"]},
     }
-    line.update({k: r[k] for k in ("p50_ms", "p99_ms", "offered_load", "saturated", "plain_fp64",
-                                   "speedup_vs_plain_fp64", "roofline", "cpu_baseline")})
+    line.update({k: r[k] for k in ("p50_ms", "p99_ms", "offered_load", "saturated", "light_load",
+                                   "plain_fp64", "speedup_vs_plain_fp64", "roofline",
+                                   "cpu_baseline")})
     print(json.dumps(line), flush=True)
 
 
@@ -523,6 +534,10 @@ def main():
     ap.add_argument("--hbm-budget", type=int, default=200, help="serving: GiB for admitted solves")
     ap.add_argument("--saturate", action="store_true",
                     help="serving: also run with every request queued at once (capacity)")
+    ap.add_argument("--light-requests", type=int, default=512,
+                    help="serving: requests in the light-load latency run (0 skips it)")
+    ap.add_argument("--light-rate", type=float, default=40.0,
+                    help="serving: arrivals per second of the light-load run")
     ap.add_argument("--plain-only", action="store_true",
                     help="serving: only the plain FP64 path (one request at a time)")
     ap.add_argument("--plain-requests", type=int, default=256,
