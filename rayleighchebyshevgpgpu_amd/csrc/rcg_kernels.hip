@@ -338,10 +338,30 @@ __device__ __forceinline__ d4v mfma16x16x4(double a, double b, d4v c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Which tile column lane cc of a 16x16 subtile t holds.  With an even number
+// of subtiles the columns are taken in pairs: lane cc reads columns 2cc and
+// 2cc+1 of each 32-column half with one 16-byte load, subtile 2h holding the
+// even and 2h+1 the odd ones; otherwise subtile t holds columns 16t..16t+15.
+// The output tile is the same sums with its rows and columns permuted.
+template <int NT>
+__device__ __forceinline__ int gram_col(int t, int cc) {
+  if constexpr (NT % 2 == 0) return 32 * (t >> 1) + 2 * cc + (t & 1);
+  return 16 * t + cc;
+}
+// the subtile that holds tile column c (the inverse of gram_col)
+template <int NT>
+__device__ __forceinline__ int gram_sub(int c) {
+  if constexpr (NT % 2 == 0) return 2 * (c >> 5) + (c & 1);
+  return c >> 4;
+}
+
 // Partial X^T Y of one 64x64 output tile over a row range.  Wave w takes the
-// 4-row groups i0 + 4w + 16t; lane (kr = l/16, cc = l%16) loads
-// X[row+kr][a0+16p+cc] and Y[row+kr][b0+16q+cc] -- 128-byte row segments --
-// which are exactly the A and B fragments of the 16 MFMA subtiles.  DIAG (a
+// 4-row groups i0 + 4w + 16t; lane (kr = l/16, cc = l%16) loads row row+kr of
+// X at the tile columns gram_col(p, cc) and of Y at gram_col(q, cc), which are
+// the A and B fragments of the NT x NT MFMA subtiles (16-byte loads of column
+// pairs when NT is even, 128-byte row segments across the 16 lanes either
+// way).  A wave's rows and the order it sums them in do not depend on the
+// column order, so every entry is the same sum as with 8-byte loads.  DIAG (a
 // tile on the diagonal of X^T X) multiplies only subtiles p <= q and needs
 // no Y loads.
 template <bool DIAG, int NT>
@@ -357,23 +377,40 @@ __device__ __forceinline__ void gram_tile_accumulate(d4v (&acc)[NT][NT], const d
   // solve 1975 -> 1989 ms.
   constexpr int U = NT == 3 ? 2 : 4;
   double xa[U][NT], yb[U][NT], xn[U][NT], yn[U][NT];
+  // column pairs are read with one 16-byte load where both columns exist and
+  // the rows are 16-byte aligned (a0, b0 are multiples of 64)
+  const bool vx = NT % 2 == 0 && ((reinterpret_cast<uintptr_t>(X) & 15) == 0) && (ldx % 2 == 0);
+  const bool vy = NT % 2 == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0) && (ldy % 2 == 0);
+  auto row_load = [&](const double* __restrict__ B, int ldb, int bw, int c0, bool vec, int64_t row,
+                      bool ok, double (&v)[NT]) {
+    if constexpr (NT % 2 == 0) {
+#pragma unroll
+      for (int h = 0; h < NT / 2; ++h) {
+        const int c = c0 + gram_col<NT>(2 * h, cc);
+        if (ok && vec && c + 1 < bw) {
+          const dvec2 t = *reinterpret_cast<const dvec2*>(B + row * ldb + c);
+          v[2 * h] = t.x;
+          v[2 * h + 1] = t.y;
+        } else {
+          v[2 * h] = (ok && c < bw) ? B[row * ldb + c] : 0.0;
+          v[2 * h + 1] = (ok && c + 1 < bw) ? B[row * ldb + c + 1] : 0.0;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int c = c0 + gram_col<NT>(t, cc);
+        v[t] = (ok && c < bw) ? B[row * ldb + c] : 0.0;
+      }
+    }
+  };
   auto load = [&](int64_t jb, double (&xv)[U][NT], double (&yv)[U][NT]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t row = jb + 16 * u + kr;
       const bool ok = row < i1;
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int c = a0 + 16 * t + cc;
-        xv[u][t] = (ok && c < bx) ? X[row * ldx + c] : 0.0;
-      }
-      if constexpr (!DIAG) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const int c = b0 + 16 * t + cc;
-          yv[u][t] = (ok && c < by) ? Y[row * ldy + c] : 0.0;
-        }
-      }
+      row_load(X, ldx, bx, a0, vx, row, ok, xv[u]);
+      if constexpr (!DIAG) row_load(Y, ldy, by, b0, vy, row, ok, yv[u]);
     }
   };
   if (ib < i1) load(ib, xa, yb);
@@ -470,7 +507,7 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
           if (diag && p > q) continue;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const int idx = (16 * p + 4 * j + kr) + kTile * (16 * q + cc);
+            const int idx = gram_col<NT>(p, 4 * j + kr) + kTile * gram_col<NT>(q, cc);
             red[idx] = (w == 0) ? acc[p][q][j] : red[idx] + acc[p][q][j];
           }
         }
@@ -485,8 +522,10 @@ __global__ __launch_bounds__(kThreads) void gram_mfma_kernel(
     const int a = a0 + al;
     const int b = b0 + bl;
     if (a >= bx || b >= by) continue;
+    // a diagonal tile holds the subtiles p <= q; the others are their mirror
     if (diag)
-      Pb[a + (int64_t)b * bx] = (al / 16 <= bl / 16) ? red[al + kTile * bl] : red[bl + kTile * al];
+      Pb[a + (int64_t)b * bx] = (gram_sub<NT>(al) <= gram_sub<NT>(bl)) ? red[al + kTile * bl]
+                                                                        : red[bl + kTile * al];
     else
       Pb[a + (int64_t)b * bx] = red[al + kTile * bl];
   }
