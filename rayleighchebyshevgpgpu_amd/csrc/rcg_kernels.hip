@@ -776,7 +776,11 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_mfma_kernel(
 // (64 KCH x 64) is staged in LDS once; the workgroup then walks 128-row tiles
 // with no further barrier.  Each wave owns 32 rows (two 16-row MFMA groups),
 // so every B fragment read from LDS feeds two MFMAs, and the next 64-wide k
-// chunk of its X rows is loaded while the current one is multiplied.
+// chunk of its X rows -- in the last chunk, the next tile's first -- is loaded
+// while the current one is multiplied.  Whether every row and column of a
+// wave's chunk exists is decided once per wave (a scalar branch): loads that
+// lanes take or skip on their own would make the wave wait for the loads in
+// flight where the paths join, which takes the prefetch's overlap away.
 template <int KCH>
 __global__ __launch_bounds__(kThreads) void tall_gemm_wide_kernel(
     int64_t n, const double* __restrict__ X, int ldx, int bx, const double* __restrict__ C,
@@ -799,55 +803,87 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_wide_kernel(
     cs[k][c] = (k < bx && c0 + c < by) ? C[k + (int64_t)(c0 + c) * ldc] : 0.0;
   }
   __syncthreads();
-  // lane (kk, r) of group g holds X[row_g][k0 + 16kk + q], q < 16
+  // lane (kk, r) of group g holds X[row_g][k0 + 16kk + q], q < 16; r0 and k0
+  // are the same for the whole wave
   auto load = [&](int64_t r0, int k0, double (&xv)[2][16]) {
     const int kb = k0 + 16 * kk;
+    const bool full = __builtin_amdgcn_readfirstlane(
+                          (int)(vec_ok && r0 + 32 <= n && k0 + kTile <= bx)) != 0;
+    if (full) {
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      const int64_t row = r0 + 16 * g + r;
-      if (row < n && kb + 16 <= bx && vec_ok) {
-        const double* xp = X + row * ldx + kb;
+      for (int g = 0; g < 2; ++g) {
+        const double* xp = X + (r0 + 16 * g + r) * ldx + kb;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           const dvec2 t = *reinterpret_cast<const dvec2*>(xp + 2 * q);
           xv[g][2 * q] = t.x;
           xv[g][2 * q + 1] = t.y;
         }
-      } else {
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int64_t row = r0 + 16 * g + r;
 #pragma unroll
         for (int q = 0; q < 16; ++q)
           xv[g][q] = (row < n && kb + q < bx) ? X[row * ldx + kb + q] : 0.0;
       }
     }
   };
+  // xn receives every load and xa is only ever written from it by copies: a
+  // register that entered the loops with a load in flight would make every
+  // chunk wait for the loads issued ahead of its MFMAs
+  double xa[2][16], xn[2][16];
+  if (stripe < ntiles) load(stripe * (2 * kTile) + wave * 32, 0, xn);
   for (int64_t tile = stripe; tile < ntiles; tile += nstripe) {
     const int64_t r0 = tile * (2 * kTile) + wave * 32;
+    const bool next = tile + nstripe < ntiles;
+    // every output row and column of the wave's tile exists (Z reads and the
+    // stores then take no per-lane branches)
+    const bool ofull =
+        __builtin_amdgcn_readfirstlane((int)(r0 + 32 <= n && c0 + kTile <= by)) != 0;
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) xa[g][q] = xn[g][q];
     d4v acc[2][4];
 #pragma unroll
     for (int g = 0; g < 2; ++g)
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[g][t] = d4v{0.0, 0.0, 0.0, 0.0};
-    double xa[2][16], xn[2][16];
     // beta Z of the epilogue, read at the start of the last chunk: Z may be
     // `out` itself, so loads issued between the stores would each wait for
     // the one before
     double zv[2][4][4];
-    load(r0, 0, xa);
 #pragma unroll 1
     for (int ci = 0; ci < KCH; ++ci) {
-      if (ci + 1 < KCH) load(r0, (ci + 1) * kTile, xn);
+      // one load site for both prefetches, so that they land in the same
+      // registers and need no copies (which would wait for them) to merge
+      if (ci + 1 < KCH || next)
+        load(ci + 1 < KCH ? r0 : r0 + nstripe * (2 * kTile), ci + 1 < KCH ? (ci + 1) * kTile : 0,
+             xn);
       if (ci + 1 == KCH && beta != 0.0) {
+        if (ofull) {
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
+          for (int g = 0; g < 2; ++g)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int64_t ro = r0 + 16 * g + 4 * j + kk;
+            for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const int c = c0 + 16 * t + r;
-              zv[g][j][t] = (ro < n && c < by) ? Z[ro * ldo + c] : 0.0;
+              for (int t = 0; t < 4; ++t)
+                zv[g][j][t] = Z[(r0 + 16 * g + 4 * j + kk) * ldo + c0 + 16 * t + r];
+        } else {
+#pragma unroll
+          for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int64_t ro = r0 + 16 * g + 4 * j + kk;
+#pragma unroll
+              for (int t = 0; t < 4; ++t) {
+                const int c = c0 + 16 * t + r;
+                zv[g][j][t] = (ro < n && c < by) ? Z[ro * ldo + c] : 0.0;
+              }
             }
-          }
+        }
       }
 #pragma unroll
       for (int sidx = 0; sidx < 16; ++sidx) {
@@ -867,21 +903,34 @@ __global__ __launch_bounds__(kThreads) void tall_gemm_wide_kernel(
           for (int q = 0; q < 16; ++q) xa[g][q] = xn[g][q];
       }
     }
+    if (ofull) {
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+      for (int g = 0; g < 2; ++g)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t ro = r0 + 16 * g + 4 * j + kk;
-        if (ro >= n) continue;
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int c = c0 + 16 * t + r;
-          if (c >= by) continue;
-          double v = alpha * acc[g][t][j];
-          if (beta != 0.0) v = fma(beta, zv[g][j][t], v);
-          out[ro * ldo + c] = v;
+          for (int t = 0; t < 4; ++t) {
+            double v = alpha * acc[g][t][j];
+            if (beta != 0.0) v = fma(beta, zv[g][j][t], v);
+            out[(r0 + 16 * g + 4 * j + kk) * ldo + c0 + 16 * t + r] = v;
+          }
+    } else {
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int64_t ro = r0 + 16 * g + 4 * j + kk;
+          if (ro >= n) continue;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int c = c0 + 16 * t + r;
+            if (c >= by) continue;
+            double v = alpha * acc[g][t][j];
+            if (beta != 0.0) v = fma(beta, zv[g][j][t], v);
+            out[ro * ldo + c] = v;
+          }
         }
-      }
+    }
   }
 }
 
