@@ -1026,9 +1026,35 @@ __global__ __launch_bounds__(kThreads, 2) void dense_step_kernel(
   const bool a_vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % W == 0);
   const bool x_vec = ((reinterpret_cast<uintptr_t>(X) & 15) == 0) && (ld % W == 0) &&
                      (c0 + xc + XE <= ncols);
+  // every row of the tile and every column of its X slice exist and both
+  // operands are 16-byte aligned: whole chunks are then loaded with no per-lane
+  // guard (decided once per wave: loads that lanes take or skip on their own
+  // make the wave wait, where the paths join, for the loads still in flight,
+  // which would undo the prefetch)
+  const bool tile_ok = a_vec && i0 + BM <= n && ((reinterpret_cast<uintptr_t>(X) & 15) == 0) &&
+                       (ld % W == 0) && c0 + BN <= ncols;
   T ra2[2][AE], rx2[2][XE];
   auto fetch_to = [&](int64_t k0, T (&ra)[AE], T (&rx)[XE]) {
     const int64_t ka = k0 + ak;
+    if (__builtin_amdgcn_readfirstlane((int)(tile_ok && k0 + BK <= n)) != 0) {
+      const T* ap = A + ai * lda + ka;
+      const T* xp = X + (k0 + xk) * ld + c0 + xc;
+#pragma unroll
+      for (int q = 0; q < AE / W; ++q) {
+        T t[W];
+        VT::ld(ap + q * W, t);
+#pragma unroll
+        for (int w = 0; w < W; ++w) ra[q * W + w] = t[w];
+      }
+#pragma unroll
+      for (int q = 0; q < XE / W; ++q) {
+        T t[W];
+        VT::ld(xp + q * W, t);
+#pragma unroll
+        for (int w = 0; w < W; ++w) rx[q * W + w] = t[w];
+      }
+      return;
+    }
     if (a_vec && ai < n && ka + AE <= n) {
       const T* ap = A + ai * lda + ka;
 #pragma unroll
