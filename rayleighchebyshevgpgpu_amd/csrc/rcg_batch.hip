@@ -9,6 +9,22 @@ namespace {
 
 static_assert(kBatchMaxRows <= kMaxRpb, "batched row blocks must fit the LDS row-pointer stage");
 
+// Nonzeros per gather chunk of the batched step (the served operators' rows
+// hold 5 or 7), measured separately: FP64 3 (32 served operators, degree 24:
+// 8.39 ms per call; 2: 8.61, 4: 8.88), FP32 2 (the served stream's shared
+// launches, 1024 requests: 8.67 s of device time; 3: 9.07-9.20 s, 4:
+// 9.09-9.22 s).
+#ifndef RCG_BATCH_CHUNK_F64
+#define RCG_BATCH_CHUNK_F64 3
+#endif
+#ifndef RCG_BATCH_CHUNK_F32
+#define RCG_BATCH_CHUNK_F32 2
+#endif
+template <typename T>
+constexpr int batch_chunk() {
+  return std::is_same<T, float>::value ? RCG_BATCH_CHUNK_F32 : RCG_BATCH_CHUNK_F64;
+}
+
 // ---------------------------------------------------- batched filter step
 // Step j of the Chebyshev filters of many operators at once: task t is row
 // block tasks[t].blk of request tasks[t].req.  Requests hold their own CSR
@@ -37,19 +53,19 @@ __global__ __launch_bounds__(kThreads) void csr_batch_kernel(const BatchDesc* __
     if constexpr (SRC) {
       T* xin = (j % 2 == 0) ? b1 : b0;
       T* xz = (j % 2 == 0) ? b0 : b1;
-      csr_rows<T, G, NV, kChebSrc, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, d.tail, g1,
+      csr_rows<T, G, NV, kChebSrc, 1, CX, batch_chunk<T>()>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, d.tail, g1,
                                           g2, T(1), i0, i1, static_cast<const T*>(d.R),
                                           static_cast<const T*>(d.kap) + (int64_t)j * d.ld, s_rp,
                                           s_col, s_val, kNoStart);
     } else if (j == 1) {
-      csr_rows<T, G, NV, kChebFirst, 1, CX>(d.n, d.rowptr, d.col, val, b0, nullptr, b1, d.ld,
+      csr_rows<T, G, NV, kChebFirst, 1, CX, batch_chunk<T>()>(d.n, d.rowptr, d.col, val, b0, nullptr, b1, d.ld,
                                             d.ncv, d.tail, g1, g2, T(d.m == 1 ? 0.5 : 1.0), i0, i1,
                                             nullptr, nullptr, s_rp, s_col, s_val, kNoStart);
     } else {
       T* xin = (j % 2 == 0) ? b1 : b0;
       T* xz = (j % 2 == 0) ? b0 : b1;
       const T sc = (j == d.m) ? T(1.0 / double(d.m + 1)) : T(1);
-      csr_rows<T, G, NV, kCheb, 1, CX>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, d.tail, g1, g2,
+      csr_rows<T, G, NV, kCheb, 1, CX, batch_chunk<T>()>(d.n, d.rowptr, d.col, val, xin, xz, xz, d.ld, d.ncv, d.tail, g1, g2,
                                        sc, i0, i1, nullptr, nullptr, s_rp, s_col, s_val, kNoStart);
     }
     __syncthreads();  // the LDS stages are reused by the next task

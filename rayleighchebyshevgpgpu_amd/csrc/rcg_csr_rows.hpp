@@ -152,11 +152,25 @@ __device__ __forceinline__ void mixed_start_elem(double v, double w, double thet
 // Rows per workgroup block (cap) and nonzeros staged in LDS per block.
 constexpr int kMaxRpb = 512;
 constexpr int kStageNz = 1024;
-// Nonzeros of a row whose gathers are issued together before their FMAs.
+// Nonzeros of a row whose gathers are issued together before their FMAs, by
+// kernel (a template argument of csr_rows; the FMAs run in the same order for
+// every chunk size).  Measured at 128^3 x 64 and on 32 served operators:
+// single-operator FP64 step 4 (830 us; 3: 841, 2: 856, 8: +3%), real FP32
+// correction step 2 (512 us; 3: 523, 4: 537), batched FP64 step 3 (8.39 ms per
+// 24-step call; 2: 8.61, 4: 8.88, 8: 11.5).  More gathers in flight cost the
+// address unit more than the trips to memory they save.
 #ifndef RCG_CSR_CHUNK
 #define RCG_CSR_CHUNK 4
 #endif
+#ifndef RCG_CSR_CHUNK_F32
+#define RCG_CSR_CHUNK_F32 2
+#endif
 constexpr int kChunk = RCG_CSR_CHUNK;
+// the single-operator step's chunk: real FP32 steps take fewer nonzeros
+template <typename T, bool CX>
+constexpr int csr_chunk() {
+  return (std::is_same<T, float>::value && !CX) ? RCG_CSR_CHUNK_F32 : kChunk;
+}
 
 // G lanes own one row; each lane handles NV 16-byte column vectors at
 // cv = g + k*G.  256/G rows are in flight per workgroup.  The block's row
@@ -169,7 +183,7 @@ constexpr int kChunk = RCG_CSR_CHUNK;
 // the call; the rest of that vector is read but never written.
 // Rows [i0, i1) of one step (the body shared by the single-operator and the
 // batched kernels); s_rp / s_col / s_val are the workgroup's LDS stages.
-template <typename T, int G, int NV, int MODE, int RPI, bool CX>
+template <typename T, int G, int NV, int MODE, int RPI, bool CX, int CH>
 __device__ __forceinline__ void csr_rows(
     int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const T* __restrict__ X, const T* Z, T* out, int ld, int ncv,
@@ -269,15 +283,15 @@ __device__ __forceinline__ void csr_rows(
     int maxlen = 0;
 #pragma unroll
     for (int q = 0; q < RPI; ++q) maxlen = max(maxlen, pe[q] - pb[q]);
-    // chunks of kChunk nonzeros per row: every gather of a chunk (over the
+    // chunks of CH nonzeros per row: every gather of a chunk (over the
     // RPI rows) is issued before its FMAs
-    for (int c = 0; c < maxlen; c += kChunk) {
-      int jj[RPI][kChunk];
-      T aa[RPI][kChunk], ai[RPI][kChunk];
+    for (int c = 0; c < maxlen; c += CH) {
+      int jj[RPI][CH];
+      T aa[RPI][CH], ai[RPI][CH];
 #pragma unroll
       for (int q = 0; q < RPI; ++q)
 #pragma unroll
-        for (int e = 0; e < kChunk; ++e) {
+        for (int e = 0; e < CH; ++e) {
           const int p = pb[q] + c + e;
           const bool live = p < pe[q];
           if constexpr (CX) {
@@ -300,11 +314,11 @@ __device__ __forceinline__ void csr_rows(
             }
           }
         }
-      T u[RPI][kChunk][NV][W];
+      T u[RPI][CH][NV][W];
 #pragma unroll
       for (int q = 0; q < RPI; ++q)
 #pragma unroll
-        for (int e = 0; e < kChunk; ++e) {
+        for (int e = 0; e < CH; ++e) {
           if (pb[q] + c + e < pe[q]) {
             const T* xr = X + (int64_t)jj[q][e] * ld;
 #pragma unroll
@@ -317,7 +331,7 @@ __device__ __forceinline__ void csr_rows(
 #pragma unroll
       for (int q = 0; q < RPI; ++q)
 #pragma unroll
-        for (int e = 0; e < kChunk; ++e) {
+        for (int e = 0; e < CH; ++e) {
           if (pb[q] + c + e < pe[q]) {
 #pragma unroll
             for (int k = 0; k < NV; ++k) {

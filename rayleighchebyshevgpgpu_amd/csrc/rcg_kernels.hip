@@ -37,8 +37,9 @@ __global__ __launch_bounds__(kThreads) void csr_block_kernel(
   const int64_t lb = order ? (int64_t)order[lq] : lq;
   const int64_t i0 = lb * rows_per_blk;
   const int64_t i1 = min(n, i0 + rows_per_blk);
-  csr_rows<T, G, NV, MODE, RPI, CX>(n, rowptr, col, val, X, Z, out, ld, ncv, tail, p0, p1, p2, i0,
-                                    i1, R, kap, s_rp, s_col, s_val, ms);
+  csr_rows<T, G, NV, MODE, RPI, CX, csr_chunk<T, CX>()>(n, rowptr, col, val, X, Z, out, ld, ncv,
+                                                       tail, p0, p1, p2, i0, i1, R, kap, s_rp,
+                                                       s_col, s_val, ms);
 }
 
 #ifndef RCG_CSR_RPI
