@@ -157,19 +157,28 @@ constexpr int kStageNz = 1024;
 // every chunk size).  Measured at 128^3 x 64 and on 32 served operators:
 // single-operator FP64 step 4 (830 us; 3: 841, 2: 856, 8: +3%), real FP32
 // correction step 2 (512 us; 3: 523, 4: 537), batched FP64 step 3 (8.39 ms per
-// 24-step call; 2: 8.61, 4: 8.88, 8: 11.5).  More gathers in flight cost the
-// address unit more than the trips to memory they save.
+// 24-step call; 2: 8.61, 4: 8.88, 8: 11.5); at 1024^2 x 96 complex columns,
+// complex128 step 3 (1043 us; 2: 1079, 4: 1066) and complex64 step 3 (708 us;
+// 2: 720, 4: 770).  More gathers in flight cost the address unit more than
+// the trips to memory they save.
 #ifndef RCG_CSR_CHUNK
 #define RCG_CSR_CHUNK 4
 #endif
 #ifndef RCG_CSR_CHUNK_F32
 #define RCG_CSR_CHUNK_F32 2
 #endif
+#ifndef RCG_CSR_CHUNK_C64
+#define RCG_CSR_CHUNK_C64 3
+#endif
+#ifndef RCG_CSR_CHUNK_C128
+#define RCG_CSR_CHUNK_C128 3
+#endif
 constexpr int kChunk = RCG_CSR_CHUNK;
-// the single-operator step's chunk: real FP32 steps take fewer nonzeros
+// the single-operator step's chunk by element type
 template <typename T, bool CX>
 constexpr int csr_chunk() {
-  return (std::is_same<T, float>::value && !CX) ? RCG_CSR_CHUNK_F32 : kChunk;
+  if (CX) return std::is_same<T, float>::value ? RCG_CSR_CHUNK_C64 : RCG_CSR_CHUNK_C128;
+  return std::is_same<T, float>::value ? RCG_CSR_CHUNK_F32 : kChunk;
 }
 
 // G lanes own one row; each lane handles NV 16-byte column vectors at
