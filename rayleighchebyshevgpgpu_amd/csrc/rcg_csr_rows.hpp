@@ -156,8 +156,9 @@ constexpr int kStageNz = 1024;
 // kernel (a template argument of csr_rows; the FMAs run in the same order for
 // every chunk size).  Measured at 128^3 x 64 and on 32 served operators:
 // single-operator FP64 step 4 (830 us; 3: 841, 2: 856, 8: +3%), real FP32
-// correction step 2 (512 us; 3: 523, 4: 537), batched FP64 step 3 (8.39 ms per
-// 24-step call; 2: 8.61, 4: 8.88, 8: 11.5); at 1024^2 x 96 complex columns,
+// correction step 2 (512 us; 1: 530, 3: 523, 4: 537), batched FP64 step 3
+// (8.39 ms per 24-step call; 2: 8.61, 4: 8.88, 8: 11.5); at 1024^2 x 96 complex
+// columns,
 // complex128 step 3 (1043 us; 2: 1079, 4: 1066) and complex64 step 3 (708 us;
 // 2: 720, 4: 770).  More gathers in flight cost the address unit more than
 // the trips to memory they save.
